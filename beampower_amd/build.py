@@ -1,0 +1,55 @@
+"""Build recipe of the gfx950 library beampower_amd/lib/beamform_gpu.so.
+
+    python -m beampower_amd.build
+
+Compiles the HIP sources under beampower_amd/csrc with hipcc for gfx950
+only.  Packed-FP32 instruction selection is disabled (-packed-fp32-ops).
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+REPO_DIR = os.path.dirname(PKG_DIR)
+CSRC = os.path.join(PKG_DIR, "csrc")
+LIB_DIR = os.path.join(PKG_DIR, "lib")
+LIB_PATH = os.path.join(LIB_DIR, "beamform_gpu.so")
+SOURCES = [os.path.join(CSRC, "beamform_gpu.hip")]
+HEADERS = [os.path.join(CSRC, f) for f in ("bp_device.h", "bp_kernels.h")] + [
+    os.path.join(REPO_DIR, "include", "beamform_gpu.h")]
+
+
+def hipcc():
+    path = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(path):
+        raise RuntimeError("hipcc not found: the gfx950 library cannot be built")
+    return path
+
+
+def command(out=LIB_PATH, extra=()):
+    return [
+        hipcc(), "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+        "-Wall", "-Wno-unused-function",
+        "-I", os.path.join(REPO_DIR, "include"), "-I", CSRC,
+        *extra, *SOURCES, "-o", out,
+    ]
+
+
+def build(verbose=False):
+    os.makedirs(LIB_DIR, exist_ok=True)
+    tmp = LIB_PATH + ".tmp"
+    cmd = command(tmp)
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("hipcc failed building beamform_gpu.so")
+    os.replace(tmp, LIB_PATH)
+    return LIB_PATH
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

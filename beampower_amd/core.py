@@ -1,0 +1,125 @@
+# coding: utf-8
+"""ctypes loader of the beamforming libraries.
+
+Mirrors beampower/core.py:7-169 of the reference: the same LIBRARIES table
+layout, the same load-once cache and the same errors (NameError for an
+unknown device, RuntimeError when the GPU library is missing, OSError when
+the CPU library is missing).  The "gpu" entry is this package's gfx950
+library, built from beampower_amd/csrc; it additionally exports the GPU
+restatement of prestack_waveform_features and the bp_* extension API, and
+every entry point returns an int status (0 = success).  The "cpu" entry
+points at lib/beamform_cpu.so, which this package does not build: it is the
+reference's own CPU engine, loaded only if the user places it there.
+"""
+import ctypes as ct
+import os
+
+DIRPATH_LIBRARIES = os.path.join(os.path.dirname(__file__), "lib")
+
+_P_F = ct.POINTER(ct.c_float)
+_P_I = ct.POINTER(ct.c_int)
+_SZ = ct.c_size_t
+
+LIBRARIES = {
+    "cpu": {
+        "filepath_library": os.path.join(DIRPATH_LIBRARIES, "beamform_cpu.so"),
+        "is_loaded": False,
+        "lib": None,
+        # core.py:14-49 of the reference
+        "beamform_argtypes": [_P_F, _P_I, _P_F, _SZ, _SZ, _SZ, _SZ, ct.c_int, ct.c_int, _P_F],
+        "beamform_differential_argtypes": [_P_F, _P_I, _P_F, _SZ, _SZ, _SZ, _SZ, ct.c_int,
+                                           _P_F],
+        "beamform_max_argtypes": [_P_F, _P_I, _P_F, _SZ, _SZ, _SZ, _SZ, ct.c_int, ct.c_int,
+                                  _P_F, _P_I],
+        "prestack_waveform_features_argtypes": [_P_F, _P_F, _SZ, _SZ, _SZ, _SZ, ct.c_int, _P_F],
+    },
+    "gpu": {
+        "filepath_library": os.path.join(DIRPATH_LIBRARIES, "beamform_gpu.so"),
+        "is_loaded": False,
+        "lib": None,
+        # core.py:65-87 of the reference
+        "beamform_argtypes": [_P_F, _P_I, _P_F, _SZ, _SZ, _SZ, _SZ, ct.c_int, _P_F],
+        "beamform_max_argtypes": [_P_F, _P_I, _P_F, _SZ, _SZ, _SZ, _SZ, ct.c_int, _P_F, _P_I],
+        # the CPU library's prestack signature (core.py:40-49), on the GPU
+        "prestack_waveform_features_argtypes": [_P_F, _P_F, _SZ, _SZ, _SZ, _SZ, ct.c_int, _P_F],
+    },
+}
+
+# Symbols the gpu library declares in include/beamform_gpu.h.
+GPU_SYMBOLS = (
+    "beamform", "beamform_max", "prestack_waveform_features",
+    "bp_last_error", "bp_version",
+    "bp_network_create", "bp_network_destroy", "bp_network_info",
+    "bp_network_beamform_max", "bp_network_beamform",
+    "bp_interleaved_to_planar", "bp_prestack_planar",
+    "bp_profiling", "bp_profiling_read",
+)
+
+
+def _declare_gpu_extension(lib):
+    vp = ct.c_void_p
+    lib.bp_last_error.restype = ct.c_char_p
+    lib.bp_last_error.argtypes = []
+    lib.bp_version.restype = ct.c_char_p
+    lib.bp_version.argtypes = []
+    lib.bp_network_create.argtypes = [vp, vp, _SZ, _SZ, _SZ, ct.POINTER(vp)]
+    lib.bp_network_destroy.argtypes = [vp]
+    lib.bp_network_info.argtypes = [vp, _P_I, _P_I, _P_I]
+    lib.bp_network_beamform_max.argtypes = [vp, vp, ct.c_int, vp, _SZ, ct.c_int, vp, vp, vp,
+                                            vp]
+    lib.bp_network_beamform.argtypes = [vp, vp, _SZ, ct.c_int, _SZ, _SZ, vp, vp]
+    lib.bp_interleaved_to_planar.argtypes = [vp, _SZ, _SZ, _SZ, vp, vp]
+    lib.bp_prestack_planar.argtypes = [vp, vp, _SZ, _SZ, _SZ, _SZ, vp, vp]
+    lib.bp_profiling.argtypes = [ct.c_int]
+    lib.bp_profiling_read.argtypes = [ct.POINTER(ct.c_double), ct.POINTER(ct.c_longlong)]
+    for name in GPU_SYMBOLS:
+        if name not in ("bp_last_error", "bp_version"):
+            getattr(lib, name).restype = ct.c_int
+
+
+def load_library(device="cpu"):
+    """Load the library for `device` ("cpu" or "gpu") once and cache it.
+
+    Raises NameError for an unknown device, RuntimeError if the GPU library
+    is missing and OSError if the CPU library is missing
+    (beampower/core.py:92-169)."""
+    device_name = device.lower()
+    if device_name not in LIBRARIES:
+        raise NameError(f"Device should be cpu or gpu, not {device_name}")
+    info = LIBRARIES[device_name]
+    path = info["filepath_library"]
+    if os.path.exists(path):
+        if info["is_loaded"] is True:
+            return info["lib"]
+        lib = ct.cdll.LoadLibrary(path)
+        lib.beamform.argtypes = info["beamform_argtypes"]
+        lib.beamform_max.argtypes = info["beamform_max_argtypes"]
+        lib.prestack_waveform_features.argtypes = info["prestack_waveform_features_argtypes"]
+        if device_name == "cpu":
+            lib.beamform_differential.argtypes = info["beamform_differential_argtypes"]
+        else:
+            for name in ("beamform", "beamform_max", "prestack_waveform_features"):
+                getattr(lib, name).restype = ct.c_int
+            _declare_gpu_extension(lib)
+        info["is_loaded"] = True
+        info["lib"] = lib
+        return lib
+    if device_name == "gpu":
+        raise RuntimeError(
+            f"GPU library not available. The shared object {path} does not exist.\n"
+            "Build it with `python -m beampower_amd.build` (hipcc, gfx950).")
+    raise OSError(
+        f"CPU library is required but not found at {path}.\n"
+        "This package ships the gfx950 engine only; place the reference's "
+        "beamform_cpu.so there to use device='cpu'.")
+
+
+def last_error(lib=None):
+    lib = lib or load_library("gpu")
+    return lib.bp_last_error().decode(errors="replace")
+
+
+def check(status, what, lib=None):
+    """Raise RuntimeError for a non-zero status of a gpu-library call."""
+    if status != 0:
+        raise RuntimeError(f"{what} failed with status {status}: {last_error(lib)}")

@@ -1,0 +1,143 @@
+"""Device-resident front-end of the gfx950 library (no reference counterpart).
+
+The reference copies its inputs to the GPU on every call
+(beamform.cu:314-340).  Here a `Network` uploads moveouts and source weights
+once (`bp_network_create`) and then beamforms device-resident features.
+PyTorch provides device memory and streams only; every computation runs in
+the library's HIP kernels.
+"""
+import ctypes as ct
+
+import numpy as np
+
+from .core import check, load_library
+
+BP_F32, BP_BF16, BP_F16 = 0, 1, 2
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("beampower_amd.device needs a ROCm GPU")
+    return torch
+
+
+def _stream_ptr(stream):
+    torch = _torch()
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return ct.c_void_p(stream.cuda_stream)
+
+
+class Network:
+    """Moveouts (K,S,P) int32 and source weights (K,S) float32 resident on the
+    current device, with per-source and per-source-tile delay bounds."""
+
+    def __init__(self, time_delays, weights_sources):
+        self.lib = load_library("gpu")
+        tau = np.ascontiguousarray(time_delays, dtype=np.int32)
+        beta = np.ascontiguousarray(weights_sources, dtype=np.float32)
+        if tau.ndim != 3 or beta.shape != tau.shape[:2]:
+            raise ValueError(f"shapes {tau.shape} / {beta.shape} do not match (K,S,P) / (K,S)")
+        self.n_sources, self.n_stations, self.n_phases = tau.shape
+        handle = ct.c_void_p()
+        check(self.lib.bp_network_create(tau.ctypes.data, beta.ctypes.data, self.n_sources,
+                                         self.n_stations, self.n_phases, ct.byref(handle)),
+              "bp_network_create", self.lib)
+        self._h = handle
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.bp_network_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def info(self):
+        tile, spread, n = ct.c_int(), ct.c_int(), ct.c_int()
+        check(self.lib.bp_network_info(self._h, ct.byref(tile), ct.byref(spread),
+                                       ct.byref(n)), "bp_network_info", self.lib)
+        return dict(source_tile=tile.value, max_tile_spread=spread.value, n_tiles=n.value)
+
+    def beamform_max(self, features, out_of_bounds=0, beam_max=None, beam_argmax=None,
+                     keys=None, stream=None):
+        """features: planar (S,P,T) float32 CUDA tensor.  Returns device
+        tensors (beam_max (T,), beam_argmax (T,)); asynchronous."""
+        torch = _torch()
+        S, P, T = features.shape
+        if (S, P) != (self.n_stations, self.n_phases) or features.dtype != torch.float32:
+            raise ValueError("features must be float32 (n_stations, n_phases, n_samples)")
+        features = features.contiguous()
+        dev = features.device
+        if beam_max is None:
+            beam_max = torch.zeros(T, dtype=torch.float32, device=dev)
+        if beam_argmax is None:
+            beam_argmax = torch.empty(T, dtype=torch.int32, device=dev)
+        if keys is None:
+            keys = torch.empty(T, dtype=torch.int64, device=dev)
+        check(self.lib.bp_network_beamform_max(
+            self._h, features.data_ptr(), BP_F32, None, T, int(out_of_bounds),
+            beam_max.data_ptr(), beam_argmax.data_ptr(), keys.data_ptr(),
+            _stream_ptr(stream)), "bp_network_beamform_max", self.lib)
+        return beam_max, beam_argmax
+
+    def beamform(self, features, out_of_bounds=0, source_begin=0, source_end=None,
+                 beam=None, stream=None):
+        """Full beams of sources [source_begin, source_end) as a (n, T)
+        float32 CUDA tensor (cells invalid in strict mode stay 0)."""
+        torch = _torch()
+        S, P, T = features.shape
+        source_end = self.n_sources if source_end is None else int(source_end)
+        n = source_end - int(source_begin)
+        if beam is None:
+            beam = torch.zeros((n, T), dtype=torch.float32, device=features.device)
+        check(self.lib.bp_network_beamform(
+            self._h, features.contiguous().data_ptr(), T, int(out_of_bounds),
+            int(source_begin), source_end, beam.data_ptr(), _stream_ptr(stream)),
+            "bp_network_beamform", self.lib)
+        return beam
+
+
+def interleaved_to_planar(F_stp, stream=None):
+    """(S,T,P) float32 CUDA tensor -> (S,P,T)."""
+    torch = _torch()
+    lib = load_library("gpu")
+    S, T, P = F_stp.shape
+    out = torch.empty((S, P, T), dtype=torch.float32, device=F_stp.device)
+    check(lib.bp_interleaved_to_planar(F_stp.contiguous().data_ptr(), S, T, P,
+                                       out.data_ptr(), _stream_ptr(stream)),
+          "bp_interleaved_to_planar", lib)
+    return out
+
+
+def prestack_planar(u, weights_phases, stream=None):
+    """(S,C,T) features and (S,C,P) weights (CUDA float32) -> (S,P,T)."""
+    torch = _torch()
+    lib = load_library("gpu")
+    S, C, T = u.shape
+    P = weights_phases.shape[2]
+    out = torch.empty((S, P, T), dtype=torch.float32, device=u.device)
+    check(lib.bp_prestack_planar(u.contiguous().data_ptr(), weights_phases.contiguous().data_ptr(),
+                                 S, C, T, P, out.data_ptr(), _stream_ptr(stream)),
+          "bp_prestack_planar", lib)
+    return out
+
+
+class KernelTimer:
+    """Accumulates the HIP-event time of the library's stack kernel."""
+
+    def __init__(self):
+        self.lib = load_library("gpu")
+
+    def __enter__(self):
+        self.lib.bp_profiling_read(None, None)
+        self.lib.bp_profiling(1)
+        return self
+
+    def __exit__(self, *exc):
+        ms, n = ct.c_double(), ct.c_longlong()
+        self.lib.bp_profiling(0)
+        self.lib.bp_profiling_read(ct.byref(ms), ct.byref(n))
+        self.total_ms, self.launches = ms.value, n.value
+        return False

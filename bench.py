@@ -1,0 +1,222 @@
+"""Throughput of the FP32 drop-in beamform(reduce='max') on one MI355X.
+
+Workload (default): 50 stations in a 100x100 km box, 80x80x30 source grid
+(192,000 sources, depth 0-20 km), homogeneous Vp=6.0 / Vs=3.5 km/s at 25 Hz,
+1 h of features (90,000 samples) with 50 injected events, strict bounds.
+One step is one beamform_max pass over all sources and samples with the
+moveouts, weights and prestacked features already resident in HBM.
+
+Prints one JSON line.  `value` is source*samples per second of the whole
+step (keys init + shift-and-stack + finalize); `roofline` prices the
+dominant shift-and-stack kernel by its algorithmic LDS gather bytes;
+`cpu_baseline` times the reference C engine (oracle/_ref, its own sources
+and flags) on this host on a bounded time slice of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = ("source·samples/s, FP32 drop-in vs BF16 features; "
+          "p50/p99 served-window latency")
+
+# gfx950 (MI355X) published figures
+N_CU = 256
+CLOCK_HZ = 2.4e9
+LDS_BYTES_PER_CLK_CU = 128
+LDS_PEAK = N_CU * LDS_BYTES_PER_CLK_CU * CLOCK_HZ  # 78.6 TB/s
+HBM_PEAK = 8.0e12
+VALU_FP32_PEAK_FLOPS = 157.3e12
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_threads():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(w, budget_s):
+    """Time the reference engine (or the oracle port) on a halo-padded time
+    slice with all sources.  A slice of width W + halo (halo = max moveout +
+    1) computes every source at the first W samples exactly as the full call
+    would; the rate counts the (source, sample) pairs the call evaluated."""
+    import numpy as np
+    from oracle import oracle
+
+    kind = "reference" if oracle.ref_available("cmake") else "port"
+    threads = cpu_threads()
+    tau, beta = w.time_delays, w.weights_sources
+    F = oracle.prestack(w.waveform_features, w.weights_phases, num_threads=threads)
+    lohi = oracle.delay_bounds(tau, beta, num_threads=threads)
+    halo = int(lohi[:, 1].max()) + 1
+    start = 1000
+
+    def run(W):
+        Tp = W + halo
+        Fs = np.ascontiguousarray(F[:, start:start + Tp, :])
+        t0 = time.perf_counter()
+        if kind == "reference":
+            oracle.ref_beamform_max(Fs, tau, beta, 0, num_threads=threads, flavour="cmake")
+        else:
+            oracle.beamform_max(Fs, tau, beta, 0, num_threads=threads, flavour="fma")
+        dt = time.perf_counter() - t0
+        lo, hi = lohi[:, 0].astype(np.int64), lohi[:, 1].astype(np.int64)
+        n_valid = np.clip(np.minimum(Tp, Tp - hi) - np.maximum(0, -lo), 0, None)
+        return dt, float(n_valid.sum()), Tp
+
+    dt, work, _ = run(16)
+    rate = work / dt
+    mean_hi = float(np.clip(lohi[:, 1], 0, None).mean())
+    W = int(budget_s * rate / tau.shape[0] + mean_hi - halo)
+    W = max(16, min(w.waveform_features.shape[2] - start - halo, W))
+    dt, work, Tp = run(W)
+    return {
+        "value": work / dt,
+        "unit": "source·samples/s",
+        "cores": threads,
+        "kind": kind,
+        "sample": (f"beamform_max strict, all {tau.shape[0]} sources, {Tp}-sample slice "
+                   f"({W} + {halo} halo) of the same features; {work:.3e} valid "
+                   f"(source, sample) pairs in {dt:.2f} s; "
+                   + ("reference beamform.c built by oracle/Makefile (CMake flags)"
+                      if kind == "reference" else "oracle port (fma flavour)")),
+    }
+
+
+def profile_traffic():
+    """HBM bytes per launch of the stack kernel from the committed rocprofv3
+    counter summary, if present (FETCH_SIZE + WRITE_SIZE, copy-calibrated)."""
+    path = os.path.join(REPO, "profiles", "stack_lds_dropin_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--oob", choices=["strict", "flexible"], default="strict")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build runs on a single MI355X "
+              f"(--gpus {args.gpus} is not supported)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from beampower_amd import device as bpd
+    from beampower_amd import synthetic
+
+    assert torch.cuda.is_available(), "bench.py needs the GPU"
+    torch.cuda.set_device(0)
+    oob = 0 if args.oob == "strict" else 1
+
+    log("generating workload")
+    w = synthetic.dropin_c2()
+    S, C, T = w.waveform_features.shape
+    K, _, P = w.time_delays.shape
+    u = torch.from_numpy(w.waveform_features).cuda()
+    alpha = torch.from_numpy(w.weights_phases).cuda()
+    F = bpd.prestack_planar(u, alpha)
+    del u
+    net = bpd.Network(w.time_delays, w.weights_sources)
+    info = net.info()
+    bmax = torch.zeros(T, dtype=torch.float32, device="cuda")
+    barg = torch.empty(T, dtype=torch.int32, device="cuda")
+    keys = torch.empty(T, dtype=torch.int64, device="cuda")
+    torch.cuda.synchronize()
+
+    log(f"warmup {args.warmup}")
+    for _ in range(args.warmup):
+        net.beamform_max(F, oob, bmax, barg, keys)
+    torch.cuda.synchronize()
+
+    log(f"timing {args.steps} steps")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        net.beamform_max(F, oob, bmax, barg, keys)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms_per_step = dt * 1e3 / args.steps
+    value = K * T / (dt / args.steps)
+
+    with bpd.KernelTimer() as kt:
+        for _ in range(args.steps):
+            net.beamform_max(F, oob, bmax, barg, keys)
+        torch.cuda.synchronize()
+    kernel_ms = kt.total_ms / max(1, kt.launches)
+    log(f"step {ms_per_step:.2f} ms, stack kernel {kernel_ms:.2f} ms")
+
+    # known answer on the benchmark output itself
+    a = barg.cpu().numpy()
+    recovered = sum(int(np.array_equal(w.time_delays[a[t_e]], w.time_delays[k_e]))
+                    for k_e, t_e in w.events)
+
+    used_terms = float((w.weights_sources != 0).sum()) * P  # (k, s, p) terms per sample
+    lds_bytes = used_terms * T * 4.0
+    fmas = used_terms * T
+    achieved = lds_bytes / (kernel_ms * 1e-3)
+    roofline = {
+        "bound": "lds",
+        "achieved": achieved / 1e9,
+        "peak": LDS_PEAK / 1e9,
+        "unit": "GB/s",
+        "frac": achieved / LDS_PEAK,
+        "traffic": profile_traffic(),
+        "kernel": "stack_lds_kernel",
+        "kernel_ms": kernel_ms,
+        "algorithmic_bytes_per_launch": lds_bytes,
+        "valu_frac": (2.0 * fmas / (kernel_ms * 1e-3)) / VALU_FP32_PEAK_FLOPS,
+        "hbm_compulsory_frac": ((S * P * T * 4 + K * S * P * 4 + K * S * 4 + 8 * T)
+                                / (kernel_ms * 1e-3)) / HBM_PEAK,
+    }
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline")
+        cpu = cpu_baseline(w, args.cpu_seconds)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "source·samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded homogeneous-velocity moveouts, envelope-like features)",
+        "config": {
+            "workload": w.name, "stations": S, "channels": C, "phases": P,
+            "sources": K, "samples": T, "reduce": "max", "out_of_bounds": args.oob,
+            "source_tile": info["source_tile"], "max_tile_spread": info["max_tile_spread"],
+        },
+        "events_recovered": f"{recovered}/{len(w.events)}",
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

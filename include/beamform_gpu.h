@@ -1,0 +1,125 @@
+/*
+ * beamform_gpu.h -- C ABI of beampower_amd/lib/beamform_gpu.so (gfx950).
+ *
+ * Drop-in entry points.  Their names, argument order, types and meaning are
+ * those the reference's Python layer binds for device="gpu":
+ *   reference prototypes  beampower/src/beamform_gpu.h:1-2
+ *   ctypes argtypes       beampower/core.py:65-87
+ *   call sites            beampower/beampower.py:156-190
+ * All pointers are HOST pointers (NumPy buffers); each call is synchronous.
+ * The reference returns void and its wrapper never reads the result (ctypes
+ * restype stays c_int), so returning an int status is ABI-compatible:
+ * 0 on success, a negative BP_E* code otherwise (never exit()).
+ * Calls are re-entrant: each call uses its own stream and buffers.
+ */
+#ifndef BEAMFORM_GPU_H
+#define BEAMFORM_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BP_OK 0
+#define BP_EINVAL (-22)   /* bad argument (shape, mode, null pointer) */
+#define BP_ENOMEM (-12)   /* device allocation failed */
+#define BP_EHIP (-5)      /* HIP runtime error */
+#define BP_ENODEV (-19)   /* no usable gfx950 device */
+
+/* Replaces beamform(...) of beamform.cu:462-659 (reduce='none').
+ * F: prestacked features (n_stations, n_samples, n_phases) float32
+ * time_delays: (n_sources, n_stations, n_phases) int32 moveouts in samples
+ * weights_sources: (n_sources, n_stations) float32
+ * out_of_bounds: 0 = strict, 1 = flexible
+ * beam: (n_sources, n_samples) float32 output; cells that are invalid in
+ *       strict mode are set to 0 (the wrapper passes a zero-filled buffer). */
+int beamform(float *waveform_features, int *time_delays, float *weights_sources,
+             size_t n_samples, size_t n_sources, size_t n_stations,
+             size_t n_phases, int out_of_bounds, float *beam);
+
+/* Replaces beamform_max(...) of beamform.cu:207-460 (reduce='max').
+ * beam_max: (n_samples) float32, written only where some source is valid
+ *           (the caller's value is kept elsewhere, beamform.c:332-333)
+ * beam_argmax: (n_samples) int32, lowest source index attaining the max,
+ *           0 where no source is valid (beamform.c:269-270,334). */
+int beamform_max(float *waveform_features, int *time_delays, float *weights_sources,
+                 size_t n_samples, size_t n_sources, size_t n_stations,
+                 size_t n_phases, int out_of_bounds, float *beam_max,
+                 int *beam_argmax);
+
+/* Replaces the CPU library's prestack_waveform_features (beamform_cpu.h:4,
+ * beamform.c:108-152; argtypes core.py:40-49), computed on the GPU.
+ * waveform_features: (n_stations, n_channels, n_samples) float32
+ * weights_phases: (n_stations, n_channels, n_phases) float32
+ * prestacked: (n_stations, n_samples, n_phases) float32 output.
+ * num_threads is accepted for signature compatibility and ignored. */
+int prestack_waveform_features(float *waveform_features, float *weights_phases,
+                               size_t n_samples, size_t n_stations,
+                               size_t n_channels, size_t n_phases,
+                               int num_threads, float *prestacked);
+
+/* ---- extension API (no counterpart in the reference) -------------------- */
+
+/* Thread-local message for the last non-zero status of this thread. */
+const char *bp_last_error(void);
+
+/* Library version string. */
+const char *bp_version(void);
+
+/* Feature storage formats for the device-resident entry points. */
+#define BP_F32 0
+#define BP_BF16 1
+#define BP_F16 2
+
+/* A network: moveouts and source weights resident on the device, with the
+ * per-source and per-source-tile delay bounds precomputed once.
+ * tau, beta are HOST pointers with the layouts of beamform_max. */
+typedef struct bp_network bp_network;
+int bp_network_create(const int *time_delays, const float *weights_sources,
+                      size_t n_sources, size_t n_stations, size_t n_phases,
+                      bp_network **out);
+int bp_network_destroy(bp_network *net);
+/* Largest per-(station, phase) delay spread inside one source tile. */
+int bp_network_info(const bp_network *net, int *source_tile, int *max_tile_spread,
+                    int *n_tiles);
+
+/* reduce='max' on device-resident planar features.
+ * features: DEVICE pointer, planar (n_stations, n_phases, n_samples) in the
+ *   format `fmt` (BP_F32 here; BP_BF16 / BP_F16 with per-station scales).
+ * station_scale: DEVICE pointer (n_stations) float32 multiplier applied to
+ *   the station's weights (NULL = 1), used by the narrow formats.
+ * beam_max, beam_argmax: DEVICE pointers (n_samples); beam_max keeps its
+ *   previous value where no source is valid.
+ * keys: DEVICE scratch of n_samples uint64 (caller-provided workspace).
+ * stream: hipStream_t (NULL = default stream).  Asynchronous. */
+int bp_network_beamform_max(bp_network *net, const void *features, int fmt,
+                            const float *station_scale, size_t n_samples,
+                            int out_of_bounds, float *beam_max, int *beam_argmax,
+                            unsigned long long *keys, void *stream);
+
+/* reduce='none' on device-resident planar f32 features for the sources
+ * [source_begin, source_end) of the network.  beam: DEVICE (n, n_samples);
+ * invalid cells are left untouched. */
+int bp_network_beamform(bp_network *net, const float *features, size_t n_samples,
+                        int out_of_bounds, size_t source_begin, size_t source_end,
+                        float *beam, void *stream);
+
+/* Layout / prestack kernels on device pointers (asynchronous on `stream`). */
+int bp_interleaved_to_planar(const float *F_stp, size_t n_stations, size_t n_samples,
+                             size_t n_phases, float *F_spt, void *stream);
+int bp_prestack_planar(const float *u_sct, const float *weights_phases,
+                       size_t n_stations, size_t n_channels, size_t n_samples,
+                       size_t n_phases, float *F_spt, void *stream);
+
+/* Kernel timing: when enabled, the dominant (stack) kernel of every
+ * bp_network_* call is bracketed with HIP events on its own stream and the
+ * elapsed milliseconds are accumulated per thread. */
+int bp_profiling(int enable);
+int bp_profiling_read(double *stack_ms_total, long long *stack_launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,125 @@
+"""Host-side checks that need no GPU: the C ABI library loads and exports
+every symbol its header declares, argument validation returns status codes
+before touching a device, the Python layer keeps the reference's error
+behaviour, and the delay-table helpers follow the tutorial's rules."""
+import ctypes as ct
+import os
+import re
+
+import numpy as np
+import pytest
+
+import beampower_amd as bp
+from beampower_amd import core, network, synthetic
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(REPO, "include", "beamform_gpu.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(\w+)\s*\(", text, flags=re.M)))
+
+
+def test_header_lists_expected_entry_points():
+    names = declared_functions()
+    assert set(core.GPU_SYMBOLS) == set(names), names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = bp.load_library("gpu")
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+    assert lib.bp_version().decode().count(".") == 2
+
+
+def test_library_is_gfx950_only():
+    out = os.popen(f"/opt/rocm/llvm/bin/clang-offload-bundler --list --type=o "
+                   f"--input={core.LIBRARIES['gpu']['filepath_library']} 2>/dev/null").read()
+    if not out:
+        pytest.skip("offload bundler listing unavailable")
+    targets = [t for t in out.split() if "amdgcn" in t]
+    assert targets and all("gfx950" in t for t in targets), targets
+
+
+def test_load_library_errors_mirror_reference():
+    with pytest.raises(NameError):
+        bp.load_library("tpu")
+    if not os.path.exists(core.LIBRARIES["cpu"]["filepath_library"]):
+        with pytest.raises(OSError):
+            bp.load_library("cpu")
+    assert bp.load_library("GPU") is bp.load_library("gpu")
+
+
+def test_bad_out_of_bounds_returns_none(capsys):
+    w = synthetic.random_case(0, S=2, C=3, P=2, T=16, K=3)
+    assert bp.beamform(w[0], w[2], w[1], w[3], device="gpu", out_of_bounds="loose") is None
+    assert "strict" in capsys.readouterr().out
+
+
+def test_bad_mode_returns_one(capsys):
+    w = synthetic.random_case(0, S=2, C=3, P=2, T=16, K=3)
+    assert bp.beamform(w[0], w[2], w[1], w[3], device="gpu", mode="sideways") == 1
+
+
+def test_differential_on_gpu_is_not_implemented(capsys):
+    w = synthetic.random_case(0, S=2, C=3, P=2, T=16, K=3)
+    assert bp.beamform(w[0], w[2], w[1], w[3], device="gpu", mode="differential") is None
+    assert "not yet implemented" in capsys.readouterr().out
+
+
+def test_c_abi_rejects_bad_arguments_without_a_device():
+    lib = bp.load_library("gpu")
+    F = np.zeros(8, np.float32)
+    tau = np.zeros(4, np.int32)
+    beta = np.ones(2, np.float32)
+    m = np.zeros(2, np.float32)
+    a = np.zeros(2, np.int32)
+    P_F, P_I = ct.POINTER(ct.c_float), ct.POINTER(ct.c_int)
+    rc = lib.beamform_max(F.ctypes.data_as(P_F), tau.ctypes.data_as(P_I),
+                          beta.ctypes.data_as(P_F), 2, 2, 1, 2, 7, m.ctypes.data_as(P_F),
+                          a.ctypes.data_as(P_I))
+    assert rc == -22 and "out_of_bounds" in core.last_error(lib)
+    rc = lib.beamform(F.ctypes.data_as(P_F), tau.ctypes.data_as(P_I), beta.ctypes.data_as(P_F),
+                      2, 2, 1, 2, -1, m.ctypes.data_as(P_F))
+    assert rc == -22
+    rc = lib.beamform_max(None, None, None, 1 << 40, 2, 1, 2, 0, None, None)
+    assert rc == -22 and "too large" in core.last_error(lib)
+    out = ct.c_void_p()
+    assert lib.bp_network_create(None, None, 0, 1, 1, ct.byref(out)) == -22
+
+
+def test_degenerate_shapes_follow_reference_semantics():
+    """No source or no station: strict keeps the caller's beam_max and
+    writes argmax 0; flexible gives every source a 0 beam (beamform.c)."""
+    lib = bp.load_library("gpu")
+    P_F, P_I = ct.POINTER(ct.c_float), ct.POINTER(ct.c_int)
+    for K, S, oob, expect in ((0, 3, 0, 7.0), (0, 3, 1, 7.0), (4, 0, 0, 7.0), (4, 0, 1, 0.0)):
+        m = np.full(5, 7.0, np.float32)
+        a = np.full(5, 9, np.int32)
+        rc = lib.beamform_max(None, None, None, 5, K, S, 2, oob, m.ctypes.data_as(P_F),
+                              a.ctypes.data_as(P_I))
+        assert rc == 0
+        assert np.all(m == expect) and np.all(a == 0)
+
+
+def test_sec_to_samp_follows_tutorial():
+    # notebooks/3_beampower.ipynb:380-392: sign(t) * int(|t*sr| + 0.2)
+    t = np.array([0.0, 0.03, 0.039, -0.039, 1.0, 2.99, -2.99, 0.0319])
+    got = network.sec_to_samp(t, 25.0)
+    expect = np.array([int(np.sign(x)) * int(abs(x * 25.0) + 0.2) for x in t])
+    assert got.dtype == np.int32 and np.array_equal(got, expect)
+
+
+def test_relative_moveouts_and_plain_geometry():
+    w = synthetic.plain_c1()
+    tau = w.time_delays
+    assert tau.shape == (4000, 20, 2) and tau.dtype == np.int32
+    assert np.all(tau.min(axis=(1, 2)) == 0)
+    # S slower than P at every station
+    assert np.all(tau[:, :, 1] >= tau[:, :, 0])
+    # geometric upper bound: box diagonal + depth at Vs, at 25 Hz
+    assert tau.max() <= int(np.sqrt(50 ** 2 * 2 + 20 ** 2) / 3.5 * 25) + 1
+    assert w.waveform_features.shape == (20, 3, 6000)
+    assert w.weights_phases[:, 0, 0].min() == 1 and w.weights_phases[:, 1:, 1].max() == 0.5

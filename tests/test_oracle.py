@@ -1,0 +1,196 @@
+"""Pins the CPU oracle (oracle/beamform_oracle.c).
+
+* against the golden fixtures written by the reference engine compiled from
+  its own sources (tests/golden/make_golden.py), bit for bit, per build
+  flavour;
+* against the reference engine itself when oracle/_ref is built;
+* against an independent NumPy FP64 brute force on tiny inputs, with
+  planted mistakes that the checks must catch;
+* against closed forms (impulse and constant features, exact prestack).
+"""
+import numpy as np
+import pytest
+
+from conftest import GOLDEN_CASES, bits_equal, load_golden
+from oracle import oracle
+import bruteforce as bf
+from beampower_amd import synthetic
+
+FLAV = {"cmake": "fma", "make": "nofma"}
+
+
+@pytest.mark.parametrize("case", GOLDEN_CASES)
+@pytest.mark.parametrize("ref_flavour", ["cmake", "make"])
+def test_oracle_matches_golden(case, ref_flavour):
+    g = load_golden(case)
+    fl = FLAV[ref_flavour]
+    F = oracle.prestack(g["u"], g["alpha"], flavour=fl)
+    assert bits_equal(F, g[f"F_{ref_flavour}"])
+    for oob in (0, 1):
+        key = f"max_{oob}_{ref_flavour}"
+        if key in g:
+            m, a = oracle.beamform_max(F, g["tau"], g["beta"], oob, flavour=fl)
+            assert bits_equal(m, g[key]), key
+            assert np.array_equal(a, g[f"argmax_{oob}_{ref_flavour}"]), key
+        key = f"none_{oob}_{ref_flavour}"
+        if key in g:
+            b = oracle.beamform(F, g["tau"], g["beta"], oob, flavour=fl)
+            assert bits_equal(b, g[key]), key
+
+
+def test_oracle_matches_golden_plain_full():
+    g = load_golden("plain_full")
+    w = synthetic.plain_c1()
+    from make_golden import inputs_digest  # noqa: E402  (tests/golden on sys.path below)
+    assert inputs_digest(w.waveform_features, w.time_delays, w.weights_phases,
+                         w.weights_sources) == eval(str(g["meta"]))["inputs_sha256"]
+    for ref_flavour, fl in FLAV.items():
+        F = oracle.prestack(w.waveform_features, w.weights_phases, flavour=fl)
+        m, a = oracle.beamform_max(F, w.time_delays, w.weights_sources, 0, flavour=fl)
+        assert bits_equal(m, g[f"max_0_{ref_flavour}"])
+        assert np.array_equal(a, g[f"argmax_0_{ref_flavour}"])
+
+
+@pytest.mark.skipif(not oracle.ref_available("cmake"), reason="oracle/_ref not built")
+@pytest.mark.parametrize("seed", [101, 102, 103])
+def test_oracle_matches_reference_build_live(seed):
+    u, alpha, tau, beta = synthetic.random_case(seed, S=7, C=3, P=2, T=300, K=90,
+                                                tau_range=(0, 120))
+    for ref_flavour, fl in FLAV.items():
+        Fr = oracle.ref_prestack(u, alpha, num_threads=2, flavour=ref_flavour)
+        F = oracle.prestack(u, alpha, flavour=fl)
+        assert bits_equal(F, Fr)
+        for oob in (0, 1):
+            mr, ar = oracle.ref_beamform_max(Fr, tau, beta, oob, num_threads=3,
+                                             flavour=ref_flavour)
+            m, a = oracle.beamform_max(F, tau, beta, oob, flavour=fl)
+            assert bits_equal(m, mr) and np.array_equal(a, ar)
+            br = oracle.ref_beamform(Fr, tau, beta, oob, num_threads=3, flavour=ref_flavour)
+            assert bits_equal(oracle.beamform(F, tau, beta, oob, flavour=fl), br)
+
+
+def test_flavours_differ():
+    """Contraction changes rounding: the two flavours are distinct oracles."""
+    u, alpha, tau, beta = synthetic.random_case(5, S=8, C=3, P=2, T=400, K=50)
+    Fa = oracle.prestack(u, alpha, flavour="fma")
+    Fb = oracle.prestack(u, alpha, flavour="nofma")
+    ba = oracle.beamform(Fa, tau, beta, 1, flavour="fma")
+    bb = oracle.beamform(Fa, tau, beta, 1, flavour="nofma")
+    assert not bits_equal(ba, bb)
+    assert np.allclose(ba, bb, rtol=1e-4, atol=1e-4)
+    assert np.allclose(Fa, Fb, rtol=1e-6, atol=1e-6)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3, 4])
+@pytest.mark.parametrize("oob", [0, 1])
+def test_oracle_vs_bruteforce_fp64(seed, oob):
+    u, alpha, tau, beta = synthetic.random_case(seed, S=4, C=3, P=2, T=50, K=9,
+                                                tau_range=(0, 15),
+                                                negative_delays=bool(oob))
+    F64 = bf.prestack(u, alpha)
+    b_ref, v_ref = bf.beams(F64, tau, beta, flexible=bool(oob))
+    # FP64 twin of the oracle: same sums in double, ~exact agreement
+    b64, v64 = oracle.beamform_f64(F64, tau, beta, oob)
+    assert np.array_equal(v64, v_ref)
+    np.testing.assert_allclose(b64[v64], b_ref[v_ref], rtol=1e-12, atol=1e-12)
+    # FP32 oracle: within float32 rounding of the FP64 sums
+    F32 = oracle.prestack(u, alpha)
+    np.testing.assert_allclose(F32, F64, rtol=1e-6, atol=1e-6)
+    b32 = oracle.beamform(F32, tau, beta, oob)
+    scale = np.abs(beta).sum(1, keepdims=True) * np.abs(F64).max() * tau.shape[2] + 1e-30
+    assert np.all(np.abs(b32 - b_ref) <= 1e-6 * scale)
+    # reduction rule: max value and its lowest index
+    best, arg, any_ = bf.reduce_max(b_ref, v_ref)
+    m32, a32 = oracle.beamform_max(F32, tau, beta, oob)
+    gap_ok = np.ones_like(any_)
+    srt = np.sort(np.where(v_ref, b_ref, -np.inf), axis=0)
+    if srt.shape[0] > 1:
+        with np.errstate(invalid="ignore"):
+            gap_ok = (srt[-1] - srt[-2]) > 1e-4 * (np.abs(srt[-1]) + 1)
+    assert np.array_equal(a32[any_ & gap_ok], arg[any_ & gap_ok])
+    assert np.all(m32[~any_] == 0.0) and np.all(a32[~any_] == 0)
+
+
+@pytest.mark.parametrize("mistake", ["keep_zero_weights", "wrong_sign", "drop_last_phase",
+                                     "transposed_phase", "off_by_one"])
+def test_planted_mistakes_are_caught(mistake):
+    """Each plausible bug, planted in the NumPy restatement, disagrees with
+    the reference's golden beams -- so these checks can fail."""
+    g = load_golden("random_small")
+    F64 = bf.prestack(g["u"], g["alpha"])
+    caught = []
+    for oob in (0, 1):
+        gold = g[f"none_{oob}_cmake"].astype(np.float64)
+        good, _ = bf.beams(F64, g["tau"], g["beta"], flexible=bool(oob))
+        bad, _ = bf.beams(F64, g["tau"], g["beta"], flexible=bool(oob), mistake=mistake)
+        assert np.allclose(good, gold, rtol=1e-5, atol=1e-5)
+        caught.append(not np.allclose(bad, gold, rtol=1e-5, atol=1e-5))
+    assert any(caught)
+
+
+def test_closed_form_impulse():
+    """A unit impulse at (s0, t0, p0) puts beta[k,s0] at t0 - tau[k,s0,p0]."""
+    rng = np.random.default_rng(7)
+    S, T, P, K = 5, 120, 2, 30
+    tau = rng.integers(0, 30, (K, S, P)).astype(np.int32)
+    beta = rng.uniform(0.5, 2.0, (K, S)).astype(np.float32)
+    F = np.zeros((S, T, P), np.float32)
+    s0, t0, p0 = 3, 70, 1
+    F[s0, t0, p0] = 1.0
+    b = oracle.beamform(F, tau, beta, 1)
+    for k in range(K):
+        expect = np.zeros(T, np.float32)
+        hits = [(s0, p) for p in range(P) if p == p0]
+        for s, p in hits:
+            expect[t0 - tau[k, s, p]] += beta[k, s]
+        assert bits_equal(b[k], expect)
+
+
+def test_closed_form_constant():
+    """Constant features c give b = c * P * sum(used beta) (exact in binary)."""
+    S, T, P, K = 6, 80, 2, 20
+    rng = np.random.default_rng(8)
+    tau = rng.integers(0, 10, (K, S, P)).astype(np.int32)
+    beta = rng.choice([0.0, 0.5, 1.0, 2.0], (K, S)).astype(np.float32)
+    F = np.full((S, T, P), 0.25, np.float32)
+    expect = (0.25 * P * beta.sum(1)).astype(np.float32)
+    for oob in (0, 1):
+        b = oracle.beamform(F, tau, beta, oob)
+        # every moveout is in [0, 10): samples below T-10 see all terms
+        assert np.array_equal(b[:, :T - 10], np.repeat(expect[:, None], T - 10, 1))
+
+
+def test_closed_form_prestack():
+    rng = np.random.default_rng(9)
+    S, C, T, P = 4, 3, 64, 2
+    u = rng.integers(-8, 8, (S, C, T)).astype(np.float32)
+    alpha = rng.choice([0.0, 0.5, 1.0], (S, C, P)).astype(np.float32)
+    for fl in ("fma", "nofma"):
+        F = oracle.prestack(u, alpha, flavour=fl)
+        np.testing.assert_array_equal(F, np.einsum("sct,scp->stp", u, alpha))
+
+
+def test_delay_bounds_vs_numpy():
+    u, alpha, tau, beta = synthetic.random_case(21, S=6, C=1, P=3, T=10, K=40,
+                                                negative_delays=True)
+    lohi = oracle.delay_bounds(tau, beta)
+    for k in range(tau.shape[0]):
+        used = beta[k] != 0
+        if used.any():
+            assert lohi[k, 0] == tau[k, used].min() and lohi[k, 1] == tau[k, used].max()
+        else:
+            assert lohi[k, 0] == np.iinfo(np.int32).max
+            assert lohi[k, 1] == np.iinfo(np.int32).min
+
+
+def test_plain_known_answer():
+    """Injected events are recovered: at each event origin the argmax is the
+    event's source or one with an identical moveout row."""
+    w = synthetic.plain_c1()
+    F = oracle.prestack(w.waveform_features, w.weights_phases)
+    m, a = oracle.beamform_max(F, w.time_delays, w.weights_sources, 0)
+    tau = w.time_delays
+    for k_e, t_e in w.events:
+        k = a[t_e]
+        assert np.array_equal(tau[k], tau[k_e])
+        assert k == np.flatnonzero((tau == tau[k_e]).all(axis=(1, 2)))[0]
