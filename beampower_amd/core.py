@@ -49,7 +49,7 @@ LIBRARIES = {
 GPU_SYMBOLS = (
     "beamform", "beamform_max", "prestack_waveform_features",
     "bp_last_error", "bp_version",
-    "bp_network_create", "bp_network_destroy", "bp_network_info",
+    "bp_network_create", "bp_network_destroy", "bp_network_info", "bp_network_classes",
     "bp_network_beamform_max", "bp_network_beamform",
     "bp_interleaved_to_planar", "bp_prestack_planar",
     "bp_profiling", "bp_profiling_read",
@@ -65,6 +65,7 @@ def _declare_gpu_extension(lib):
     lib.bp_network_create.argtypes = [vp, vp, _SZ, _SZ, _SZ, ct.POINTER(vp)]
     lib.bp_network_destroy.argtypes = [vp]
     lib.bp_network_info.argtypes = [vp, _P_I, _P_I, _P_I]
+    lib.bp_network_classes.argtypes = [vp, _P_I, _P_I]
     lib.bp_network_beamform_max.argtypes = [vp, vp, ct.c_int, vp, _SZ, ct.c_int, vp, vp, vp,
                                             vp]
     lib.bp_network_beamform.argtypes = [vp, vp, _SZ, ct.c_int, _SZ, _SZ, vp, vp]

@@ -51,15 +51,18 @@ int fail(int code, const char *fmt, ...) {
   } while (0)
 
 // Kernel configuration of the LDS shift-and-stack kernel.
-constexpr int kR = 8;    // samples per lane -> time tile 512
-constexpr int kKW = 8;   // sources per wave
-constexpr int kW = 8;    // waves per work-group -> source tile 64
-constexpr int kNJ = 4;   // staging registers per slot -> Lmax <= 2048
+constexpr int kR4 = 2;    // 4-sample runs per lane -> time tile 512
+constexpr int kKW = 8;    // sources per wave
+constexpr int kW = 8;     // waves per work-group -> source tile 64
+constexpr int kTT = 256 * kR4;
 constexpr int kTileSources = kKW * kW;
-constexpr int kDirectChunk = 64;
+constexpr int kSmallLw = 1024;   // windows of the common tiles
+constexpr int kLargeLw = 2496;   // largest window: 2 phases x 4 copies x 2 buffers in 160 KiB
+constexpr double kSmallShare = 0.97;  // share of tiles the small window must cover
+constexpr int kDirectChunk = kTileSources;
 
-template <int P, bool NONE>
-using Lds = bp::StackLds<P, 4 / P, kR, kKW, kW, kNJ, NONE>;
+template <int P, int MAXLW, bool NONE>
+using Lds = bp::StackLds<P, kR4, kKW, kW, MAXLW, NONE>;
 
 // Device buffer owned by RAII.
 struct DevBuf {
@@ -137,20 +140,23 @@ bool size_ok(size_t v) { return v <= (size_t)INT32_MAX - 4096; }
 std::once_flag g_attr_once;
 int g_attr_rc = BP_OK;
 
-template <int P, bool NONE>
+template <int P, int MAXLW, bool NONE>
 hipError_t set_lds_attr() {
-  return hipFuncSetAttribute(reinterpret_cast<const void *>(
-                                 &bp::stack_lds_kernel<P, 4 / P, kR, kKW, kW, kNJ, NONE>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)Lds<P, NONE>::lds_bytes(Lds<P, NONE>::MAX_LMAX));
+  return hipFuncSetAttribute(
+      reinterpret_cast<const void *>(&bp::stack_lds_kernel<P, kR4, kKW, kW, MAXLW, NONE>),
+      hipFuncAttributeMaxDynamicSharedMemorySize, (int)Lds<P, MAXLW, NONE>::lds_bytes(MAXLW));
 }
 
 int init_kernel_attrs() {
   std::call_once(g_attr_once, [] {
-    hipError_t e = set_lds_attr<1, false>();
-    if (e == hipSuccess) e = set_lds_attr<2, false>();
-    if (e == hipSuccess) e = set_lds_attr<1, true>();
-    if (e == hipSuccess) e = set_lds_attr<2, true>();
+    hipError_t e = hipSuccess;
+    const hipError_t r[] = {
+        set_lds_attr<1, kSmallLw, false>(), set_lds_attr<2, kSmallLw, false>(),
+        set_lds_attr<1, kLargeLw, false>(), set_lds_attr<2, kLargeLw, false>(),
+        set_lds_attr<1, kSmallLw, true>(),  set_lds_attr<2, kSmallLw, true>(),
+        set_lds_attr<1, kLargeLw, true>(),  set_lds_attr<2, kLargeLw, true>()};
+    for (hipError_t x : r)
+      if (x != hipSuccess) e = x;
     if (e != hipSuccess)
       g_attr_rc = fail(BP_EHIP, "hipFuncSetAttribute: %s", hipGetErrorString(e));
   });
@@ -159,11 +165,16 @@ int init_kernel_attrs() {
 
 }  // namespace
 
+// Source tiles are split in three classes by their largest per-(s,p) delay
+// spread: class 0 uses the LDS kernel with the small window Lw[0], class 1
+// the LDS kernel with Lw[1] <= kLargeLw, class 2 the direct kernel.
 struct bp_network {
   int K = 0, S = 0, P = 0;
   int n_tiles = 0;
   int max_spread = 0;
-  DevBuf tau, beta, lohi, tile_lo, tile_hi, tile_all;
+  int Lw[2] = {0, 0};
+  std::vector<int> tiles[3];  // ascending tile ids per class (host copy)
+  DevBuf tau, beta, lohi, tile_lo, tile_all, sched, dtiles[3];
 };
 
 namespace {
@@ -175,8 +186,8 @@ bp::StackArgs make_args(const bp_network *net, const float *F, int T, int oob) {
   a.beta = net->beta.as<float>();
   a.src_lohi = net->lohi.as<int>();
   a.tile_lo = net->tile_lo.as<int>();
-  a.tile_hi = net->tile_hi.as<int>();
   a.tile_all = net->tile_all.as<int>();
+  a.sched = net->sched.as<int>();
   a.T = T;
   a.K = net->K;
   a.S = net->S;
@@ -185,44 +196,54 @@ bp::StackArgs make_args(const bp_network *net, const float *F, int T, int oob) {
   return a;
 }
 
-// Launch the stack kernel (LDS or direct) for sources [k_begin, k_end).
+template <int MAXLW, bool NONE>
+void launch_lds(int P, const bp::StackArgs &a, unsigned blocks, hipStream_t st) {
+  if (P == 1)
+    hipLaunchKernelGGL((bp::stack_lds_kernel<1, kR4, kKW, kW, MAXLW, NONE>), dim3(blocks),
+                       dim3(64 * kW), (Lds<1, MAXLW, NONE>::lds_bytes(a.Lw)), st, a);
+  else
+    hipLaunchKernelGGL((bp::stack_lds_kernel<2, kR4, kKW, kW, MAXLW, NONE>), dim3(blocks),
+                       dim3(64 * kW), (Lds<2, MAXLW, NONE>::lds_bytes(a.Lw)), st, a);
+}
+
+// Launch the stack kernels for sources [k_begin, k_end): every class's
+// tiles overlapping the range.
 template <bool NONE>
 int launch_stack(const bp_network *net, bp::StackArgs a, int k_begin, int k_end,
                  hipStream_t st) {
   a.k_begin = k_begin;
   a.k_end = k_end;
   if (k_end <= k_begin || a.T <= 0) return BP_OK;
-  const int P = net->P;
-  const long long Lmax = (long long)kR * 64 + net->max_spread;
-  const bool lds_ok = (P == 1 || P == 2) && Lmax <= (long long)kNJ * kW * 64;
+  const int tile0 = k_begin / kTileSources;
+  const int tile1 = (k_end - 1) / kTileSources + 1;
   hipEvent_t ev[2];
   BP_TRY(prof_begin(st, ev));
-  if (lds_ok) {
-    BP_TRY(init_kernel_attrs());
-    a.Lmax = (int)Lmax;
-    a.n_time_tiles = (a.T + kR * 64 - 1) / (kR * 64);
-    const int tile0 = k_begin / kTileSources;
-    const int tile1 = (k_end - 1) / kTileSources + 1;
-    a.tile_begin = tile0;
-    const long long blocks = (long long)a.n_time_tiles * (tile1 - tile0);
-    if (blocks > INT32_MAX) return fail(BP_EINVAL, "grid too large (%lld blocks)", blocks);
-    size_t lds = P == 1 ? Lds<1, NONE>::lds_bytes(a.Lmax) : Lds<2, NONE>::lds_bytes(a.Lmax);
-    if (P == 1)
-      hipLaunchKernelGGL((bp::stack_lds_kernel<1, 4, kR, kKW, kW, kNJ, NONE>),
-                         dim3((unsigned)blocks), dim3(64 * kW), lds, st, a);
-    else
-      hipLaunchKernelGGL((bp::stack_lds_kernel<2, 2, kR, kKW, kW, kNJ, NONE>),
-                         dim3((unsigned)blocks), dim3(64 * kW), lds, st, a);
-  } else {
-    // sources per block row: at least kDirectChunk, enough to keep gridDim.y < 65536
-    const long long n = (long long)k_end - k_begin;
-    const int chunk = (int)std::max<long long>(kDirectChunk, (n + 65534) / 65535);
-    const unsigned gx = (unsigned)((a.T + 255) / 256);
-    const unsigned gy = (unsigned)((n + chunk - 1) / chunk);
-    hipLaunchKernelGGL((bp::stack_direct_kernel<NONE>), dim3(gx, gy), dim3(256), 0, st, a,
-                       chunk);
+  for (int cls = 0; cls < 3; ++cls) {
+    const std::vector<int> &v = net->tiles[cls];
+    const auto lo = std::lower_bound(v.begin(), v.end(), tile0);
+    const auto hi = std::lower_bound(v.begin(), v.end(), tile1);
+    const int n = (int)(hi - lo);
+    if (n == 0) continue;
+    a.tiles = net->dtiles[cls].as<int>() + (lo - v.begin());
+    a.n_list = n;
+    if (cls < 2) {
+      BP_TRY(init_kernel_attrs());
+      a.Lw = net->Lw[cls];
+      a.n_time_tiles = (a.T + kTT - 1) / kTT;
+      const long long blocks = (long long)a.n_time_tiles * n;
+      if (blocks > INT32_MAX) return fail(BP_EINVAL, "grid too large (%lld blocks)", blocks);
+      if (cls == 0)
+        launch_lds<kSmallLw, NONE>(net->P, a, (unsigned)blocks, st);
+      else
+        launch_lds<kLargeLw, NONE>(net->P, a, (unsigned)blocks, st);
+    } else {
+      if (n > 65535) return fail(BP_EINVAL, "too many direct-kernel tiles (%d)", n);
+      const unsigned gx = (unsigned)((a.T + 255) / 256);
+      hipLaunchKernelGGL((bp::stack_direct_kernel<NONE>), dim3(gx, (unsigned)n), dim3(256), 0,
+                         st, a, kDirectChunk);
+    }
+    HIP_TRY(hipGetLastError());
   }
-  HIP_TRY(hipGetLastError());
   BP_TRY(prof_end(st, ev));
   return BP_OK;
 }
@@ -243,6 +264,39 @@ int network_max(bp_network *net, const float *F, size_t n_samples, int oob, floa
   return BP_OK;
 }
 
+int round_up4(long long v) { return (int)((v + 3) / 4 * 4); }
+
+// Classify tiles by delay spread and choose the two LDS window lengths.
+void classify_tiles(bp_network *net, const std::vector<int> &tile_all) {
+  const int n = net->n_tiles;
+  std::vector<int> spreads;
+  spreads.reserve(n);
+  for (int j = 0; j < n; ++j) spreads.push_back(tile_all[4 * j + 3]);
+  std::vector<int> sorted = spreads;
+  std::sort(sorted.begin(), sorted.end());
+  net->max_spread = sorted.empty() ? 0 : sorted.back();
+  const bool lds = net->P == 1 || net->P == 2;
+  const int q = sorted.empty() ? 0 : sorted[std::min<size_t>(sorted.size() - 1,
+                                                   (size_t)(kSmallShare * sorted.size()))];
+  // the small window covers kSmallShare of the tiles, or as many as fit
+  const int lw0 = std::min(kSmallLw, round_up4((long long)kTT + q));
+  int lw1 = round_up4((long long)kTT + net->max_spread);
+  lw1 = std::min(lw1, kLargeLw);
+  if (lw0 > 0 && lw1 <= lw0) lw1 = 0;
+  net->Lw[0] = lw0;
+  net->Lw[1] = lw1;
+  for (int c = 0; c < 3; ++c) net->tiles[c].clear();
+  for (int j = 0; j < n; ++j) {
+    const long long need = (long long)kTT + spreads[j];
+    int cls = 2;
+    if (lds && lw0 > 0 && need <= lw0)
+      cls = 0;
+    else if (lds && lw1 > 0 && need <= lw1)
+      cls = 1;
+    net->tiles[cls].push_back(j);
+  }
+}
+
 int network_create(const int *tau, const float *beta, size_t K, size_t S, size_t P,
                    hipStream_t st, std::unique_ptr<bp_network> &out) {
   auto net = std::make_unique<bp_network>();
@@ -255,27 +309,47 @@ int network_create(const int *tau, const float *beta, size_t K, size_t S, size_t
   BP_TRY(net->beta.alloc(ks * sizeof(float)));
   BP_TRY(net->lohi.alloc(2 * K * sizeof(int)));
   BP_TRY(net->tile_lo.alloc(tsp * sizeof(int)));
-  BP_TRY(net->tile_hi.alloc(tsp * sizeof(int)));
-  BP_TRY(net->tile_all.alloc(2 * (size_t)net->n_tiles * sizeof(int)));
-  DevBuf spread;
-  BP_TRY(spread.alloc(sizeof(int)));
+  BP_TRY(net->tile_all.alloc(4 * (size_t)net->n_tiles * sizeof(int)));
   HIP_TRY(hipMemcpyAsync(net->tau.p, tau, ksp * sizeof(int), hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemcpyAsync(net->beta.p, beta, ks * sizeof(float), hipMemcpyHostToDevice, st));
-  int zero = 0;
-  HIP_TRY(hipMemcpyAsync(spread.p, &zero, sizeof(int), hipMemcpyHostToDevice, st));
   hipLaunchKernelGGL(bp::source_bounds_kernel, dim3((unsigned)((K + 255) / 256)), dim3(256),
                      0, st, net->tau.as<int>(), net->beta.as<float>(), net->K, net->S,
                      net->P, net->lohi.as<int>());
   HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(bp::tile_bounds_kernel, dim3((unsigned)net->n_tiles), dim3(256), 0, st,
                      net->tau.as<int>(), net->beta.as<float>(), net->K, net->S, net->P,
-                     kTileSources, net->tile_lo.as<int>(), net->tile_hi.as<int>(),
-                     net->tile_all.as<int>(), spread.as<int>());
+                     kTileSources, net->tile_lo.as<int>(), net->tile_all.as<int>());
   HIP_TRY(hipGetLastError());
-  int ms = 0;
-  HIP_TRY(hipMemcpyAsync(&ms, spread.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  std::vector<int> tile_all(4 * (size_t)net->n_tiles);
+  HIP_TRY(hipMemcpyAsync(tile_all.data(), net->tile_all.p, tile_all.size() * sizeof(int),
+                         hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
-  net->max_spread = ms;
+  classify_tiles(net.get(), tile_all);
+
+  std::vector<int> tile_lw(net->n_tiles, 0);
+  for (int c = 0; c < 2; ++c)
+    for (int j : net->tiles[c]) tile_lw[j] = net->Lw[c];
+  for (int c = 0; c < 3; ++c) {
+    BP_TRY(net->dtiles[c].alloc(std::max<size_t>(1, net->tiles[c].size()) * sizeof(int)));
+    if (!net->tiles[c].empty())
+      HIP_TRY(hipMemcpyAsync(net->dtiles[c].p, net->tiles[c].data(),
+                             net->tiles[c].size() * sizeof(int), hipMemcpyHostToDevice, st));
+  }
+  if (!net->tiles[0].empty() || !net->tiles[1].empty()) {
+    const long long n = (long long)net->n_tiles * S * kTileSources;
+    BP_TRY(net->sched.alloc((size_t)n * (P + 1) * sizeof(int)));
+    DevBuf dlw;
+    BP_TRY(dlw.alloc(tile_lw.size() * sizeof(int)));
+    HIP_TRY(hipMemcpyAsync(dlw.p, tile_lw.data(), tile_lw.size() * sizeof(int),
+                           hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(bp::schedule_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       st, net->tau.as<int>(), net->beta.as<float>(), net->tile_lo.as<int>(),
+                       dlw.as<int>(), net->K, net->S, net->P, kTileSources, n,
+                       net->sched.as<int>());
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));
+  }
+  HIP_TRY(hipStreamSynchronize(st));
   out = std::move(net);
   return BP_OK;
 }
@@ -358,6 +432,15 @@ int bp_network_info(const bp_network *net, int *source_tile, int *max_tile_sprea
   if (source_tile) *source_tile = kTileSources;
   if (max_tile_spread) *max_tile_spread = net->max_spread;
   if (n_tiles) *n_tiles = net->n_tiles;
+  return BP_OK;
+}
+
+int bp_network_classes(const bp_network *net, int *tiles_per_class, int *window_per_class) {
+  if (!net) return fail(BP_EINVAL, "NULL network");
+  for (int c = 0; c < 3; ++c) {
+    if (tiles_per_class) tiles_per_class[c] = (int)net->tiles[c].size();
+    if (window_per_class) window_per_class[c] = c < 2 ? net->Lw[c] : 0;
+  }
   return BP_OK;
 }
 

@@ -18,13 +18,14 @@ struct StackArgs {
   const float *beta;      // (K, S) source weights
   const int *src_lohi;    // (K, 2) per-source min/max delay over used stations
   const int *tile_lo;     // (n_tiles, S, P) min delay over the tile's users of (s,p)
-  const int *tile_hi;     // (n_tiles, S, P) max delay (INT_MIN when unused)
-  const int *tile_all;    // (n_tiles, 2) min of tile_lo / max of tile_hi
+  const int *tile_all;    // (n_tiles, 4) min lo, max hi, max lo, max (hi - lo)
+  const int *sched;       // (n_tiles, S, KT, P+1) LDS byte offsets and weight bits
+  const int *tiles;       // source tiles processed by this launch
+  int n_list;             // number of entries of `tiles`
   int T, K, S, P;
   int oob;                // 0 strict, 1 flexible
-  int Lmax;               // LDS window length per (s,p) slot
+  int Lw;                 // LDS window length per (s,p) copy, multiple of 4
   int n_time_tiles;
-  int tile_begin;         // first source tile of the launch
   int k_begin, k_end;     // sources written: [k_begin, k_end)
   unsigned long long *keys;  // reduce=max: (T) packed keys
   float *beam;               // reduce=none: beam[(k - k_begin) * T + t]
@@ -55,20 +56,21 @@ __global__ void source_bounds_kernel(const int *__restrict__ tau,
   lohi[2 * k + 1] = hi;
 }
 
-// Per source tile [tile*KT, tile*KT+KT): min/max delay of every (s,p) over
-// the tile's sources that use station s.  These bound the feature window a
-// tile needs: [t0 + lo, t0 + TT - 1 + hi].
+// Per source tile [tile*KT, tile*KT+KT): min delay of every (s,p) over the
+// tile's sources that use station s (INT_MAX when none does).  tile_all gets
+// the tile's min lo, max hi, max lo and its largest per-(s,p) spread hi - lo:
+// the tile's window of (s,p) for time tile t0 is [t0 + lo, t0 + TT - 1 + hi].
 __global__ void tile_bounds_kernel(const int *__restrict__ tau,
                                    const float *__restrict__ beta, int K, int S,
                                    int P, int KT, int *__restrict__ tile_lo,
-                                   int *__restrict__ tile_hi, int *__restrict__ tile_all,
-                                   int *__restrict__ max_spread) {
-  __shared__ int s_lo, s_hi, s_spread;
+                                   int *__restrict__ tile_all) {
+  __shared__ int s_lo, s_hi, s_maxlo, s_spread;
   const int tile = blockIdx.x;
   const int SP = S * P;
   if (threadIdx.x == 0) {
     s_lo = INT_MAX;
     s_hi = INT_MIN;
+    s_maxlo = INT_MIN;
     s_spread = 0;
   }
   __syncthreads();
@@ -84,20 +86,54 @@ __global__ void tile_bounds_kernel(const int *__restrict__ tau,
       hi = d > hi ? d : hi;
     }
     tile_lo[(size_t)tile * SP + sp] = lo;
-    tile_hi[(size_t)tile * SP + sp] = hi;
     if (lo <= hi) {
       long long spread = (long long)hi - (long long)lo;
       atomicMin(&s_lo, lo);
       atomicMax(&s_hi, hi);
+      atomicMax(&s_maxlo, lo);
       atomicMax(&s_spread, spread > INT_MAX ? INT_MAX : (int)spread);
     }
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    tile_all[2 * tile] = s_lo;
-    tile_all[2 * tile + 1] = s_hi;
-    atomicMax(max_spread, s_spread);
+    tile_all[4 * tile + 0] = s_lo;
+    tile_all[4 * tile + 1] = s_hi;
+    tile_all[4 * tile + 2] = s_maxlo;
+    tile_all[4 * tile + 3] = s_spread;
   }
+}
+
+// Per-tile schedule of the LDS kernel.  For tile j, station s and source
+// slot kk (k = j*KT + kk): P LDS byte offsets, then the weight's bits.  The
+// moveout d of phase p lies e = d - tile_lo[j,s,p] samples into the window;
+// the window is kept as 4 copies shifted by 0..3 samples, so the 4 samples
+// starting at (lane sample + e) sit 16-byte aligned in copy e&3, 16*(e>>2)
+// bytes past the lane's own offset.  Slot p occupies 4*Lw floats, copy c
+// starts 4*Lw*c bytes into it.  Sources past K and unused stations get
+// weight 0; tiles served by the direct kernel (Lw = 0) get offsets 0.
+__global__ void schedule_kernel(const int *__restrict__ tau, const float *__restrict__ beta,
+                                const int *__restrict__ tile_lo,
+                                const int *__restrict__ tile_lw, int K, int S, int P,
+                                int KT, long long n, int *__restrict__ sched) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  const int kk = (int)(idx % KT);
+  const long long js = idx / KT;
+  const int s = (int)(js % S);
+  const int tile = (int)(js / S);
+  const int k = tile * KT + kk;
+  const int Lw = tile_lw[tile];
+  const float b = k < K ? beta[(size_t)k * S + s] : 0.0f;
+  int *out = sched + idx * (P + 1);
+  for (int p = 0; p < P; ++p) {
+    int off = 0;
+    if (b != 0.0f && Lw > 0) {
+      const int e = tau[((size_t)k * S + s) * P + p] - tile_lo[((size_t)tile * S + s) * P + p];
+      off = p * 16 * Lw + (e & 3) * 4 * Lw + 16 * (e >> 2);
+    }
+    out[p] = off;
+  }
+  out[P] = __float_as_int(b);
 }
 
 // ---------------------------------------------------------------------------
@@ -168,149 +204,192 @@ __global__ void finalize_max_kernel(const unsigned long long *__restrict__ keys,
 // ---------------------------------------------------------------------------
 // Shift-and-stack, LDS-staged (the hot kernel).
 //
-// Work-group = W waves over one source tile (KT = W*KW sources, wave w owns
-// KW consecutive sources) and one time tile of TT = 64*R samples (lane l
-// owns samples t0 + l + 64 r, r < R, so every LDS gather of a wave reads 64
-// consecutive words: conflict-free).  Stations are visited in groups of G;
-// the feature windows of a group's G*P (s,p) slots,
-//   [t0 + tile_lo[s,p], t0 + TT - 1 + tile_hi[s,p]],
-// are staged in LDS (double-buffered: the next group's global loads are in
-// flight while the current group is consumed).  Accumulators acc[KW][R]
-// live in registers across all groups, so the per-(k,t) FMA chain runs in
-// ascending (s, p) order.  Moveouts and weights are wave-uniform scalar loads.
+// A work-group of W waves covers one source tile (KT = W*KW sources; wave w
+// owns KW of them) and one time tile of TT = 256*R4 samples.  Lane l owns
+// the 4-sample runs starting at t0 + 4l + 256r (r < R4), so a wave's gather
+// of one (source, station, phase) term is R4 ds_read_b128 of 64 consecutive
+// 16-byte words: conflict-free, and 4 samples per LDS lane-access, which is
+// what the gfx950 LDS pipe needs to reach its bandwidth (it serves 16 lanes
+// per clock at any width up to 16 bytes).
+//
+// Stations are visited one at a time (ascending, so every (k,t) FMA chain
+// runs in ascending (s, p) order).  For each of the station's P phases the
+// window [t0 + lo, t0 + lo + Lw + 3) of planar features is staged into LDS
+// as 4 copies shifted by 0..3 samples, double-buffered: the next station's
+// global loads are in flight while the current one is consumed.  Positions
+// outside [0, T) are staged as exact zeros.  Accumulators acc[KW][4*R4]
+// stay in registers for the whole tile.  The per-(source, station) LDS
+// offsets and weights come from the precomputed schedule as wave-uniform
+// scalar loads.
 // ---------------------------------------------------------------------------
 
-template <int P, int G, int R, int KW, int W, int NJ, bool NONE>
-struct StackLds {
-  static constexpr int TT = 64 * R;
-  static constexpr int KT = KW * W;
-  static constexpr int SLOTS = G * P;
-  static constexpr int NT = 64 * W;
-  static constexpr int MAX_LMAX = NJ * NT;
+// Block -> (time tile, listed tile) with XCD locality: work-groups are
+// dispatched round-robin over the 8 XCDs, so block b runs on XCD b % 8;
+// XCD x is given the contiguous range [x*N/8, (x+1)*N/8) of the time-major
+// order, so the feature windows of a time tile stay in that XCD's L2.
+__device__ __forceinline__ int xcd_remap(int b, int n) {
+  const int n8 = n & ~7;
+  if (b >= n8) return b;
+  return (b & 7) * (n8 >> 3) + (b >> 3);
+}
 
-  static size_t lds_bytes(int Lmax) {
-    return (NONE ? 0 : (size_t)TT * 8) + 2ull * SLOTS * Lmax * sizeof(float);
+template <int P, int R4, int KW, int W, int MAXLW, bool NONE>
+struct StackLds {
+  static constexpr int TT = 256 * R4;
+  static constexpr int KT = KW * W;
+  static constexpr int NT = 64 * W;
+  static constexpr int NACC = 4 * R4;
+  static constexpr int MAX_LW = MAXLW;         // largest window this instantiation stages
+  static constexpr int NU = (MAX_LW + NT - 1) / NT;  // staging units per thread
+
+  static size_t lds_bytes(int Lw) {
+    return (NONE ? 0 : (size_t)TT * 8) + 2ull * P * 4 * (size_t)Lw * sizeof(float);
   }
 
-  __device__ __forceinline__ static void load_group(const StackArgs &a, const int *tlo, const int *thi,
-                                    int g, int t0, int tid, float (&pre)[SLOTS][NJ]) {
+  // Staging of one station: for slot p and unit u = c*(Lw/4) + m the thread
+  // loads x[w0 + 4m + c .. +3] (w0 = t0 + tile_lo[s,p]) into one register
+  // quad, later written as copy c's aligned quad m.
+  __device__ __forceinline__ static void load_station(const StackArgs &a, const int *tlo, int s,
+                                                      int t0, bool edge, const int (&goff)[NU],
+                                                      const bool (&uok)[NU],
+                                                      float4 (&pre)[P][NU]) {
 #pragma unroll
-    for (int slot = 0; slot < SLOTS; ++slot) {
-      const int si = slot / P, p = slot % P;
-      const int s = g * G + si;
-      int lo = INT_MAX, len = 0;
-      if (s < a.S) {
-        lo = tlo[s * P + p];
-        if (lo != INT_MAX) len = TT + (thi[s * P + p] - lo);
-      }
-      const float *src = a.F + ((size_t)s * P + p) * a.T;
+    for (int p = 0; p < P; ++p) {
+      const int lo = tlo[s * P + p];
+      const float *plane = a.F + ((size_t)s * P + p) * a.T;
+      const long long w0 = (long long)t0 + lo;
 #pragma unroll
-      for (int j = 0; j < NJ; ++j) {
-        const int e = tid + j * NT;
-        const long long tg = (long long)t0 + lo + e;
-        float v = 0.0f;
-        if (e < len && tg >= 0 && tg < a.T) v = src[tg];
-        pre[slot][j] = v;
+      for (int i = 0; i < NU; ++i) {
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (lo != INT_MAX && uok[i]) {
+          if (!edge) {
+            __builtin_memcpy(&v, plane + w0 + goff[i], sizeof(v));
+          } else {
+            const long long t = w0 + goff[i];
+            v.x = (t + 0 >= 0 && t + 0 < a.T) ? plane[t + 0] : 0.f;
+            v.y = (t + 1 >= 0 && t + 1 < a.T) ? plane[t + 1] : 0.f;
+            v.z = (t + 2 >= 0 && t + 2 < a.T) ? plane[t + 2] : 0.f;
+            v.w = (t + 3 >= 0 && t + 3 < a.T) ? plane[t + 3] : 0.f;
+          }
+        }
+        pre[p][i] = v;
       }
     }
   }
 
-  __device__ __forceinline__ static void store_group(float *buf, int Lmax, int tid,
-                                     const float (&pre)[SLOTS][NJ]) {
+  __device__ __forceinline__ static void store_station(unsigned char *buf, int Lw,
+                                                       const int (&loff)[NU],
+                                                       const bool (&uok)[NU],
+                                                       const float4 (&pre)[P][NU]) {
 #pragma unroll
-    for (int slot = 0; slot < SLOTS; ++slot)
+    for (int p = 0; p < P; ++p)
 #pragma unroll
-      for (int j = 0; j < NJ; ++j) {
-        const int e = tid + j * NT;
-        if (e < Lmax) buf[slot * Lmax + e] = pre[slot][j];
-      }
+      for (int i = 0; i < NU; ++i)
+        if (uok[i])
+          *reinterpret_cast<float4 *>(buf + p * 16 * Lw + loff[i]) = pre[p][i];
   }
 
-  // Window positions outside [0, T) are staged as exact zeros, and
-  // fma(b, 0, acc) == acc for every finite b (acc is never -0.0), so in
-  // flexible mode an out-of-range term drops out exactly as the reference's
-  // explicit skip (beamform.c:97-98).  Only a non-finite weight (b*0 = NaN)
-  // needs the explicit per-term range test: `edge_flex` marks flexible-mode
-  // tiles whose windows cross [0, T), where that slow path can be taken.
-  __device__ __forceinline__ static void consume_group(const StackArgs &a, const int *tlo,
-                                                       const float *buf, int g, int kbase,
-                                                       int t0, int lane, bool edge_flex,
-                                                       float (&acc)[KW][R]) {
-    const int S = a.S, Lmax = a.Lmax;
+  __device__ __forceinline__ static void consume_station(const StackArgs &a, const int *sg,
+                                                         const unsigned char *buf, int s,
+                                                         int kbase, int t0, int lane,
+                                                         bool edge_flex,
+                                                         float (&acc)[KW][NACC]) {
+    const unsigned char *lbase = buf + 16 * lane;
 #pragma unroll
     for (int kk = 0; kk < KW; ++kk) {
-      const int k = kbase + kk;
-      if (k < a.k_begin || k >= a.k_end) continue;
-#pragma unroll
-      for (int si = 0; si < G; ++si) {
-        const int s = g * G + si;
-        if (s >= S) continue;
-        const float b = a.beta[(size_t)k * S + s];
-        if (b == 0.0f) continue;  // unused station (beamform.c:61-62)
-        if (edge_flex && !__builtin_isfinite(b)) {
+      const float b = __int_as_float(sg[kk * (P + 1) + P]);
+      if (b == 0.0f) continue;  // unused station or no such source (beamform.c:61-62)
+      if (edge_flex && !__builtin_isfinite(b)) {
+        // flexible mode, window crossing [0, T), non-finite weight: drop
+        // out-of-range terms explicitly (b * 0 would be NaN).
+        const int k = kbase + kk;
 #pragma unroll 1
-          for (int p = 0; p < P; ++p) {
-            const int d = a.tau[((size_t)k * S + s) * P + p];
-            const float *x = buf + (si * P + p) * Lmax + (d - tlo[s * P + p]) + lane;
+        for (int p = 0; p < P; ++p) {
+          const int d = a.tau[((size_t)k * a.S + s) * P + p];
+          const float4 *x = reinterpret_cast<const float4 *>(lbase + sg[kk * (P + 1) + p]);
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-              const int tg = t0 + lane + 64 * r + d;
-              if (tg >= 0 && tg < a.T) acc[kk][r] = fmaf(b, x[64 * r], acc[kk][r]);
+          for (int r = 0; r < R4; ++r) {
+            const float4 v = x[64 * r];
+            const float vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              const int tg = t0 + 4 * lane + 256 * r + i + d;
+              if (tg >= 0 && tg < a.T) acc[kk][4 * r + i] = fmaf(b, vv[i], acc[kk][4 * r + i]);
             }
           }
-          continue;
         }
+        continue;
+      }
 #pragma unroll
-        for (int p = 0; p < P; ++p) {
-          const int d = a.tau[((size_t)k * S + s) * P + p];
-          const float *x = buf + (si * P + p) * Lmax + (d - tlo[s * P + p]) + lane;
+      for (int p = 0; p < P; ++p) {
+        const float4 *x = reinterpret_cast<const float4 *>(lbase + sg[kk * (P + 1) + p]);
 #pragma unroll
-          for (int r = 0; r < R; ++r) acc[kk][r] = fmaf(b, x[64 * r], acc[kk][r]);
+        for (int r = 0; r < R4; ++r) {
+          const float4 v = x[64 * r];
+          acc[kk][4 * r + 0] = fmaf(b, v.x, acc[kk][4 * r + 0]);
+          acc[kk][4 * r + 1] = fmaf(b, v.y, acc[kk][4 * r + 1]);
+          acc[kk][4 * r + 2] = fmaf(b, v.z, acc[kk][4 * r + 2]);
+          acc[kk][4 * r + 3] = fmaf(b, v.w, acc[kk][4 * r + 3]);
         }
       }
     }
   }
 
   __device__ __forceinline__ static void run(const StackArgs &a, unsigned char *smem) {
+    const int Lw = a.Lw;
     unsigned long long *red = reinterpret_cast<unsigned long long *>(smem);
-    float *stage = reinterpret_cast<float *>(smem + (NONE ? 0 : (size_t)TT * 8));
-    const int Lmax = a.Lmax;
-    const int buf_elems = SLOTS * Lmax;
+    unsigned char *stage = smem + (NONE ? 0 : TT * 8);
+    const int buf_bytes = P * 16 * Lw;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = uniform(tid >> 6);
-    const int tt = blockIdx.x % a.n_time_tiles;
-    const int tile = a.tile_begin + blockIdx.x / a.n_time_tiles;
+    const int L = xcd_remap(blockIdx.x, a.n_time_tiles * a.n_list);
+    const int tt = L / a.n_list;
+    const int tile = a.tiles[L - tt * a.n_list];
     const int t0 = tt * TT;
-    const int SP = a.S * P;
-    const int *tlo = a.tile_lo + (size_t)tile * SP;
-    const int *thi = a.tile_hi + (size_t)tile * SP;
+    const int S = a.S;
+    const int *tlo = a.tile_lo + (size_t)tile * S * P;
+    const int *tall = a.tile_all + 4 * tile;
     const int kbase = tile * KT + wave * KW;
-    const bool edge_flex =
-        a.oob && ((long long)t0 + a.tile_all[2 * tile] < 0 ||
-                  (long long)t0 + TT - 1 + a.tile_all[2 * tile + 1] >= a.T);
+    // staged range of every slot: [t0 + lo, t0 + lo + Lw + 3)
+    const bool edge = (long long)t0 + tall[0] < 0 ||
+                      (long long)t0 + tall[2] + Lw + 3 > (long long)a.T;
+    const bool edge_flex = a.oob && edge;
+
+    // this thread's staging units (the same for every station)
+    int goff[NU], loff[NU];
+    bool uok[NU];
+    const int quads = Lw >> 2;
+#pragma unroll
+    for (int i = 0; i < NU; ++i) {
+      const int u = tid + i * NT;
+      const int c = u / quads, m = u - c * quads;
+      uok[i] = u < Lw;
+      goff[i] = 4 * m + c;
+      loff[i] = c * 4 * Lw + 16 * m;
+    }
 
     if (!NONE)
       for (int i = tid; i < TT; i += NT) red[i] = 0ull;
 
-    float acc[KW][R];
+    float acc[KW][NACC];
 #pragma unroll
     for (int kk = 0; kk < KW; ++kk)
 #pragma unroll
-      for (int r = 0; r < R; ++r) acc[kk][r] = 0.0f;
+      for (int j = 0; j < NACC; ++j) acc[kk][j] = 0.0f;
 
-    const int n_groups = (a.S + G - 1) / G;
-    float pre[SLOTS][NJ];
-    load_group(a, tlo, thi, 0, t0, tid, pre);
-    store_group(stage, Lmax, tid, pre);
+    float4 pre[P][NU];
+    load_station(a, tlo, 0, t0, edge, goff, uok, pre);
+    store_station(stage, Lw, loff, uok, pre);
     __syncthreads();
-    for (int g = 0; g < n_groups; ++g) {
-      const float *cur = stage + (g & 1) * buf_elems;
-      float *nxt = stage + ((g + 1) & 1) * buf_elems;
-      const bool more = g + 1 < n_groups;
-      if (more) load_group(a, tlo, thi, g + 1, t0, tid, pre);
-      consume_group(a, tlo, cur, g, kbase, t0, lane, edge_flex, acc);
-      if (more) store_group(nxt, Lmax, tid, pre);
+    for (int s = 0; s < S; ++s) {
+      const unsigned char *cur = stage + (s & 1) * buf_bytes;
+      unsigned char *nxt = stage + ((s + 1) & 1) * buf_bytes;
+      const bool more = s + 1 < S;
+      if (more) load_station(a, tlo, s + 1, t0, edge, goff, uok, pre);
+      const int *sg = a.sched + (((size_t)tile * S + s) * KT + wave * KW) * (P + 1);
+      consume_station(a, sg, cur, s, kbase, t0, lane, edge_flex, acc);
+      if (more) store_station(nxt, Lw, loff, uok, pre);
       __syncthreads();
     }
 
@@ -323,31 +402,39 @@ struct StackLds {
         if (!a.oob) strict_range(a.src_lohi[2 * k], a.src_lohi[2 * k + 1], a.T, vlo, vhi);
         float *row = a.beam + (size_t)(k - a.k_begin) * a.T;
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-          const int t = t0 + lane + 64 * r;
-          if (t >= vlo && t < vhi) row[t] = acc[kk][r];
+        for (int j = 0; j < NACC; ++j) {
+          const int t = t0 + 4 * lane + 256 * (j >> 2) + (j & 3);
+          if (t >= vlo && t < vhi) row[t] = acc[kk][j];
         }
       }
     } else {
+      float bv[NACC];
+      int bk[NACC];
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int t = t0 + lane + 64 * r;
-        float bv = -FLT_MAX;
-        int bk = 0;
+      for (int j = 0; j < NACC; ++j) {
+        bv[j] = -FLT_MAX;
+        bk[j] = 0;
+      }
 #pragma unroll
-        for (int kk = 0; kk < KW; ++kk) {
-          const int k = kbase + kk;
-          if (k < a.k_begin || k >= a.k_end) continue;
-          int vlo = 0, vhi = a.T;
-          if (!a.oob) strict_range(a.src_lohi[2 * k], a.src_lohi[2 * k + 1], a.T, vlo, vhi);
-          const float v = acc[kk][r];
-          if (t >= vlo && t < vhi && v > bv) {
-            bv = v;
-            bk = k;
+      for (int kk = 0; kk < KW; ++kk) {
+        const int k = kbase + kk;
+        if (k < a.k_begin || k >= a.k_end) continue;
+        int vlo = 0, vhi = a.T;
+        if (!a.oob) strict_range(a.src_lohi[2 * k], a.src_lohi[2 * k + 1], a.T, vlo, vhi);
+#pragma unroll
+        for (int j = 0; j < NACC; ++j) {
+          const int t = t0 + 4 * lane + 256 * (j >> 2) + (j & 3);
+          const float v = acc[kk][j];
+          if (t >= vlo && t < vhi && v > bv[j]) {
+            bv[j] = v;
+            bk[j] = k;
           }
         }
-        if (bv > -FLT_MAX) atomicMax(&red[lane + 64 * r], pack_key(bv, bk));
       }
+#pragma unroll
+      for (int j = 0; j < NACC; ++j)
+        if (bv[j] > -FLT_MAX)
+          atomicMax(&red[4 * lane + 256 * (j >> 2) + (j & 3)], pack_key(bv[j], bk[j]));
       __syncthreads();
       for (int i = tid; i < TT; i += NT) {
         const unsigned long long key = red[i];
@@ -357,23 +444,25 @@ struct StackLds {
   }
 };
 
-template <int P, int G, int R, int KW, int W, int NJ, bool NONE>
+template <int P, int R4, int KW, int W, int MAXLW, bool NONE>
 __global__ __launch_bounds__(64 * W) void stack_lds_kernel(StackArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  StackLds<P, G, R, KW, W, NJ, NONE>::run(a, smem);
+  StackLds<P, R4, KW, W, MAXLW, NONE>::run(a, smem);
 }
 
 // ---------------------------------------------------------------------------
-// Shift-and-stack, direct gathers from global memory.  Used when the LDS
-// kernel does not apply (n_phases outside {1,2}, or a source tile whose delay
-// spread exceeds the LDS window).  Same FMA order, so same bits.
-// One thread per time sample; blockIdx.y selects a chunk of sources.
+// Shift-and-stack, direct gathers from global memory.  Used for source tiles
+// the LDS kernel does not take (n_phases outside {1,2}, or a tile whose
+// delay spread exceeds the largest LDS window).  Same FMA order, same bits.
+// One thread per time sample; blockIdx.y selects a listed source tile of
+// `chunk` sources.
 // ---------------------------------------------------------------------------
 template <bool NONE>
 __global__ __launch_bounds__(256) void stack_direct_kernel(StackArgs a, int chunk) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int k0 = a.k_begin + blockIdx.y * chunk;
-  const int k1 = min(a.k_end, k0 + chunk);
+  const long long tile = a.tiles[blockIdx.y];
+  const int k0 = (int)max((long long)a.k_begin, tile * chunk);
+  const int k1 = (int)min((long long)a.k_end, tile * chunk + chunk);
   const int S = a.S, P = a.P, T = a.T;
   float bv = -FLT_MAX;
   int bk = 0;

@@ -58,7 +58,11 @@ class Network:
         tile, spread, n = ct.c_int(), ct.c_int(), ct.c_int()
         check(self.lib.bp_network_info(self._h, ct.byref(tile), ct.byref(spread),
                                        ct.byref(n)), "bp_network_info", self.lib)
-        return dict(source_tile=tile.value, max_tile_spread=spread.value, n_tiles=n.value)
+        counts, windows = (ct.c_int * 3)(), (ct.c_int * 3)()
+        check(self.lib.bp_network_classes(self._h, counts, windows), "bp_network_classes",
+              self.lib)
+        return dict(source_tile=tile.value, max_tile_spread=spread.value, n_tiles=n.value,
+                    tiles_per_class=list(counts), window_per_class=list(windows))
 
     def beamform_max(self, features, out_of_bounds=0, beam_max=None, beam_argmax=None,
                      keys=None, stream=None):

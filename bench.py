@@ -210,6 +210,7 @@ def main():
             "workload": w.name, "stations": S, "channels": C, "phases": P,
             "sources": K, "samples": T, "reduce": "max", "out_of_bounds": args.oob,
             "source_tile": info["source_tile"], "max_tile_spread": info["max_tile_spread"],
+            "tiles_per_class": info["tiles_per_class"], "window_per_class": info["window_per_class"],
         },
         "events_recovered": f"{recovered}/{len(w.events)}",
         "roofline": roofline,

@@ -84,6 +84,9 @@ int bp_network_destroy(bp_network *net);
 /* Largest per-(station, phase) delay spread inside one source tile. */
 int bp_network_info(const bp_network *net, int *source_tile, int *max_tile_spread,
                     int *n_tiles);
+/* Source tiles per kernel class (small LDS window, large LDS window,
+ * direct gathers) and the LDS window length of each class (0 = direct). */
+int bp_network_classes(const bp_network *net, int *tiles_per_class, int *window_per_class);
 
 /* reduce='max' on device-resident planar features.
  * features: DEVICE pointer, planar (n_stations, n_phases, n_samples) in the

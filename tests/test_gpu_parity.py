@@ -136,3 +136,50 @@ def test_nonfinite_weights_in_flexible_edges(gpu_lib):
     assert bits_equal(m, mo) and np.array_equal(a, ao)
     b = abi_none(gpu_lib, F, tau, beta, 1)
     assert bits_equal(b, oracle.beamform(F, tau, beta, 1))
+
+
+def test_all_tile_classes_vs_oracle(gpu_lib):
+    """Moveout spreads that put source tiles in each kernel class: small LDS
+    window, large LDS window and direct gathers."""
+    from beampower_amd import device as bpd
+    rng = np.random.default_rng(51)
+    S, C, P, T, K = 12, 3, 2, 4600, 64 * 60
+    u = rng.standard_normal((S, C, T)).astype(np.float32)
+    alpha = rng.uniform(-1, 1, (S, C, P)).astype(np.float32)
+    tau = rng.integers(0, 60, (K, S, P)).astype(np.int32)
+    tau[64 * 5:64 * 6] = rng.integers(0, 1400, (64, S, P))    # large window
+    tau[64 * 9:64 * 10] = rng.integers(0, 1900, (64, S, P))   # large window
+    tau[64 * 20:64 * 21] = rng.integers(0, 4000, (64, S, P))  # direct
+    beta = rng.uniform(0.5, 1.5, (K, S)).astype(np.float32)
+    beta[rng.random((K, S)) < 0.1] = 0.0
+    net = bpd.Network(tau, beta)
+    n = (ct.c_int * 3)()
+    lw = (ct.c_int * 3)()
+    assert gpu_lib.bp_network_classes(ct.c_void_p(net._h.value), n, lw) == 0
+    assert n[0] > 0 and n[1] > 0 and n[2] > 0, list(n)
+    F = oracle.prestack(u, alpha)
+    for oob in (0, 1):
+        m, a = abi_max(gpu_lib, F, tau, beta, oob)
+        mo, ao = oracle.beamform_max(F, tau, beta, oob)
+        assert bits_equal(m, mo) and np.array_equal(a, ao), oob
+        b = abi_none(gpu_lib, F, tau, beta, oob)
+        assert bits_equal(b, oracle.beamform(F, tau, beta, oob)), oob
+
+
+def test_device_api_matches_drop_in(gpu_lib):
+    """Device-resident Network path equals the host drop-in on the same data,
+    including reduce='none' over a source sub-range."""
+    import torch
+    from beampower_amd import device as bpd
+    w = synthetic.plain_c1()
+    F = oracle.prestack(w.waveform_features, w.weights_phases)
+    m0, a0 = abi_max(gpu_lib, F, w.time_delays, w.weights_sources, 0)
+    net = bpd.Network(w.time_delays, w.weights_sources)
+    u = torch.from_numpy(w.waveform_features).cuda()
+    Fd = bpd.prestack_planar(u, torch.from_numpy(w.weights_phases).cuda())
+    m, a = net.beamform_max(Fd, 0)
+    torch.cuda.synchronize()
+    assert bits_equal(m.cpu().numpy(), m0) and np.array_equal(a.cpu().numpy(), a0)
+    b = net.beamform(Fd, 1, source_begin=1000, source_end=1300).cpu().numpy()
+    bo = oracle.beamform(F, w.time_delays[1000:1300], w.weights_sources[1000:1300], 1)
+    assert bits_equal(b, bo)
