@@ -57,7 +57,7 @@ constexpr int kW = 8;     // waves per work-group -> source tile 64
 constexpr int kTT = 256 * kR4;
 constexpr int kTileSources = kKW * kW;
 constexpr int kSmallLw = 1024;   // windows of the common tiles
-constexpr int kLargeLw = 2496;   // largest window: 2 phases x 4 copies x 2 buffers in 160 KiB
+constexpr int kLargeLw = 2304;   // largest window: 2 phases x 4 copies x 2 buffers in 160 KiB
 constexpr double kSmallShare = 0.97;  // share of tiles the small window must cover
 constexpr int kDirectChunk = kTileSources;
 
@@ -271,7 +271,7 @@ void classify_tiles(bp_network *net, const std::vector<int> &tile_all) {
   const int n = net->n_tiles;
   std::vector<int> spreads;
   spreads.reserve(n);
-  for (int j = 0; j < n; ++j) spreads.push_back(tile_all[4 * j + 3]);
+  for (int j = 0; j < n; ++j) spreads.push_back(tile_all[8 * j + 3]);
   std::vector<int> sorted = spreads;
   std::sort(sorted.begin(), sorted.end());
   net->max_spread = sorted.empty() ? 0 : sorted.back();
@@ -288,10 +288,13 @@ void classify_tiles(bp_network *net, const std::vector<int> &tile_all) {
   for (int c = 0; c < 3; ++c) net->tiles[c].clear();
   for (int j = 0; j < n; ++j) {
     const long long need = (long long)kTT + spreads[j];
+    // a tile with an infinite or NaN weight goes to the direct kernel, which
+    // drops out-of-range terms explicitly (b * 0 would be NaN)
+    const bool ok = lds && tile_all[8 * j + 4] == 0;
     int cls = 2;
-    if (lds && lw0 > 0 && need <= lw0)
+    if (ok && lw0 > 0 && need <= lw0)
       cls = 0;
-    else if (lds && lw1 > 0 && need <= lw1)
+    else if (ok && lw1 > 0 && need <= lw1)
       cls = 1;
     net->tiles[cls].push_back(j);
   }
@@ -309,7 +312,7 @@ int network_create(const int *tau, const float *beta, size_t K, size_t S, size_t
   BP_TRY(net->beta.alloc(ks * sizeof(float)));
   BP_TRY(net->lohi.alloc(2 * K * sizeof(int)));
   BP_TRY(net->tile_lo.alloc(tsp * sizeof(int)));
-  BP_TRY(net->tile_all.alloc(4 * (size_t)net->n_tiles * sizeof(int)));
+  BP_TRY(net->tile_all.alloc(8 * (size_t)net->n_tiles * sizeof(int)));
   HIP_TRY(hipMemcpyAsync(net->tau.p, tau, ksp * sizeof(int), hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemcpyAsync(net->beta.p, beta, ks * sizeof(float), hipMemcpyHostToDevice, st));
   hipLaunchKernelGGL(bp::source_bounds_kernel, dim3((unsigned)((K + 255) / 256)), dim3(256),
@@ -320,7 +323,7 @@ int network_create(const int *tau, const float *beta, size_t K, size_t S, size_t
                      net->tau.as<int>(), net->beta.as<float>(), net->K, net->S, net->P,
                      kTileSources, net->tile_lo.as<int>(), net->tile_all.as<int>());
   HIP_TRY(hipGetLastError());
-  std::vector<int> tile_all(4 * (size_t)net->n_tiles);
+  std::vector<int> tile_all(8 * (size_t)net->n_tiles);
   HIP_TRY(hipMemcpyAsync(tile_all.data(), net->tile_all.p, tile_all.size() * sizeof(int),
                          hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));

@@ -18,7 +18,7 @@ struct StackArgs {
   const float *beta;      // (K, S) source weights
   const int *src_lohi;    // (K, 2) per-source min/max delay over used stations
   const int *tile_lo;     // (n_tiles, S, P) min delay over the tile's users of (s,p)
-  const int *tile_all;    // (n_tiles, 4) min lo, max hi, max lo, max (hi - lo)
+  const int *tile_all;    // (n_tiles, 8) min lo, max hi, max lo, max (hi - lo), non-finite weight flag
   const int *sched;       // (n_tiles, S, KT, P+1) LDS byte offsets and weight bits
   const int *tiles;       // source tiles processed by this launch
   int n_list;             // number of entries of `tiles`
@@ -57,14 +57,15 @@ __global__ void source_bounds_kernel(const int *__restrict__ tau,
 }
 
 // Per source tile [tile*KT, tile*KT+KT): min delay of every (s,p) over the
-// tile's sources that use station s (INT_MAX when none does).  tile_all gets
-// the tile's min lo, max hi, max lo and its largest per-(s,p) spread hi - lo:
-// the tile's window of (s,p) for time tile t0 is [t0 + lo, t0 + TT - 1 + hi].
+// tile's sources that use station s (INT_MAX when none does).  tile_all
+// (8 ints per tile) gets the tile's min lo, max hi, max lo, its largest
+// per-(s,p) spread hi - lo, and 1 if some weight of the tile is infinite or
+// NaN.  The tile's window of (s,p) for time tile t0 is [t0 + lo, t0 + TT - 1 + hi].
 __global__ void tile_bounds_kernel(const int *__restrict__ tau,
                                    const float *__restrict__ beta, int K, int S,
                                    int P, int KT, int *__restrict__ tile_lo,
                                    int *__restrict__ tile_all) {
-  __shared__ int s_lo, s_hi, s_maxlo, s_spread;
+  __shared__ int s_lo, s_hi, s_maxlo, s_spread, s_nonfinite;
   const int tile = blockIdx.x;
   const int SP = S * P;
   if (threadIdx.x == 0) {
@@ -72,6 +73,7 @@ __global__ void tile_bounds_kernel(const int *__restrict__ tau,
     s_hi = INT_MIN;
     s_maxlo = INT_MIN;
     s_spread = 0;
+    s_nonfinite = 0;
   }
   __syncthreads();
   const int k0 = tile * KT;
@@ -80,7 +82,9 @@ __global__ void tile_bounds_kernel(const int *__restrict__ tau,
     const int s = sp / P;
     int lo = INT_MAX, hi = INT_MIN;
     for (int k = k0; k < k1; ++k) {
-      if (beta[(size_t)k * S + s] == 0.0f) continue;
+      const float b = beta[(size_t)k * S + s];
+      if (b == 0.0f) continue;
+      if (!__builtin_isfinite(b)) s_nonfinite = 1;
       int d = tau[(size_t)k * SP + sp];
       lo = d < lo ? d : lo;
       hi = d > hi ? d : hi;
@@ -96,10 +100,13 @@ __global__ void tile_bounds_kernel(const int *__restrict__ tau,
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    tile_all[4 * tile + 0] = s_lo;
-    tile_all[4 * tile + 1] = s_hi;
-    tile_all[4 * tile + 2] = s_maxlo;
-    tile_all[4 * tile + 3] = s_spread;
+    int *o = tile_all + 8 * tile;
+    o[0] = s_lo;
+    o[1] = s_hi;
+    o[2] = s_maxlo;
+    o[3] = s_spread;
+    o[4] = s_nonfinite;
+    o[5] = o[6] = o[7] = 0;
   }
 }
 
@@ -108,9 +115,12 @@ __global__ void tile_bounds_kernel(const int *__restrict__ tau,
 // moveout d of phase p lies e = d - tile_lo[j,s,p] samples into the window;
 // the window is kept as 4 copies shifted by 0..3 samples, so the 4 samples
 // starting at (lane sample + e) sit 16-byte aligned in copy e&3, 16*(e>>2)
-// bytes past the lane's own offset.  Slot p occupies 4*Lw floats, copy c
-// starts 4*Lw*c bytes into it.  Sources past K and unused stations get
-// weight 0; tiles served by the direct kernel (Lw = 0) get offsets 0.
+// bytes past the lane's own offset.  A copy holds CS = Lw rounded up to 256
+// floats; slot p starts at p*16*CS bytes and copy c 4*CS*c bytes into it.
+// Sources past K and unused stations get weight 0 and the offset P*16*CS of
+// the all-zero block that follows the slots, so the kernel needs no branch
+// for them: fma(0, 0, acc) == acc.
+// Tiles served by the direct kernel (Lw = 0) get offsets 0.
 __global__ void schedule_kernel(const int *__restrict__ tau, const float *__restrict__ beta,
                                 const int *__restrict__ tile_lo,
                                 const int *__restrict__ tile_lw, int K, int S, int P,
@@ -127,9 +137,14 @@ __global__ void schedule_kernel(const int *__restrict__ tau, const float *__rest
   int *out = sched + idx * (P + 1);
   for (int p = 0; p < P; ++p) {
     int off = 0;
-    if (b != 0.0f && Lw > 0) {
-      const int e = tau[((size_t)k * S + s) * P + p] - tile_lo[((size_t)tile * S + s) * P + p];
-      off = p * 16 * Lw + (e & 3) * 4 * Lw + 16 * (e >> 2);
+    if (Lw > 0) {
+      const int CS = (Lw + 255) & ~255;  // copy stride, whole 1 KiB rows
+      off = P * 16 * CS;
+      if (b != 0.0f) {
+        const int e =
+            tau[((size_t)k * S + s) * P + p] - tile_lo[((size_t)tile * S + s) * P + p];
+        off = p * 16 * CS + (e & 3) * 4 * CS + 16 * (e >> 2);
+      }
     }
     out[p] = off;
   }
@@ -233,6 +248,13 @@ __device__ __forceinline__ int xcd_remap(int b, int n) {
   return (b & 7) * (n8 >> 3) + (b >> 3);
 }
 
+typedef __attribute__((address_space(1))) void global_void;
+typedef __attribute__((address_space(3))) void lds_void;
+
+// Copy stride of a window of Lw samples: whole 1 KiB chunks (256 floats),
+// so staging always moves complete 64-lane x 16-byte rows.
+__host__ __device__ __forceinline__ int copy_stride(int Lw) { return (Lw + 255) & ~255; }
+
 template <int P, int R4, int KW, int W, int MAXLW, bool NONE>
 struct StackLds {
   static constexpr int TT = 256 * R4;
@@ -240,106 +262,124 @@ struct StackLds {
   static constexpr int NT = 64 * W;
   static constexpr int NACC = 4 * R4;
   static constexpr int MAX_LW = MAXLW;         // largest window this instantiation stages
-  static constexpr int NU = (MAX_LW + NT - 1) / NT;  // staging units per thread
+  static constexpr int NU = (MAX_LW + NT - 1) / NT;  // edge-path staging units per thread
+  static constexpr int ZERO_BYTES = 16 * 64 * R4;  // all-zero block read by unused terms
 
-  static size_t lds_bytes(int Lw) {
-    return (NONE ? 0 : (size_t)TT * 8) + 2ull * P * 4 * (size_t)Lw * sizeof(float);
+  // LDS of one buffer: P slots of 4 copies (stride copy_stride(Lw) floats),
+  // then the zero block.
+  __host__ __device__ static size_t buf_bytes(int Lw) {
+    return (size_t)P * 16 * copy_stride(Lw) + ZERO_BYTES;
+  }
+  static size_t lds_bytes(int Lw) { return (NONE ? 0 : (size_t)TT * 8) + 2 * buf_bytes(Lw); }
+
+  // Interior tiles: the station's windows go global -> LDS directly
+  // (global_load_lds_dwordx4, no registers).  Row (p, c, j) of the station
+  // is copy c of phase p, quads [64j, 64j + 64): lane l loads
+  // x[w0 + 4m + c .. +3], m = 64j + l (clamped to the last quad, so the
+  // padding past the window gets a harmless duplicate), into LDS at
+  // slot p + copy c + 16m.  Row r is issued by wave r % W.
+  __device__ __forceinline__ static void dma_station(const StackArgs &a, const int *tlo, int s,
+                                                     int t0, int wave, int lane,
+                                                     unsigned char *buf) {
+    const int CS = copy_stride(a.Lw);
+    const int nch = CS >> 8;
+    const int last_quad = (a.Lw >> 2) - 1;
+    const int rows = P * 4 * nch;
+    for (int row = wave; row < rows; row += W) {
+      const int p = row / (4 * nch);
+      const int rem = row - p * 4 * nch;
+      const int c = rem / nch;
+      const int j = rem - c * nch;
+      const int lo = tlo[s * P + p];
+      if (lo == INT_MAX) continue;  // no source of the tile uses station s
+      const int m = min(64 * j + lane, last_quad);
+      const float *src = a.F + ((size_t)s * P + p) * a.T + (t0 + lo) + 4 * m + c;
+      unsigned char *dst = buf + p * 16 * CS + c * 4 * CS + 1024 * j;
+      __builtin_amdgcn_global_load_lds((global_void *)src, (lds_void *)dst, 16, 0, 0);
+    }
   }
 
-  // Staging of one station: for slot p and unit u = c*(Lw/4) + m the thread
-  // loads x[w0 + 4m + c .. +3] (w0 = t0 + tile_lo[s,p]) into one register
-  // quad, later written as copy c's aligned quad m.
-  __device__ __forceinline__ static void load_station(const StackArgs &a, const int *tlo, int s,
-                                                      int t0, bool edge, const int (&goff)[NU],
-                                                      const bool (&uok)[NU],
-                                                      float4 (&pre)[P][NU]) {
+  // Edge tiles (a window crosses [0, T)): guarded loads through registers,
+  // zeros outside [0, T).  Unit u = c*(Lw/4) + m of slot p.
+  __device__ __forceinline__ static void stage_station_guarded(const StackArgs &a,
+                                                               const int *tlo, int s, int t0,
+                                                               int tid, unsigned char *buf) {
+    const int Lw = a.Lw, CS = copy_stride(Lw), quads = Lw >> 2;
 #pragma unroll
     for (int p = 0; p < P; ++p) {
       const int lo = tlo[s * P + p];
+      if (lo == INT_MAX) continue;
       const float *plane = a.F + ((size_t)s * P + p) * a.T;
       const long long w0 = (long long)t0 + lo;
-#pragma unroll
-      for (int i = 0; i < NU; ++i) {
-        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (lo != INT_MAX && uok[i]) {
-          if (!edge) {
-            __builtin_memcpy(&v, plane + w0 + goff[i], sizeof(v));
-          } else {
-            const long long t = w0 + goff[i];
-            v.x = (t + 0 >= 0 && t + 0 < a.T) ? plane[t + 0] : 0.f;
-            v.y = (t + 1 >= 0 && t + 1 < a.T) ? plane[t + 1] : 0.f;
-            v.z = (t + 2 >= 0 && t + 2 < a.T) ? plane[t + 2] : 0.f;
-            v.w = (t + 3 >= 0 && t + 3 < a.T) ? plane[t + 3] : 0.f;
-          }
-        }
-        pre[p][i] = v;
+      for (int u = tid; u < Lw; u += NT) {
+        const int c = u / quads, m = u - c * quads;
+        const long long t = w0 + 4 * m + c;
+        float4 v;
+        v.x = (t + 0 >= 0 && t + 0 < a.T) ? plane[t + 0] : 0.f;
+        v.y = (t + 1 >= 0 && t + 1 < a.T) ? plane[t + 1] : 0.f;
+        v.z = (t + 2 >= 0 && t + 2 < a.T) ? plane[t + 2] : 0.f;
+        v.w = (t + 3 >= 0 && t + 3 < a.T) ? plane[t + 3] : 0.f;
+        *reinterpret_cast<float4 *>(buf + p * 16 * CS + c * 4 * CS + 16 * m) = v;
       }
     }
   }
 
-  __device__ __forceinline__ static void store_station(unsigned char *buf, int Lw,
-                                                       const int (&loff)[NU],
-                                                       const bool (&uok)[NU],
-                                                       const float4 (&pre)[P][NU]) {
+  // Operands of one station for this wave's KW sources: KW*(P+1)
+  // wave-uniform words (P byte offsets and the weight per source).
+  __device__ __forceinline__ static void load_sched(const int *sg, int (&sv)[KW * (P + 1)]) {
+#pragma unroll
+    for (int i = 0; i < KW * (P + 1); ++i) sv[i] = sg[i];
+  }
+
+  __device__ __forceinline__ static void gather(const unsigned char *lbase, const int *svk,
+                                                float4 (&v)[P][R4]) {
 #pragma unroll
     for (int p = 0; p < P; ++p)
 #pragma unroll
-      for (int i = 0; i < NU; ++i)
-        if (uok[i])
-          *reinterpret_cast<float4 *>(buf + p * 16 * Lw + loff[i]) = pre[p][i];
+      for (int r = 0; r < R4; ++r)
+        v[p][r] = *reinterpret_cast<const float4 *>(lbase + svk[p] + 1024 * r);
   }
 
-  __device__ __forceinline__ static void consume_station(const StackArgs &a, const int *sg,
-                                                         const unsigned char *buf, int s,
-                                                         int kbase, int t0, int lane,
-                                                         bool edge_flex,
+  // One station: straight-line gathers and FMAs, software-pipelined by one
+  // source (the gathers of source kk+1 are in flight during the FMAs of kk;
+  // the scheduling fences keep the compiler from hoisting every gather of
+  // the station at once, which would exhaust the registers).
+  __device__ __forceinline__ static void consume_station(const int (&sv)[KW * (P + 1)],
+                                                         const unsigned char *buf, int lane,
                                                          float (&acc)[KW][NACC]) {
     const unsigned char *lbase = buf + 16 * lane;
+    float4 v[2][P][R4];
+    gather(lbase, sv, v[0]);
 #pragma unroll
     for (int kk = 0; kk < KW; ++kk) {
-      const float b = __int_as_float(sg[kk * (P + 1) + P]);
-      if (b == 0.0f) continue;  // unused station or no such source (beamform.c:61-62)
-      if (edge_flex && !__builtin_isfinite(b)) {
-        // flexible mode, window crossing [0, T), non-finite weight: drop
-        // out-of-range terms explicitly (b * 0 would be NaN).
-        const int k = kbase + kk;
-#pragma unroll 1
-        for (int p = 0; p < P; ++p) {
-          const int d = a.tau[((size_t)k * a.S + s) * P + p];
-          const float4 *x = reinterpret_cast<const float4 *>(lbase + sg[kk * (P + 1) + p]);
+      const int c = kk & 1;
+      if (kk + 1 < KW) gather(lbase, sv + (kk + 1) * (P + 1), v[c ^ 1]);
+      const float b = __int_as_float(sv[kk * (P + 1) + P]);
 #pragma unroll
-          for (int r = 0; r < R4; ++r) {
-            const float4 v = x[64 * r];
-            const float vv[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-              const int tg = t0 + 4 * lane + 256 * r + i + d;
-              if (tg >= 0 && tg < a.T) acc[kk][4 * r + i] = fmaf(b, vv[i], acc[kk][4 * r + i]);
-            }
-          }
-        }
-        continue;
-      }
-#pragma unroll
-      for (int p = 0; p < P; ++p) {
-        const float4 *x = reinterpret_cast<const float4 *>(lbase + sg[kk * (P + 1) + p]);
+      for (int p = 0; p < P; ++p)
 #pragma unroll
         for (int r = 0; r < R4; ++r) {
-          const float4 v = x[64 * r];
-          acc[kk][4 * r + 0] = fmaf(b, v.x, acc[kk][4 * r + 0]);
-          acc[kk][4 * r + 1] = fmaf(b, v.y, acc[kk][4 * r + 1]);
-          acc[kk][4 * r + 2] = fmaf(b, v.z, acc[kk][4 * r + 2]);
-          acc[kk][4 * r + 3] = fmaf(b, v.w, acc[kk][4 * r + 3]);
+          acc[kk][4 * r + 0] = fmaf(b, v[c][p][r].x, acc[kk][4 * r + 0]);
+          acc[kk][4 * r + 1] = fmaf(b, v[c][p][r].y, acc[kk][4 * r + 1]);
+          acc[kk][4 * r + 2] = fmaf(b, v[c][p][r].z, acc[kk][4 * r + 2]);
+          acc[kk][4 * r + 3] = fmaf(b, v[c][p][r].w, acc[kk][4 * r + 3]);
         }
-      }
+      // Fence: this source's FMAs complete here (the empty asm statements
+      // take its accumulators as operands) and later gathers stay behind it
+      // (memory clobber), so at most two sources' gathers are live.
+#pragma unroll
+      for (int j = 0; j < NACC; ++j) asm volatile("" : "+v"(acc[kk][j]));
+      asm volatile("" ::: "memory");
+      __builtin_amdgcn_sched_barrier(0);
     }
   }
 
   __device__ __forceinline__ static void run(const StackArgs &a, unsigned char *smem) {
     const int Lw = a.Lw;
+    const int CS = copy_stride(Lw);
     unsigned long long *red = reinterpret_cast<unsigned long long *>(smem);
     unsigned char *stage = smem + (NONE ? 0 : TT * 8);
-    const int buf_bytes = P * 16 * Lw;
+    const int bbytes = (int)buf_bytes(Lw);
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = uniform(tid >> 6);
@@ -349,28 +389,19 @@ struct StackLds {
     const int t0 = tt * TT;
     const int S = a.S;
     const int *tlo = a.tile_lo + (size_t)tile * S * P;
-    const int *tall = a.tile_all + 4 * tile;
+    const int *tall = a.tile_all + 8 * tile;
     const int kbase = tile * KT + wave * KW;
     // staged range of every slot: [t0 + lo, t0 + lo + Lw + 3)
     const bool edge = (long long)t0 + tall[0] < 0 ||
                       (long long)t0 + tall[2] + Lw + 3 > (long long)a.T;
-    const bool edge_flex = a.oob && edge;
-
-    // this thread's staging units (the same for every station)
-    int goff[NU], loff[NU];
-    bool uok[NU];
-    const int quads = Lw >> 2;
-#pragma unroll
-    for (int i = 0; i < NU; ++i) {
-      const int u = tid + i * NT;
-      const int c = u / quads, m = u - c * quads;
-      uok[i] = u < Lw;
-      goff[i] = 4 * m + c;
-      loff[i] = c * 4 * Lw + 16 * m;
-    }
 
     if (!NONE)
       for (int i = tid; i < TT; i += NT) red[i] = 0ull;
+    for (int i = tid; i < ZERO_BYTES / 16; i += NT) {
+      const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+      *reinterpret_cast<float4 *>(stage + P * 16 * CS + 16 * i) = z;
+      *reinterpret_cast<float4 *>(stage + bbytes + P * 16 * CS + 16 * i) = z;
+    }
 
     float acc[KW][NACC];
 #pragma unroll
@@ -378,18 +409,29 @@ struct StackLds {
 #pragma unroll
       for (int j = 0; j < NACC; ++j) acc[kk][j] = 0.0f;
 
-    float4 pre[P][NU];
-    load_station(a, tlo, 0, t0, edge, goff, uok, pre);
-    store_station(stage, Lw, loff, uok, pre);
+    // schedule rows of this wave: station s at sched_w + s * KT * (P + 1)
+    const int *sched_w = a.sched + ((size_t)tile * S * KT + wave * KW) * (P + 1);
+    int sv[KW * (P + 1)];
+    load_sched(sched_w, sv);
+    if (edge)
+      stage_station_guarded(a, tlo, 0, t0, tid, stage);
+    else
+      dma_station(a, tlo, 0, t0, wave, lane, stage);
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     for (int s = 0; s < S; ++s) {
-      const unsigned char *cur = stage + (s & 1) * buf_bytes;
-      unsigned char *nxt = stage + ((s + 1) & 1) * buf_bytes;
+      const unsigned char *cur = stage + (s & 1) * bbytes;
+      unsigned char *nxt = stage + ((s + 1) & 1) * bbytes;
       const bool more = s + 1 < S;
-      if (more) load_station(a, tlo, s + 1, t0, edge, goff, uok, pre);
-      const int *sg = a.sched + (((size_t)tile * S + s) * KT + wave * KW) * (P + 1);
-      consume_station(a, sg, cur, s, kbase, t0, lane, edge_flex, acc);
-      if (more) store_station(nxt, Lw, loff, uok, pre);
+      if (more && !edge) dma_station(a, tlo, s + 1, t0, wave, lane, nxt);
+      consume_station(sv, cur, lane, acc);
+      if (more) {
+        // next station's operands: the scalar loads overlap the barrier
+        load_sched(sched_w + (size_t)(s + 1) * KT * (P + 1), sv);
+        if (edge) stage_station_guarded(a, tlo, s + 1, t0, tid, nxt);
+      }
+      // the station's LDS-DMA writes must land before the barrier
+      __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
       __syncthreads();
     }
 
@@ -444,8 +486,11 @@ struct StackLds {
   }
 };
 
+// At least 4 waves per SIMD (VGPRs <= 128): two work-groups per CU, so one
+// computes while the other waits at its per-station barrier.
 template <int P, int R4, int KW, int W, int MAXLW, bool NONE>
-__global__ __launch_bounds__(64 * W) void stack_lds_kernel(StackArgs a) {
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4)))
+void stack_lds_kernel(StackArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   StackLds<P, R4, KW, W, MAXLW, NONE>::run(a, smem);
 }
