@@ -250,6 +250,14 @@ __device__ __forceinline__ int xcd_remap(int b, int n) {
 
 typedef __attribute__((address_space(1))) void global_void;
 typedef __attribute__((address_space(3))) void lds_void;
+// Read-only per-network tables are read through the constant address space:
+// wave-uniform loads from it are always scalar (s_load), whatever the
+// compiler can prove about intervening memory operations.
+typedef const __attribute__((address_space(4))) int const_int;
+
+__device__ __forceinline__ const_int *as_const(const int *p) {
+  return (const_int *)(p);
+}
 
 // Copy stride of a window of Lw samples: whole 1 KiB chunks (256 floats),
 // so staging always moves complete 64-lane x 16-byte rows.
@@ -257,6 +265,7 @@ __host__ __device__ __forceinline__ int copy_stride(int Lw) { return (Lw + 255) 
 
 template <int P, int R4, int KW, int W, int MAXLW, bool NONE>
 struct StackLds {
+  static_assert(P == 1 || P == 2, "the LDS kernel takes one or two phases");
   static constexpr int TT = 256 * R4;
   static constexpr int KT = KW * W;
   static constexpr int NT = 64 * W;
@@ -273,36 +282,70 @@ struct StackLds {
   static size_t lds_bytes(int Lw) { return (NONE ? 0 : (size_t)TT * 8) + 2 * buf_bytes(Lw); }
 
   // Interior tiles: the station's windows go global -> LDS directly
-  // (global_load_lds_dwordx4, no registers).  Row (p, c, j) of the station
-  // is copy c of phase p, quads [64j, 64j + 64): lane l loads
-  // x[w0 + 4m + c .. +3], m = 64j + l (clamped to the last quad, so the
-  // padding past the window gets a harmless duplicate), into LDS at
-  // slot p + copy c + 16m.  Row r is issued by wave r % W.
-  __device__ __forceinline__ static void dma_station(const StackArgs &a, const int *tlo, int s,
-                                                     int t0, int wave, int lane,
-                                                     unsigned char *buf) {
-    const int CS = copy_stride(a.Lw);
+  // (global_load_lds_dwordx4, no registers).  Row (p, c, j) of a station is
+  // copy c of phase p, quads [64j, 64j + 64): lane l loads
+  // x[w0 + 4m + c .. +3], m = 64j + l (clamped to the window's last quad, so
+  // the padding past the window gets a harmless duplicate), into LDS at
+  // slot p + copy c + 16m.  Row r belongs to wave r % W.  The rows of a wave
+  // are the same for every station, so their LDS offsets and per-lane
+  // element offsets are computed once (RowPlan).
+  static constexpr int NR = (P * 4 * (((MAXLW + 255) & ~255) >> 8) + W - 1) / W;
+
+  struct RowPlan {
+    int n;          // rows of this wave
+    int p[NR];      // phase of each row (wave-uniform)
+    int lds[NR];    // LDS byte offset of each row in a buffer (wave-uniform)
+    unsigned eoff[NR];  // this lane's byte offset 4*(4m + c) within the window
+  };
+
+  __device__ __forceinline__ static void plan_rows(int Lw, int wave, int lane, RowPlan &rp) {
+    const int CS = copy_stride(Lw);
     const int nch = CS >> 8;
-    const int last_quad = (a.Lw >> 2) - 1;
+    const int last_quad = (Lw >> 2) - 1;
     const int rows = P * 4 * nch;
-    for (int row = wave; row < rows; row += W) {
-      const int p = row / (4 * nch);
-      const int rem = row - p * 4 * nch;
+    rp.n = 0;
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+      const int row = wave + i * W;
+      const int rr = row < rows ? row : 0;
+      const int p = rr / (4 * nch);
+      const int rem = rr - p * 4 * nch;
       const int c = rem / nch;
       const int j = rem - c * nch;
-      const int lo = tlo[s * P + p];
-      if (lo == INT_MAX) continue;  // no source of the tile uses station s
-      const int m = min(64 * j + lane, last_quad);
-      const float *src = a.F + ((size_t)s * P + p) * a.T + (t0 + lo) + 4 * m + c;
-      unsigned char *dst = buf + p * 16 * CS + c * 4 * CS + 1024 * j;
-      __builtin_amdgcn_global_load_lds((global_void *)src, (lds_void *)dst, 16, 0, 0);
+      rp.p[i] = p;
+      rp.lds[i] = p * 16 * CS + c * 4 * CS + 1024 * j;
+      rp.eoff[i] = 4u * (unsigned)(4 * min(64 * j + lane, last_quad) + c);
+      rp.n += row < rows ? 1 : 0;
+    }
+  }
+
+  __device__ __forceinline__ static void dma_station(const StackArgs &a, const_int *tlo, int s,
+                                                     int t0, const RowPlan &rp,
+                                                     unsigned char *__restrict__ buf) {
+    int lo[P];
+    const float *plane[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      lo[p] = tlo[s * P + p];
+      plane[p] = a.F + ((size_t)s * P + p) * a.T + (t0 + lo[p]);
+    }
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+      if (i >= rp.n) break;
+      const int p = rp.p[i];
+      const int lop = P == 1 ? lo[0] : (p == 0 ? lo[0] : lo[P - 1]);
+      if (lop == INT_MAX) continue;  // no source of the tile uses station s
+      const char *base =
+          reinterpret_cast<const char *>(P == 1 ? plane[0] : (p == 0 ? plane[0] : plane[P - 1]));
+      __builtin_amdgcn_global_load_lds((global_void *)(base + rp.eoff[i]),
+                                       (lds_void *)(buf + rp.lds[i]), 16, 0, 0);
     }
   }
 
   // Edge tiles (a window crosses [0, T)): guarded loads through registers,
   // zeros outside [0, T).  Unit u = c*(Lw/4) + m of slot p.
   __device__ __forceinline__ static void stage_station_guarded(const StackArgs &a,
-                                                               const int *tlo, int s, int t0,
+                                                               const_int *tlo, int s, int t0,
                                                                int tid, unsigned char *buf) {
     const int Lw = a.Lw, CS = copy_stride(Lw), quads = Lw >> 2;
 #pragma unroll
@@ -326,34 +369,36 @@ struct StackLds {
 
   // Operands of one station for this wave's KW sources: KW*(P+1)
   // wave-uniform words (P byte offsets and the weight per source).
-  __device__ __forceinline__ static void load_sched(const int *sg, int (&sv)[KW * (P + 1)]) {
+  __device__ __forceinline__ static void load_sched(const_int *sg, int (&sv)[KW * (P + 1)]) {
 #pragma unroll
     for (int i = 0; i < KW * (P + 1); ++i) sv[i] = sg[i];
   }
 
-  __device__ __forceinline__ static void gather(const unsigned char *lbase, const int *svk,
-                                                float4 (&v)[P][R4]) {
+  __device__ __forceinline__ static void gather(const unsigned char *buf, unsigned lb,
+                                                const int *svk, float4 (&v)[P][R4]) {
 #pragma unroll
     for (int p = 0; p < P; ++p)
 #pragma unroll
       for (int r = 0; r < R4; ++r)
-        v[p][r] = *reinterpret_cast<const float4 *>(lbase + svk[p] + 1024 * r);
+        v[p][r] = *reinterpret_cast<const float4 *>(buf + (lb + (unsigned)svk[p]) + 1024 * r);
   }
 
   // One station: straight-line gathers and FMAs, software-pipelined by one
-  // source (the gathers of source kk+1 are in flight during the FMAs of kk;
-  // the scheduling fences keep the compiler from hoisting every gather of
-  // the station at once, which would exhaust the registers).
+  // source (the gathers of source kk+1 are in flight during the FMAs of kk).
+  // A fence after each source keeps the compiler from hoisting every gather
+  // of the station at once, which would exhaust the registers: the empty asm
+  // statements take the source's accumulators (its FMAs complete there) and
+  // the lane's LDS offset (later gathers depend on it, so they stay behind).
   __device__ __forceinline__ static void consume_station(const int (&sv)[KW * (P + 1)],
                                                          const unsigned char *buf, int lane,
                                                          float (&acc)[KW][NACC]) {
-    const unsigned char *lbase = buf + 16 * lane;
+    unsigned lb = 16u * (unsigned)lane;
     float4 v[2][P][R4];
-    gather(lbase, sv, v[0]);
+    gather(buf, lb, sv, v[0]);
 #pragma unroll
     for (int kk = 0; kk < KW; ++kk) {
       const int c = kk & 1;
-      if (kk + 1 < KW) gather(lbase, sv + (kk + 1) * (P + 1), v[c ^ 1]);
+      if (kk + 1 < KW) gather(buf, lb, sv + (kk + 1) * (P + 1), v[c ^ 1]);
       const float b = __int_as_float(sv[kk * (P + 1) + P]);
 #pragma unroll
       for (int p = 0; p < P; ++p)
@@ -364,14 +409,26 @@ struct StackLds {
           acc[kk][4 * r + 2] = fmaf(b, v[c][p][r].z, acc[kk][4 * r + 2]);
           acc[kk][4 * r + 3] = fmaf(b, v[c][p][r].w, acc[kk][4 * r + 3]);
         }
-      // Fence: this source's FMAs complete here (the empty asm statements
-      // take its accumulators as operands) and later gathers stay behind it
-      // (memory clobber), so at most two sources' gathers are live.
 #pragma unroll
       for (int j = 0; j < NACC; ++j) asm volatile("" : "+v"(acc[kk][j]));
-      asm volatile("" ::: "memory");
+      asm volatile("" : "+v"(lb));
       __builtin_amdgcn_sched_barrier(0);
     }
+  }
+
+  // One pipeline step: start the next station's global->LDS copies into
+  // `nxt`, then consume the current station from `cur`.  Both buffers are
+  // restrict-qualified in one scope, so the compiler knows the copies cannot
+  // alias the gathers and does not wait for them before reading `cur`.
+  __device__ __forceinline__ static void station_step(const StackArgs &a, const_int *tlo,
+                                                      int s_next, int t0, bool dma,
+                                                      const RowPlan &rp,
+                                                      const int (&sv)[KW * (P + 1)],
+                                                      const unsigned char *__restrict__ cur,
+                                                      unsigned char *__restrict__ nxt,
+                                                      int lane, float (&acc)[KW][NACC]) {
+    if (dma) dma_station(a, tlo, s_next, t0, rp, nxt);
+    consume_station(sv, cur, lane, acc);
   }
 
   __device__ __forceinline__ static void run(const StackArgs &a, unsigned char *smem) {
@@ -385,11 +442,11 @@ struct StackLds {
     const int wave = uniform(tid >> 6);
     const int L = xcd_remap(blockIdx.x, a.n_time_tiles * a.n_list);
     const int tt = L / a.n_list;
-    const int tile = a.tiles[L - tt * a.n_list];
+    const int tile = as_const(a.tiles)[L - tt * a.n_list];
     const int t0 = tt * TT;
     const int S = a.S;
-    const int *tlo = a.tile_lo + (size_t)tile * S * P;
-    const int *tall = a.tile_all + 8 * tile;
+    const_int *tlo = as_const(a.tile_lo) + (size_t)tile * S * P;
+    const_int *tall = as_const(a.tile_all) + 8 * tile;
     const int kbase = tile * KT + wave * KW;
     // staged range of every slot: [t0 + lo, t0 + lo + Lw + 3)
     const bool edge = (long long)t0 + tall[0] < 0 ||
@@ -410,21 +467,22 @@ struct StackLds {
       for (int j = 0; j < NACC; ++j) acc[kk][j] = 0.0f;
 
     // schedule rows of this wave: station s at sched_w + s * KT * (P + 1)
-    const int *sched_w = a.sched + ((size_t)tile * S * KT + wave * KW) * (P + 1);
+    const_int *sched_w = as_const(a.sched) + ((size_t)tile * S * KT + wave * KW) * (P + 1);
     int sv[KW * (P + 1)];
     load_sched(sched_w, sv);
+    RowPlan rp;
+    plan_rows(Lw, wave, lane, rp);
     if (edge)
       stage_station_guarded(a, tlo, 0, t0, tid, stage);
     else
-      dma_station(a, tlo, 0, t0, wave, lane, stage);
+      dma_station(a, tlo, 0, t0, rp, stage);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     for (int s = 0; s < S; ++s) {
       const unsigned char *cur = stage + (s & 1) * bbytes;
       unsigned char *nxt = stage + ((s + 1) & 1) * bbytes;
       const bool more = s + 1 < S;
-      if (more && !edge) dma_station(a, tlo, s + 1, t0, wave, lane, nxt);
-      consume_station(sv, cur, lane, acc);
+      station_step(a, tlo, s + 1, t0, more && !edge, rp, sv, cur, nxt, lane, acc);
       if (more) {
         // next station's operands: the scalar loads overlap the barrier
         load_sched(sched_w + (size_t)(s + 1) * KT * (P + 1), sv);
@@ -441,7 +499,9 @@ struct StackLds {
         const int k = kbase + kk;
         if (k < a.k_begin || k >= a.k_end) continue;
         int vlo = 0, vhi = a.T;
-        if (!a.oob) strict_range(a.src_lohi[2 * k], a.src_lohi[2 * k + 1], a.T, vlo, vhi);
+        if (!a.oob)
+          strict_range(as_const(a.src_lohi)[2 * k], as_const(a.src_lohi)[2 * k + 1], a.T, vlo,
+                       vhi);
         float *row = a.beam + (size_t)(k - a.k_begin) * a.T;
 #pragma unroll
         for (int j = 0; j < NACC; ++j) {
@@ -462,7 +522,9 @@ struct StackLds {
         const int k = kbase + kk;
         if (k < a.k_begin || k >= a.k_end) continue;
         int vlo = 0, vhi = a.T;
-        if (!a.oob) strict_range(a.src_lohi[2 * k], a.src_lohi[2 * k + 1], a.T, vlo, vhi);
+        if (!a.oob)
+          strict_range(as_const(a.src_lohi)[2 * k], as_const(a.src_lohi)[2 * k + 1], a.T, vlo,
+                       vhi);
 #pragma unroll
         for (int j = 0; j < NACC; ++j) {
           const int t = t0 + 4 * lane + 256 * (j >> 2) + (j & 3);
