@@ -51,5 +51,22 @@ def build(verbose=False):
     return LIB_PATH
 
 
+def build_probe(verbose=False):
+    """Peak probe of the LDS and VALU pipes (measurement only, not part of
+    the library): beampower_amd/lib/probe_peaks."""
+    out = os.path.join(LIB_DIR, "probe_peaks")
+    cmd = [hipcc(), "--offload-arch=gfx950", "-O3", "-Xclang", "-target-feature", "-Xclang",
+           "-packed-fp32-ops", os.path.join(CSRC, "probe_peaks.hip"), "-o", out]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("hipcc failed building probe_peaks")
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    if "--probe" in sys.argv:
+        print(build_probe(verbose=True))

@@ -24,13 +24,16 @@ sys.path.insert(0, REPO)
 METRIC = ("source·samples/s, FP32 drop-in vs BF16 features; "
           "p50/p99 served-window latency")
 
-# gfx950 (MI355X) published figures
+# gfx950 (MI355X) ceilings.  The LDS pipe serves 16 lanes per clock at any
+# width up to 16 bytes, so the gather roofline is the ds_read_b128 rate:
+# 256 B/clk/CU (profiles/peaks_probe.md measures 138 TB/s of the 157 TB/s).
 N_CU = 256
 CLOCK_HZ = 2.4e9
-LDS_BYTES_PER_CLK_CU = 128
-LDS_PEAK = N_CU * LDS_BYTES_PER_CLK_CU * CLOCK_HZ  # 78.6 TB/s
+LDS_BYTES_PER_CLK_CU = 256
+LDS_PEAK = N_CU * LDS_BYTES_PER_CLK_CU * CLOCK_HZ  # 157.3 TB/s
 HBM_PEAK = 8.0e12
-VALU_FP32_PEAK_FLOPS = 157.3e12
+VALU_FP32_PEAK_FLOPS = 157.3e12          # published, counts packed FMA
+VALU_FP32_UNPACKED_FLOPS = 78.6e12       # v_fma_f32: 64 FMA/clk/CU
 
 
 def log(msg):
@@ -184,6 +187,7 @@ def main():
         "kernel_ms": kernel_ms,
         "algorithmic_bytes_per_launch": lds_bytes,
         "valu_frac": (2.0 * fmas / (kernel_ms * 1e-3)) / VALU_FP32_PEAK_FLOPS,
+        "valu_frac_unpacked": (2.0 * fmas / (kernel_ms * 1e-3)) / VALU_FP32_UNPACKED_FLOPS,
         "hbm_compulsory_frac": ((S * P * T * 4 + K * S * P * 4 + K * S * 4 + 8 * T)
                                 / (kernel_ms * 1e-3)) / HBM_PEAK,
     }
