@@ -52,6 +52,7 @@ GPU_SYMBOLS = (
     "bp_network_create", "bp_network_destroy", "bp_network_info", "bp_network_classes",
     "bp_network_beamform_max", "bp_network_beamform",
     "bp_interleaved_to_planar", "bp_prestack_planar",
+    "bp_parity_stride", "bp_quantize_planar",
     "bp_profiling", "bp_profiling_read",
 )
 
@@ -71,11 +72,14 @@ def _declare_gpu_extension(lib):
     lib.bp_network_beamform.argtypes = [vp, vp, _SZ, ct.c_int, _SZ, _SZ, vp, vp]
     lib.bp_interleaved_to_planar.argtypes = [vp, _SZ, _SZ, _SZ, vp, vp]
     lib.bp_prestack_planar.argtypes = [vp, vp, _SZ, _SZ, _SZ, _SZ, vp, vp]
+    lib.bp_parity_stride.argtypes = [_SZ]
+    lib.bp_quantize_planar.argtypes = [vp, _SZ, _SZ, _SZ, ct.c_int, vp, vp, vp]
     lib.bp_profiling.argtypes = [ct.c_int]
     lib.bp_profiling_read.argtypes = [ct.POINTER(ct.c_double), ct.POINTER(ct.c_longlong)]
     for name in GPU_SYMBOLS:
-        if name not in ("bp_last_error", "bp_version"):
+        if name not in ("bp_last_error", "bp_version", "bp_parity_stride"):
             getattr(lib, name).restype = ct.c_int
+    lib.bp_parity_stride.restype = _SZ
 
 
 def load_library(device="cpu"):

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -50,19 +51,25 @@ int fail(int code, const char *fmt, ...) {
     if (rc_ != BP_OK) return rc_; \
   } while (0)
 
-// Kernel configuration of the LDS shift-and-stack kernel.
-constexpr int kR4 = 2;    // 4-sample runs per lane -> time tile 512
-constexpr int kKW = 8;    // sources per wave
-constexpr int kW = 8;     // waves per work-group -> source tile 64
-constexpr int kTT = 256 * kR4;
+// Kernel configuration of the LDS shift-and-stack kernel (overridable at
+// build time for tuning sweeps).
+#ifndef BP_KW
+#define BP_KW 8
+#endif
+#ifndef BP_W
+#define BP_W 8
+#endif
+constexpr int kKW = BP_KW;  // sources per wave
+constexpr int kW = BP_W;    // waves per work-group -> source tile KW*W
+constexpr int kTT = 512;    // time tile (samples)
 constexpr int kTileSources = kKW * kW;
 constexpr int kSmallLw = 1024;   // windows of the common tiles
-constexpr int kLargeLw = 2304;   // largest window: 2 phases x 4 copies x 2 buffers in 160 KiB
+constexpr int kLargeLw = 2048;   // largest window: 2 phases x copies x 2 buffers in 160 KiB
 constexpr double kSmallShare = 0.97;  // share of tiles the small window must cover
 constexpr int kDirectChunk = kTileSources;
 
-template <int P, int MAXLW, bool NONE>
-using Lds = bp::StackLds<P, kR4, kKW, kW, MAXLW, NONE>;
+template <int FMT, int P, int MAXLW, bool NONE>
+using Lds = bp::StackLds<FMT, P, kKW, kW, MAXLW, NONE>;
 
 // Device buffer owned by RAII.
 struct DevBuf {
@@ -140,54 +147,74 @@ bool size_ok(size_t v) { return v <= (size_t)INT32_MAX - 4096; }
 std::once_flag g_attr_once;
 int g_attr_rc = BP_OK;
 
-template <int P, int MAXLW, bool NONE>
+template <int FMT, int P, int MAXLW, bool NONE>
 hipError_t set_lds_attr() {
   return hipFuncSetAttribute(
-      reinterpret_cast<const void *>(&bp::stack_lds_kernel<P, kR4, kKW, kW, MAXLW, NONE>),
-      hipFuncAttributeMaxDynamicSharedMemorySize, (int)Lds<P, MAXLW, NONE>::lds_bytes(MAXLW));
+      reinterpret_cast<const void *>(&bp::stack_lds_kernel<FMT, P, kKW, kW, MAXLW, NONE>),
+      hipFuncAttributeMaxDynamicSharedMemorySize,
+      (int)Lds<FMT, P, MAXLW, NONE>::lds_bytes(MAXLW));
+}
+
+template <int FMT, bool NONE>
+hipError_t set_lds_attrs() {
+  hipError_t r[] = {set_lds_attr<FMT, 1, kSmallLw, NONE>(), set_lds_attr<FMT, 2, kSmallLw, NONE>(),
+                    set_lds_attr<FMT, 1, kLargeLw, NONE>(), set_lds_attr<FMT, 2, kLargeLw, NONE>()};
+  for (hipError_t x : r)
+    if (x != hipSuccess) return x;
+  return hipSuccess;
 }
 
 int init_kernel_attrs() {
   std::call_once(g_attr_once, [] {
-    hipError_t e = hipSuccess;
-    const hipError_t r[] = {
-        set_lds_attr<1, kSmallLw, false>(), set_lds_attr<2, kSmallLw, false>(),
-        set_lds_attr<1, kLargeLw, false>(), set_lds_attr<2, kLargeLw, false>(),
-        set_lds_attr<1, kSmallLw, true>(),  set_lds_attr<2, kSmallLw, true>(),
-        set_lds_attr<1, kLargeLw, true>(),  set_lds_attr<2, kLargeLw, true>()};
-    for (hipError_t x : r)
-      if (x != hipSuccess) e = x;
-    if (e != hipSuccess)
-      g_attr_rc = fail(BP_EHIP, "hipFuncSetAttribute: %s", hipGetErrorString(e));
+    hipError_t r[] = {set_lds_attrs<BP_F32, false>(), set_lds_attrs<BP_F32, true>(),
+                      set_lds_attrs<BP_F16, false>(), set_lds_attrs<BP_BF16, false>()};
+    for (hipError_t e : r)
+      if (e != hipSuccess) {
+        g_attr_rc = fail(BP_EHIP, "hipFuncSetAttribute: %s", hipGetErrorString(e));
+        break;
+      }
   });
   return g_attr_rc;
 }
+
+// Row stride of the 16-bit parity planes: T + 1 rounded up to 8 elements
+// (even, so every plane starts 4-byte aligned; 16-byte aligned in fact).
+size_t parity_stride(size_t T) { return (T + 1 + 7) / 8 * 8; }
 
 }  // namespace
 
 // Source tiles are split in three classes by their largest per-(s,p) delay
 // spread: class 0 uses the LDS kernel with the small window Lw[0], class 1
-// the LDS kernel with Lw[1] <= kLargeLw, class 2 the direct kernel.
+// the LDS kernel with Lw[1] <= kLargeLw, class 2 the direct kernel.  The
+// LDS kernel's schedule depends on the feature format (copy layout, weight
+// encoding, per-station scale) and is built per format on first use, and
+// again when the scales change.
 struct bp_network {
   int K = 0, S = 0, P = 0;
   int n_tiles = 0;
   int max_spread = 0;
   int Lw[2] = {0, 0};
   std::vector<int> tiles[3];  // ascending tile ids per class (host copy)
-  DevBuf tau, beta, lohi, tile_lo, tile_all, sched, dtiles[3];
+  DevBuf tau, beta, lohi, tile_lo, tile_all, tile_lw, dtiles[3];
+  DevBuf sched[3], scale[3];
+  bool sched_ok[3] = {false, false, false};
+  std::vector<float> sched_scale[3];
+  std::mutex mu;  // schedule (re)builds
 };
 
 namespace {
 
-bp::StackArgs make_args(const bp_network *net, const float *F, int T, int oob) {
+bp::StackArgs make_args(const bp_network *net, const void *F, int fmt, int T, int oob) {
   bp::StackArgs a{};
   a.F = F;
+  a.ldf = fmt == BP_F32 ? (long long)T : (long long)parity_stride(T);
+  a.scale = fmt == BP_F32 ? nullptr : net->scale[fmt].as<float>();
   a.tau = net->tau.as<int>();
   a.beta = net->beta.as<float>();
   a.src_lohi = net->lohi.as<int>();
   a.tile_lo = net->tile_lo.as<int>();
   a.tile_all = net->tile_all.as<int>();
-  a.sched = net->sched.as<int>();
+  a.sched = net->sched[fmt].as<int>();
   a.T = T;
   a.K = net->K;
   a.S = net->S;
@@ -196,19 +223,50 @@ bp::StackArgs make_args(const bp_network *net, const float *F, int T, int oob) {
   return a;
 }
 
-template <int MAXLW, bool NONE>
+bool has_lds_tiles(const bp_network *net) {
+  return !net->tiles[0].empty() || !net->tiles[1].empty();
+}
+
+// Build (or rebuild after a scale change) the schedule of format `fmt`.
+// scales: host pointer to n_stations floats, or NULL for 1.
+int ensure_schedule(bp_network *net, int fmt, const float *scales, hipStream_t st) {
+  std::lock_guard<std::mutex> lock(net->mu);
+  std::vector<float> want(net->S, 1.0f);
+  if (scales) want.assign(scales, scales + net->S);
+  if (net->sched_ok[fmt] && net->sched_scale[fmt] == want) return BP_OK;
+  const long long n = (long long)net->n_tiles * net->S * kTileSources;
+  if (!net->scale[fmt].p) BP_TRY(net->scale[fmt].alloc((size_t)net->S * sizeof(float)));
+  HIP_TRY(hipMemcpyAsync(net->scale[fmt].p, want.data(), want.size() * sizeof(float),
+                         hipMemcpyHostToDevice, st));
+  if (has_lds_tiles(net)) {
+    if (!net->sched[fmt].p) BP_TRY(net->sched[fmt].alloc((size_t)n * (net->P + 1) * sizeof(int)));
+    hipLaunchKernelGGL(bp::schedule_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                       net->tau.as<int>(), net->beta.as<float>(), net->tile_lo.as<int>(),
+                       net->tile_lw.as<int>(),
+                       fmt == BP_F32 ? (const float *)nullptr : net->scale[fmt].as<float>(),
+                       net->K, net->S, net->P, kTileSources, fmt, n, net->sched[fmt].as<int>());
+    HIP_TRY(hipGetLastError());
+  }
+  // the host vector `want` is read by the copy above: finish before returning
+  HIP_TRY(hipStreamSynchronize(st));
+  net->sched_scale[fmt] = want;
+  net->sched_ok[fmt] = true;
+  return BP_OK;
+}
+
+template <int FMT, int MAXLW, bool NONE>
 void launch_lds(int P, const bp::StackArgs &a, unsigned blocks, hipStream_t st) {
   if (P == 1)
-    hipLaunchKernelGGL((bp::stack_lds_kernel<1, kR4, kKW, kW, MAXLW, NONE>), dim3(blocks),
-                       dim3(64 * kW), (Lds<1, MAXLW, NONE>::lds_bytes(a.Lw)), st, a);
+    hipLaunchKernelGGL((bp::stack_lds_kernel<FMT, 1, kKW, kW, MAXLW, NONE>), dim3(blocks),
+                       dim3(64 * kW), (Lds<FMT, 1, MAXLW, NONE>::lds_bytes(a.Lw)), st, a);
   else
-    hipLaunchKernelGGL((bp::stack_lds_kernel<2, kR4, kKW, kW, MAXLW, NONE>), dim3(blocks),
-                       dim3(64 * kW), (Lds<2, MAXLW, NONE>::lds_bytes(a.Lw)), st, a);
+    hipLaunchKernelGGL((bp::stack_lds_kernel<FMT, 2, kKW, kW, MAXLW, NONE>), dim3(blocks),
+                       dim3(64 * kW), (Lds<FMT, 2, MAXLW, NONE>::lds_bytes(a.Lw)), st, a);
 }
 
 // Launch the stack kernels for sources [k_begin, k_end): every class's
 // tiles overlapping the range.
-template <bool NONE>
+template <int FMT, bool NONE>
 int launch_stack(const bp_network *net, bp::StackArgs a, int k_begin, int k_end,
                  hipStream_t st) {
   a.k_begin = k_begin;
@@ -233,14 +291,14 @@ int launch_stack(const bp_network *net, bp::StackArgs a, int k_begin, int k_end,
       const long long blocks = (long long)a.n_time_tiles * n;
       if (blocks > INT32_MAX) return fail(BP_EINVAL, "grid too large (%lld blocks)", blocks);
       if (cls == 0)
-        launch_lds<kSmallLw, NONE>(net->P, a, (unsigned)blocks, st);
+        launch_lds<FMT, kSmallLw, NONE>(net->P, a, (unsigned)blocks, st);
       else
-        launch_lds<kLargeLw, NONE>(net->P, a, (unsigned)blocks, st);
+        launch_lds<FMT, kLargeLw, NONE>(net->P, a, (unsigned)blocks, st);
     } else {
       if (n > 65535) return fail(BP_EINVAL, "too many direct-kernel tiles (%d)", n);
       const unsigned gx = (unsigned)((a.T + 255) / 256);
-      hipLaunchKernelGGL((bp::stack_direct_kernel<NONE>), dim3(gx, (unsigned)n), dim3(256), 0,
-                         st, a, kDirectChunk);
+      hipLaunchKernelGGL((bp::stack_direct_kernel<FMT, NONE>), dim3(gx, (unsigned)n), dim3(256),
+                         0, st, a, kDirectChunk);
     }
     HIP_TRY(hipGetLastError());
   }
@@ -248,25 +306,33 @@ int launch_stack(const bp_network *net, bp::StackArgs a, int k_begin, int k_end,
   return BP_OK;
 }
 
-int network_max(bp_network *net, const float *F, size_t n_samples, int oob, float *bmax,
-                int *barg, unsigned long long *keys, hipStream_t st) {
+int network_max(bp_network *net, const void *F, int fmt, const float *scales, size_t n_samples,
+                int oob, float *bmax, int *barg, unsigned long long *keys, hipStream_t st) {
   const int T = (int)n_samples;
+  BP_TRY(ensure_schedule(net, fmt, fmt == BP_F32 ? nullptr : scales, st));
   const unsigned nb = (unsigned)((T + 255) / 256);
   hipLaunchKernelGGL(bp::fill_u64_kernel, dim3(nb), dim3(256), 0, st, keys, (long long)T,
                      0ull);
   HIP_TRY(hipGetLastError());
-  bp::StackArgs a = make_args(net, F, T, oob);
+  bp::StackArgs a = make_args(net, F, fmt, T, oob);
   a.keys = keys;
-  BP_TRY(launch_stack<false>(net, a, 0, net->K, st));
+  if (fmt == BP_F32)
+    BP_TRY((launch_stack<BP_F32, false>(net, a, 0, net->K, st)));
+  else if (fmt == BP_F16)
+    BP_TRY((launch_stack<BP_F16, false>(net, a, 0, net->K, st)));
+  else
+    BP_TRY((launch_stack<BP_BF16, false>(net, a, 0, net->K, st)));
   hipLaunchKernelGGL(bp::finalize_max_kernel, dim3(nb), dim3(256), 0, st, keys, T, bmax,
                      barg);
   HIP_TRY(hipGetLastError());
   return BP_OK;
 }
 
-int round_up4(long long v) { return (int)((v + 3) / 4 * 4); }
+int round_up8(long long v) { return (int)((v + 7) / 8 * 8); }
 
-// Classify tiles by delay spread and choose the two LDS window lengths.
+// Classify tiles by delay spread and choose the two LDS window lengths
+// (multiples of 8 samples, so a 16-byte row of 16-bit samples never reads
+// past the window's staged range).
 void classify_tiles(bp_network *net, const std::vector<int> &tile_all) {
   const int n = net->n_tiles;
   std::vector<int> spreads;
@@ -279,8 +345,8 @@ void classify_tiles(bp_network *net, const std::vector<int> &tile_all) {
   const int q = sorted.empty() ? 0 : sorted[std::min<size_t>(sorted.size() - 1,
                                                    (size_t)(kSmallShare * sorted.size()))];
   // the small window covers kSmallShare of the tiles, or as many as fit
-  const int lw0 = std::min(kSmallLw, round_up4((long long)kTT + q));
-  int lw1 = round_up4((long long)kTT + net->max_spread);
+  const int lw0 = std::min(kSmallLw, round_up8((long long)kTT + q));
+  int lw1 = round_up8((long long)kTT + net->max_spread);
   lw1 = std::min(lw1, kLargeLw);
   if (lw0 > 0 && lw1 <= lw0) lw1 = 0;
   net->Lw[0] = lw0;
@@ -332,27 +398,17 @@ int network_create(const int *tau, const float *beta, size_t K, size_t S, size_t
   std::vector<int> tile_lw(net->n_tiles, 0);
   for (int c = 0; c < 2; ++c)
     for (int j : net->tiles[c]) tile_lw[j] = net->Lw[c];
+  BP_TRY(net->tile_lw.alloc(tile_lw.size() * sizeof(int)));
+  HIP_TRY(hipMemcpyAsync(net->tile_lw.p, tile_lw.data(), tile_lw.size() * sizeof(int),
+                         hipMemcpyHostToDevice, st));
   for (int c = 0; c < 3; ++c) {
     BP_TRY(net->dtiles[c].alloc(std::max<size_t>(1, net->tiles[c].size()) * sizeof(int)));
     if (!net->tiles[c].empty())
       HIP_TRY(hipMemcpyAsync(net->dtiles[c].p, net->tiles[c].data(),
                              net->tiles[c].size() * sizeof(int), hipMemcpyHostToDevice, st));
   }
-  if (!net->tiles[0].empty() || !net->tiles[1].empty()) {
-    const long long n = (long long)net->n_tiles * S * kTileSources;
-    BP_TRY(net->sched.alloc((size_t)n * (P + 1) * sizeof(int)));
-    DevBuf dlw;
-    BP_TRY(dlw.alloc(tile_lw.size() * sizeof(int)));
-    HIP_TRY(hipMemcpyAsync(dlw.p, tile_lw.data(), tile_lw.size() * sizeof(int),
-                           hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(bp::schedule_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                       st, net->tau.as<int>(), net->beta.as<float>(), net->tile_lo.as<int>(),
-                       dlw.as<int>(), net->K, net->S, net->P, kTileSources, n,
-                       net->sched.as<int>());
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(st));
-  }
   HIP_TRY(hipStreamSynchronize(st));
+  BP_TRY(ensure_schedule(net.get(), BP_F32, nullptr, st));
   out = std::move(net);
   return BP_OK;
 }
@@ -456,12 +512,58 @@ int bp_network_beamform_max(bp_network *net, const void *features, int fmt,
   if (out_of_bounds != 0 && out_of_bounds != 1)
     return fail(BP_EINVAL, "out_of_bounds must be 0 (strict) or 1 (flexible), not %d",
                 out_of_bounds);
-  if (fmt != BP_F32) return fail(BP_EINVAL, "feature format %d not supported", fmt);
-  if (station_scale) return fail(BP_EINVAL, "station_scale applies to narrow formats only");
+  if (fmt != BP_F32 && fmt != BP_F16 && fmt != BP_BF16)
+    return fail(BP_EINVAL, "unknown feature format %d", fmt);
+  if (fmt == BP_F32 && station_scale)
+    return fail(BP_EINVAL, "station_scale applies to the 16-bit formats only");
   BP_TRY(check_shapes(n_samples, net->K, net->S, net->P));
   if (n_samples == 0) return BP_OK;
-  return network_max(net, static_cast<const float *>(features), n_samples, out_of_bounds,
-                     beam_max, beam_argmax, keys, static_cast<hipStream_t>(stream));
+  return network_max(net, features, fmt, station_scale, n_samples, out_of_bounds, beam_max,
+                     beam_argmax, keys, static_cast<hipStream_t>(stream));
+}
+
+size_t bp_parity_stride(size_t n_samples) { return parity_stride(n_samples); }
+
+int bp_quantize_planar(const float *F_spt, size_t n_stations, size_t n_phases, size_t n_samples,
+                       int fmt, float *station_scale, void *out, void *stream) {
+  if (!F_spt || !station_scale || !out) return fail(BP_EINVAL, "NULL argument");
+  if (fmt != BP_F16 && fmt != BP_BF16)
+    return fail(BP_EINVAL, "quantization targets BP_F16 or BP_BF16, not %d", fmt);
+  BP_TRY(check_shapes(n_samples, 0, n_stations, n_phases));
+  const size_t S = n_stations, P = n_phases, T = n_samples;
+  if (S == 0 || P == 0 || T == 0) return BP_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  DevBuf maxbits, dscale;
+  BP_TRY(maxbits.alloc(S * sizeof(unsigned)));
+  BP_TRY(dscale.alloc(S * sizeof(float)));
+  std::vector<unsigned> mb(S, 0u);
+  HIP_TRY(hipMemcpyAsync(maxbits.p, mb.data(), S * sizeof(unsigned), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(bp::station_maxabs_kernel, dim3(64, (unsigned)S), dim3(256), 0, st, F_spt,
+                     (int)P, (long long)T, maxbits.as<unsigned>());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(mb.data(), maxbits.p, S * sizeof(unsigned), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  // power-of-two scales: FP16 maps the station's max |F| into [2^14, 2^15)
+  // (headroom below 65504); BF16 keeps the float32 exponent range: scale 1.
+  // The caller gets the weight multipliers 1/scale (exact).
+  std::vector<float> sigma(S, 1.0f);
+  for (size_t s = 0; s < S; ++s) {
+    const float m = __builtin_bit_cast(float, mb[s]);
+    if (fmt == BP_F16 && m > 0.0f && m <= 3.0e38f) {
+      int e = 0;
+      std::frexp(m, &e);  // m = f * 2^e, f in [0.5, 1)
+      sigma[s] = std::ldexp(1.0f, 15 - e);
+    }
+    station_scale[s] = 1.0f / sigma[s];
+  }
+  HIP_TRY(hipMemcpyAsync(dscale.p, sigma.data(), S * sizeof(float), hipMemcpyHostToDevice, st));
+  const size_t ldq = parity_stride(T);
+  hipLaunchKernelGGL(bp::quantize_kernel, dim3((unsigned)((ldq + 255) / 256), (unsigned)(S * P)),
+                     dim3(256), 0, st, F_spt, (int)P, (long long)T, (long long)ldq,
+                     dscale.as<float>(), fmt, static_cast<unsigned short *>(out));
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(st));
+  return BP_OK;
 }
 
 int bp_network_beamform(bp_network *net, const float *features, size_t n_samples,
@@ -476,10 +578,10 @@ int bp_network_beamform(bp_network *net, const float *features, size_t n_samples
                 source_end, net->K);
   BP_TRY(check_shapes(n_samples, net->K, net->S, net->P));
   if (n_samples == 0) return BP_OK;
-  bp::StackArgs a = make_args(net, features, (int)n_samples, out_of_bounds);
+  bp::StackArgs a = make_args(net, features, BP_F32, (int)n_samples, out_of_bounds);
   a.beam = beam;
-  return launch_stack<true>(net, a, (int)source_begin, (int)source_end,
-                            static_cast<hipStream_t>(stream));
+  return launch_stack<BP_F32, true>(net, a, (int)source_begin, (int)source_end,
+                                    static_cast<hipStream_t>(stream));
 }
 
 int bp_interleaved_to_planar(const float *F_stp, size_t n_stations, size_t n_samples,
@@ -546,8 +648,8 @@ int beamform_max(float *waveform_features, int *time_delays, float *weights_sour
   HIP_TRY(hipMemcpyAsync(bmax.p, beam_max, T * sizeof(float), hipMemcpyHostToDevice, st.s));
   BP_TRY(bp_interleaved_to_planar(f_stp.as<float>(), n_stations, n_samples, n_phases,
                                   f_spt.as<float>(), st.s));
-  BP_TRY(network_max(net.get(), f_spt.as<float>(), T, out_of_bounds, bmax.as<float>(),
-                     barg.as<int>(), keys.as<unsigned long long>(), st.s));
+  BP_TRY(network_max(net.get(), f_spt.as<float>(), BP_F32, nullptr, T, out_of_bounds,
+                     bmax.as<float>(), barg.as<int>(), keys.as<unsigned long long>(), st.s));
   HIP_TRY(hipMemcpyAsync(beam_max, bmax.p, T * sizeof(float), hipMemcpyDeviceToHost, st.s));
   HIP_TRY(hipMemcpyAsync(beam_argmax, barg.p, T * sizeof(int), hipMemcpyDeviceToHost, st.s));
   HIP_TRY(hipStreamSynchronize(st.s));
@@ -591,9 +693,9 @@ int beamform(float *waveform_features, int *time_delays, float *weights_sources,
   hipLaunchKernelGGL(bp::fill_f32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                      st.s, dbeam.as<float>(), n, 0.0f);
   HIP_TRY(hipGetLastError());
-  bp::StackArgs a = make_args(net.get(), f_spt.as<float>(), (int)T, out_of_bounds);
+  bp::StackArgs a = make_args(net.get(), f_spt.as<float>(), BP_F32, (int)T, out_of_bounds);
   a.beam = dbeam.as<float>();
-  BP_TRY(launch_stack<true>(net.get(), a, 0, (int)K, st.s));
+  BP_TRY((launch_stack<BP_F32, true>(net.get(), a, 0, (int)K, st.s)));
   HIP_TRY(hipMemcpyAsync(beam, dbeam.p, K * T * sizeof(float), hipMemcpyDeviceToHost, st.s));
   HIP_TRY(hipStreamSynchronize(st.s));
   return BP_OK;

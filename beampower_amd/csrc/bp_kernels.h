@@ -13,7 +13,10 @@
 namespace bp {
 
 struct StackArgs {
-  const float *F;         // planar features (S, P, T)
+  const void *F;          // features: float32 planar (S, P, T), or for the 16-bit
+                          // formats parity planes (S, P, 2, ldf): x[t] and x[t+1]
+  long long ldf;          // feature row stride in elements (T, or the parity-plane stride)
+  const float *scale;     // per-station weight multiplier of the 16-bit formats (or NULL)
   const int *tau;         // (K, S, P) moveouts
   const float *beta;      // (K, S) source weights
   const int *src_lohi;    // (K, 2) per-source min/max delay over used stations
@@ -110,21 +113,32 @@ __global__ void tile_bounds_kernel(const int *__restrict__ tau,
   }
 }
 
+// Round a float to the nearest bfloat16 (ties to even); returns its bits.
+__host__ __device__ __forceinline__ unsigned bf16_bits(float x) {
+  unsigned u = __float_as_uint(x);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (u >> 16) | ((u & 0xffffu) ? 0x40u : 0u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return u >> 16;
+}
+
 // Per-tile schedule of the LDS kernel.  For tile j, station s and source
-// slot kk (k = j*KT + kk): P LDS byte offsets, then the weight's bits.  The
-// moveout d of phase p lies e = d - tile_lo[j,s,p] samples into the window;
-// the window is kept as 4 copies shifted by 0..3 samples, so the 4 samples
-// starting at (lane sample + e) sit 16-byte aligned in copy e&3, 16*(e>>2)
-// bytes past the lane's own offset.  A copy holds CS = Lw rounded up to 256
-// floats; slot p starts at p*16*CS bytes and copy c 4*CS*c bytes into it.
-// Sources past K and unused stations get weight 0 and the offset P*16*CS of
-// the all-zero block that follows the slots, so the kernel needs no branch
-// for them: fma(0, 0, acc) == acc.
-// Tiles served by the direct kernel (Lw = 0) get offsets 0.
+// slot kk (k = j*KT + kk): P LDS byte offsets, then the weight word.  A
+// window is kept as SPA copies shifted by 0..SPA-1 samples (SPA = 4 for
+// float32 features, 8 for 16-bit ones) with a copy stride of CB bytes; the
+// moveout d of phase p lies e = d - tile_lo[j,s,p] samples into the window,
+// so the SPA samples starting at (lane sample + e) sit 16-byte aligned in
+// copy e % SPA, 16*(e / SPA) bytes past the lane's own offset; slot p holds
+// copies p*SPA .. p*SPA+SPA-1.  The weight word is beta[k,s] * scale[s] as
+// float32 bits, or for bfloat16 features its bf16 rounding in the high half.
+// Sources past K and unused stations get weight 0 and the offset of the
+// all-zero block that follows the slots, so the kernel needs no branch for
+// them: fma(0, 0, acc) == acc.  Tiles served by the direct kernel (Lw = 0)
+// get offsets 0.
 __global__ void schedule_kernel(const int *__restrict__ tau, const float *__restrict__ beta,
                                 const int *__restrict__ tile_lo,
-                                const int *__restrict__ tile_lw, int K, int S, int P,
-                                int KT, long long n, int *__restrict__ sched) {
+                                const int *__restrict__ tile_lw,
+                                const float *__restrict__ scale, int K, int S, int P, int KT,
+                                int fmt, long long n, int *__restrict__ sched) {
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= n) return;
   const int kk = (int)(idx % KT);
@@ -133,22 +147,75 @@ __global__ void schedule_kernel(const int *__restrict__ tau, const float *__rest
   const int tile = (int)(js / S);
   const int k = tile * KT + kk;
   const int Lw = tile_lw[tile];
+  const int spa = fmt == BP_F32 ? 4 : 8, es = fmt == BP_F32 ? 4 : 2;
   const float b = k < K ? beta[(size_t)k * S + s] : 0.0f;
   int *out = sched + idx * (P + 1);
   for (int p = 0; p < P; ++p) {
     int off = 0;
     if (Lw > 0) {
-      const int CS = (Lw + 255) & ~255;  // copy stride, whole 1 KiB rows
-      off = P * 16 * CS;
+      const int row = 64 * spa;
+      const int CB = (Lw + row - 1) / row * row * es;  // copy stride in bytes
+      off = P * spa * CB;
       if (b != 0.0f) {
         const int e =
             tau[((size_t)k * S + s) * P + p] - tile_lo[((size_t)tile * S + s) * P + p];
-        off = p * 16 * CS + (e & 3) * 4 * CS + 16 * (e >> 2);
+        off = (p * spa + e % spa) * CB + 16 * (e / spa);
       }
     }
     out[p] = off;
   }
-  out[P] = __float_as_int(b);
+  const float w = b * (scale ? scale[s] : 1.0f);
+  out[P] = fmt == BP_BF16 ? (int)(bf16_bits(w) << 16) : __float_as_int(w);
+}
+
+// ---------------------------------------------------------------------------
+// Reduced-precision feature storage.
+// ---------------------------------------------------------------------------
+
+// Per-station max |F| over phases and time (float bits are order-preserving
+// for non-negative values, so an integer max of |x| bits is the max).
+__global__ void station_maxabs_kernel(const float *__restrict__ F, int P, long long T,
+                                      unsigned *__restrict__ maxbits) {
+  const int s = blockIdx.y;
+  unsigned m = 0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (long long)P * T;
+       i += (long long)gridDim.x * blockDim.x) {
+    const unsigned u = __float_as_uint(F[(size_t)s * P * T + i]) & 0x7fffffffu;
+    if (u <= 0x7f800000u) m = u > m ? u : m;  // NaN excluded
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned v = __shfl_xor(m, o);
+    m = v > m ? v : m;
+  }
+  if ((threadIdx.x & 63) == 0) atomicMax(maxbits + s, m);
+}
+
+// Scaled 16-bit parity planes: plane 0 holds q(x[t] * scale[s]), plane 1
+// holds q(x[t+1] * scale[s]) (0 past the end), row stride ldq elements.
+// Non-finite inputs saturate to the format's largest finite value (NaN to 0).
+__global__ void quantize_kernel(const float *__restrict__ F, int P, long long T, long long ldq,
+                                const float *__restrict__ scale, int fmt,
+                                unsigned short *__restrict__ out) {
+  const int sp = blockIdx.y;
+  const int s = sp / P;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ldq) return;
+  const float sc = scale[s];
+  unsigned short q[2];
+  for (int par = 0; par < 2; ++par) {
+    const long long tt = t + par;
+    float x = tt < T ? F[(size_t)sp * T + tt] * sc : 0.0f;
+    if (x != x) x = 0.0f;
+    if (fmt == BP_F16) {
+      x = fminf(fmaxf(x, -65504.0f), 65504.0f);
+      q[par] = __builtin_bit_cast(unsigned short, (_Float16)x);
+    } else {
+      x = fminf(fmaxf(x, -3.3895313892515355e38f), 3.3895313892515355e38f);
+      q[par] = (unsigned short)bf16_bits(x);
+    }
+  }
+  out[((size_t)sp * 2 + 0) * ldq + t] = q[0];
+  out[((size_t)sp * 2 + 1) * ldq + t] = q[1];
 }
 
 // ---------------------------------------------------------------------------
@@ -220,22 +287,25 @@ __global__ void finalize_max_kernel(const unsigned long long *__restrict__ keys,
 // Shift-and-stack, LDS-staged (the hot kernel).
 //
 // A work-group of W waves covers one source tile (KT = W*KW sources; wave w
-// owns KW of them) and one time tile of TT = 256*R4 samples.  Lane l owns
-// the 4-sample runs starting at t0 + 4l + 256r (r < R4), so a wave's gather
-// of one (source, station, phase) term is R4 ds_read_b128 of 64 consecutive
-// 16-byte words: conflict-free, and 4 samples per LDS lane-access, which is
-// what the gfx950 LDS pipe needs to reach its bandwidth (it serves 16 lanes
-// per clock at any width up to 16 bytes).
+// owns KW of them) and one time tile of TT = 512 samples.  A feature sample
+// is ES bytes (4 for float32, 2 for the 16-bit storage formats), so one
+// 16-byte LDS access holds SPA = 16/ES consecutive samples.  Lane l owns
+// R = 8/SPA runs of SPA consecutive samples, starting at
+// t0 + SPA*l + 64*SPA*r, so a wave's gather of one (source, station, phase)
+// term is R ds_read_b128 of 64 consecutive 16-byte words: conflict-free, and
+// SPA samples per LDS lane-access, which is what the gfx950 LDS pipe needs
+// to reach its bandwidth (it serves 16 lanes per clock at any width up to
+// 16 bytes).
 //
 // Stations are visited one at a time (ascending, so every (k,t) FMA chain
 // runs in ascending (s, p) order).  For each of the station's P phases the
-// window [t0 + lo, t0 + lo + Lw + 3) of planar features is staged into LDS
-// as 4 copies shifted by 0..3 samples, double-buffered: the next station's
-// global loads are in flight while the current one is consumed.  Positions
-// outside [0, T) are staged as exact zeros.  Accumulators acc[KW][4*R4]
-// stay in registers for the whole tile.  The per-(source, station) LDS
-// offsets and weights come from the precomputed schedule as wave-uniform
-// scalar loads.
+// window [t0 + lo, t0 + lo + Lw + SPA - 1) of planar features is staged into
+// LDS as SPA copies shifted by 0..SPA-1 samples, double-buffered: the next
+// station's global->LDS copies are in flight while the current one is
+// consumed.  Positions outside [0, T) are staged as exact zeros.
+// Accumulators acc[KW][8] stay in registers for the whole tile.  The
+// per-(source, station) LDS offsets and weights come from the precomputed
+// schedule as wave-uniform scalar loads.
 // ---------------------------------------------------------------------------
 
 // Block -> (time tile, listed tile) with XCD locality: work-groups are
@@ -248,6 +318,10 @@ __device__ __forceinline__ int xcd_remap(int b, int n) {
   return (b & 7) * (n8 >> 3) + (b >> 3);
 }
 
+#ifndef BP_PIPE_DEPTH
+#define BP_PIPE_DEPTH 2  // (source, phase) gather steps in flight
+#endif
+
 typedef __attribute__((address_space(1))) void global_void;
 typedef __attribute__((address_space(3))) void lds_void;
 // Read-only per-network tables are read through the constant address space:
@@ -259,156 +333,250 @@ __device__ __forceinline__ const_int *as_const(const int *p) {
   return (const_int *)(p);
 }
 
-// Copy stride of a window of Lw samples: whole 1 KiB chunks (256 floats),
-// so staging always moves complete 64-lane x 16-byte rows.
-__host__ __device__ __forceinline__ int copy_stride(int Lw) { return (Lw + 255) & ~255; }
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
 
-template <int P, int R4, int KW, int W, int MAXLW, bool NONE>
+// Storage formats of the staged features.  Each provides the element size
+// and the accumulation of one 16-byte LDS word (SPA samples) with weight
+// word `bw` from the schedule:
+//   F32  : 4 samples, v_fma_f32 with the float weight;
+//   F16  : 8 samples, v_fma_mix_f32 (f16 sample x f32 weight + f32 acc, one
+//          rounding, no conversion instruction);
+//   BF16 : 8 samples, v_dot2_f32_bf16 against (w, 0) and (0, w): the product
+//          of two bf16 values is exact in f32, so each is one FMA with one
+//          rounding; the weight word holds the bf16 weight in its high half.
+template <int FMT>
+struct Fmt;
+
+template <>
+struct Fmt<BP_F32> {
+  static constexpr int ES = 4, SPA = 4;
+  __device__ __forceinline__ static void mac(float *acc, const uint4 &v, int bw) {
+    const float b = __int_as_float(bw);
+    acc[0] = fmaf(b, __uint_as_float(v.x), acc[0]);
+    acc[1] = fmaf(b, __uint_as_float(v.y), acc[1]);
+    acc[2] = fmaf(b, __uint_as_float(v.z), acc[2]);
+    acc[3] = fmaf(b, __uint_as_float(v.w), acc[3]);
+  }
+};
+
+template <>
+struct Fmt<BP_F16> {
+  static constexpr int ES = 2, SPA = 8;
+  __device__ __forceinline__ static float lo(unsigned w) {
+    return (float)__builtin_bit_cast(_Float16, (unsigned short)(w & 0xffffu));
+  }
+  __device__ __forceinline__ static float hi(unsigned w) {
+    return (float)__builtin_bit_cast(_Float16, (unsigned short)(w >> 16));
+  }
+  __device__ __forceinline__ static void mac(float *acc, const uint4 &v, int bw) {
+    const float b = __int_as_float(bw);
+    const unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      acc[2 * i] = fmaf(lo(w[i]), b, acc[2 * i]);
+      acc[2 * i + 1] = fmaf(hi(w[i]), b, acc[2 * i + 1]);
+    }
+  }
+};
+
+template <>
+struct Fmt<BP_BF16> {
+  static constexpr int ES = 2, SPA = 8;
+  __device__ __forceinline__ static void mac(float *acc, const uint4 &v, int bw) {
+    const bf16x2_t whi = __builtin_bit_cast(bf16x2_t, (unsigned)bw & 0xffff0000u);  // (0, w)
+    const bf16x2_t wlo = __builtin_bit_cast(bf16x2_t, (unsigned)bw >> 16);          // (w, 0)
+    const unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const bf16x2_t x = __builtin_bit_cast(bf16x2_t, w[i]);
+      acc[2 * i] = __builtin_amdgcn_fdot2_f32_bf16(x, wlo, acc[2 * i], false);
+      acc[2 * i + 1] = __builtin_amdgcn_fdot2_f32_bf16(x, whi, acc[2 * i + 1], false);
+    }
+  }
+};
+
+// Copy stride (elements) of a window of Lw samples: whole 1 KiB rows of
+// 64 lanes x 16 bytes, so staging always moves complete rows.
+__host__ __device__ __forceinline__ int copy_stride(int Lw, int spa) {
+  const int row = 64 * spa;
+  return (Lw + row - 1) / row * row;
+}
+
+template <int FMT, int P, int KW, int W, int MAXLW, bool NONE>
 struct StackLds {
   static_assert(P == 1 || P == 2, "the LDS kernel takes one or two phases");
-  static constexpr int TT = 256 * R4;
+  using F = Fmt<FMT>;
+  static constexpr int ES = F::ES;
+  static constexpr int SPA = F::SPA;
+  static constexpr int NC = SPA;          // shifted copies of a window
+  static constexpr int R = 8 / SPA;       // 16-byte words per lane per term
+  static constexpr int TT = 512;
   static constexpr int KT = KW * W;
   static constexpr int NT = 64 * W;
-  static constexpr int NACC = 4 * R4;
-  static constexpr int MAX_LW = MAXLW;         // largest window this instantiation stages
-  static constexpr int NU = (MAX_LW + NT - 1) / NT;  // edge-path staging units per thread
-  static constexpr int ZERO_BYTES = 16 * 64 * R4;  // all-zero block read by unused terms
+  static constexpr int NACC = 8;
+  static constexpr int ZERO_BYTES = 1024 * R;  // all-zero block read by unused terms
 
-  // LDS of one buffer: P slots of 4 copies (stride copy_stride(Lw) floats),
+  // LDS of one buffer: P slots of NC copies (stride copy_stride(Lw) elements),
   // then the zero block.
+  __host__ __device__ static int copy_bytes(int Lw) { return copy_stride(Lw, SPA) * ES; }
   __host__ __device__ static size_t buf_bytes(int Lw) {
-    return (size_t)P * 16 * copy_stride(Lw) + ZERO_BYTES;
+    return (size_t)P * NC * copy_bytes(Lw) + ZERO_BYTES;
   }
   static size_t lds_bytes(int Lw) { return (NONE ? 0 : (size_t)TT * 8) + 2 * buf_bytes(Lw); }
 
-  // Interior tiles: the station's windows go global -> LDS directly
-  // (global_load_lds_dwordx4, no registers).  Row (p, c, j) of a station is
-  // copy c of phase p, quads [64j, 64j + 64): lane l loads
-  // x[w0 + 4m + c .. +3], m = 64j + l (clamped to the window's last quad, so
-  // the padding past the window gets a harmless duplicate), into LDS at
-  // slot p + copy c + 16m.  Row r belongs to wave r % W.  The rows of a wave
-  // are the same for every station, so their LDS offsets and per-lane
-  // element offsets are computed once (RowPlan).
-  static constexpr int NR = (P * 4 * (((MAXLW + 255) & ~255) >> 8) + W - 1) / W;
+  // Global->LDS staging of interior tiles (global_load_lds_dwordx4, no
+  // registers).  Row (p, c, j) is copy c of phase p, 16-byte words
+  // [64j, 64j + 64): lane l moves the SPA samples x[w0 + c + SPA*m ..],
+  // m = 64j + l (clamped to the window's last word, so the padding past the
+  // window gets a harmless duplicate), to LDS at slot p + copy c + 16m.
+  // For 16-bit samples the features are kept twice in HBM, as x[t] and as
+  // x[t+1] ("parity planes"), so every row starts 4-byte aligned: an odd
+  // start w0 + c reads plane 1 at w0 + c - 1.  Row r belongs to wave r % W;
+  // a wave's rows are the same for every station, so their LDS offsets and
+  // per-lane byte offsets are computed once.
+  static constexpr int NR = (P * NC * ((MAXLW + 64 * SPA - 1) / (64 * SPA)) + W - 1) / W;
 
   struct RowPlan {
-    int n;          // rows of this wave
-    int p[NR];      // phase of each row (wave-uniform)
-    int lds[NR];    // LDS byte offset of each row in a buffer (wave-uniform)
-    unsigned eoff[NR];  // this lane's byte offset 4*(4m + c) within the window
+    int n;              // rows of this wave
+    int p[NR];          // phase of the row (wave-uniform)
+    int c[NR];          // copy (shift) of the row (wave-uniform)
+    int lds[NR];        // LDS byte offset of the row in a buffer (wave-uniform)
+    unsigned loff[NR];  // this lane's byte offset 16*m within the copy
   };
 
   __device__ __forceinline__ static void plan_rows(int Lw, int wave, int lane, RowPlan &rp) {
-    const int CS = copy_stride(Lw);
-    const int nch = CS >> 8;
-    const int last_quad = (Lw >> 2) - 1;
-    const int rows = P * 4 * nch;
+    const int CB = copy_bytes(Lw);
+    const int nch = CB >> 10;
+    const int last_word = (Lw + SPA - 1) / SPA - 1;
+    const int rows = P * NC * nch;
     rp.n = 0;
 #pragma unroll
     for (int i = 0; i < NR; ++i) {
       const int row = wave + i * W;
       const int rr = row < rows ? row : 0;
-      const int p = rr / (4 * nch);
-      const int rem = rr - p * 4 * nch;
+      const int p = rr / (NC * nch);
+      const int rem = rr - p * NC * nch;
       const int c = rem / nch;
       const int j = rem - c * nch;
       rp.p[i] = p;
-      rp.lds[i] = p * 16 * CS + c * 4 * CS + 1024 * j;
-      rp.eoff[i] = 4u * (unsigned)(4 * min(64 * j + lane, last_quad) + c);
+      rp.c[i] = c;
+      rp.lds[i] = (p * NC + c) * CB + 1024 * j;
+      rp.loff[i] = 16u * (unsigned)min(64 * j + lane, last_word);
       rp.n += row < rows ? 1 : 0;
     }
+  }
+
+  // Byte address of feature element t of (s, p) in the staging source:
+  // F32: planar (S, P, ldf); 16-bit: parity planes (S, P, 2, ldf), where an
+  // odd t is read as element t - 1 of plane 1 (ldf is even, so the address
+  // is 4-byte aligned).
+  __device__ __forceinline__ static const char *elem_ptr(const StackArgs &a, int s, int p,
+                                                         long long t) {
+    const char *base = reinterpret_cast<const char *>(a.F);
+    if (ES == 4) return base + (((size_t)s * P + p) * a.ldf + t) * 4;
+    const int par = (int)(t & 1);
+    return base + ((((size_t)s * P + p) * 2 + par) * a.ldf + (t - par)) * 2;
   }
 
   __device__ __forceinline__ static void dma_station(const StackArgs &a, const_int *tlo, int s,
                                                      int t0, const RowPlan &rp,
                                                      unsigned char *__restrict__ buf) {
     int lo[P];
-    const float *plane[P];
 #pragma unroll
-    for (int p = 0; p < P; ++p) {
-      lo[p] = tlo[s * P + p];
-      plane[p] = a.F + ((size_t)s * P + p) * a.T + (t0 + lo[p]);
-    }
+    for (int p = 0; p < P; ++p) lo[p] = tlo[s * P + p];
 #pragma unroll
     for (int i = 0; i < NR; ++i) {
       if (i >= rp.n) break;
       const int p = rp.p[i];
       const int lop = P == 1 ? lo[0] : (p == 0 ? lo[0] : lo[P - 1]);
       if (lop == INT_MAX) continue;  // no source of the tile uses station s
-      const char *base =
-          reinterpret_cast<const char *>(P == 1 ? plane[0] : (p == 0 ? plane[0] : plane[P - 1]));
-      __builtin_amdgcn_global_load_lds((global_void *)(base + rp.eoff[i]),
+      const char *base = elem_ptr(a, s, p, (long long)t0 + lop + rp.c[i]);
+      __builtin_amdgcn_global_load_lds((global_void *)(base + rp.loff[i]),
                                        (lds_void *)(buf + rp.lds[i]), 16, 0, 0);
     }
   }
 
-  // Edge tiles (a window crosses [0, T)): guarded loads through registers,
-  // zeros outside [0, T).  Unit u = c*(Lw/4) + m of slot p.
+  // Edge tiles (a window crosses [0, T)): element-wise guarded copies
+  // through registers, zeros outside [0, T).  Unit u = c*words + m of slot p
+  // fills copy c's 16-byte word m with x[w0 + c + SPA*m .. + SPA-1].
+  __device__ __forceinline__ static unsigned load_elem(const StackArgs &a, int s, int p,
+                                                       long long t) {
+    if (t < 0 || t >= a.T) return 0u;
+    const char *base = reinterpret_cast<const char *>(a.F);
+    if (ES == 4)
+      return *reinterpret_cast<const unsigned *>(base + (((size_t)s * P + p) * a.ldf + t) * 4);
+    return *reinterpret_cast<const unsigned short *>(base +
+                                                     (((size_t)s * P + p) * 2 * a.ldf + t) * 2);
+  }
+
   __device__ __forceinline__ static void stage_station_guarded(const StackArgs &a,
                                                                const_int *tlo, int s, int t0,
                                                                int tid, unsigned char *buf) {
-    const int Lw = a.Lw, CS = copy_stride(Lw), quads = Lw >> 2;
+    const int Lw = a.Lw, CB = copy_bytes(Lw), words = (Lw + SPA - 1) / SPA;
 #pragma unroll
     for (int p = 0; p < P; ++p) {
       const int lo = tlo[s * P + p];
       if (lo == INT_MAX) continue;
-      const float *plane = a.F + ((size_t)s * P + p) * a.T;
       const long long w0 = (long long)t0 + lo;
-      for (int u = tid; u < Lw; u += NT) {
-        const int c = u / quads, m = u - c * quads;
-        const long long t = w0 + 4 * m + c;
-        float4 v;
-        v.x = (t + 0 >= 0 && t + 0 < a.T) ? plane[t + 0] : 0.f;
-        v.y = (t + 1 >= 0 && t + 1 < a.T) ? plane[t + 1] : 0.f;
-        v.z = (t + 2 >= 0 && t + 2 < a.T) ? plane[t + 2] : 0.f;
-        v.w = (t + 3 >= 0 && t + 3 < a.T) ? plane[t + 3] : 0.f;
-        *reinterpret_cast<float4 *>(buf + p * 16 * CS + c * 4 * CS + 16 * m) = v;
+      for (int u = tid; u < NC * words; u += NT) {
+        const int c = u / words, m = u - c * words;
+        const long long t = w0 + c + (long long)SPA * m;
+        unsigned q[4];
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+          if (ES == 4) {
+            q[d] = load_elem(a, s, p, t + d);
+          } else {
+            q[d] = load_elem(a, s, p, t + 2 * d) | (load_elem(a, s, p, t + 2 * d + 1) << 16);
+          }
+        }
+        *reinterpret_cast<uint4 *>(buf + (p * NC + c) * CB + 16 * m) =
+            make_uint4(q[0], q[1], q[2], q[3]);
       }
     }
   }
 
   // Operands of one station for this wave's KW sources: KW*(P+1)
-  // wave-uniform words (P byte offsets and the weight per source).
+  // wave-uniform words (P byte offsets and the weight word per source).
   __device__ __forceinline__ static void load_sched(const_int *sg, int (&sv)[KW * (P + 1)]) {
 #pragma unroll
     for (int i = 0; i < KW * (P + 1); ++i) sv[i] = sg[i];
   }
 
-  __device__ __forceinline__ static void gather(const unsigned char *buf, unsigned lb,
-                                                const int *svk, float4 (&v)[P][R4]) {
+  // One gather step is one (source, phase) term: R ds_read_b128.
+  __device__ __forceinline__ static void gather(const unsigned char *buf, unsigned lb, int off,
+                                                uint4 (&v)[R]) {
 #pragma unroll
-    for (int p = 0; p < P; ++p)
-#pragma unroll
-      for (int r = 0; r < R4; ++r)
-        v[p][r] = *reinterpret_cast<const float4 *>(buf + (lb + (unsigned)svk[p]) + 1024 * r);
+    for (int r = 0; r < R; ++r)
+      v[r] = *reinterpret_cast<const uint4 *>(buf + (lb + (unsigned)off) + 1024 * r);
   }
 
-  // One station: straight-line gathers and FMAs, software-pipelined by one
-  // source (the gathers of source kk+1 are in flight during the FMAs of kk).
-  // A fence after each source keeps the compiler from hoisting every gather
-  // of the station at once, which would exhaust the registers: the empty asm
-  // statements take the source's accumulators (its FMAs complete there) and
+  // One station: straight-line gathers and FMAs over the KW*P (source,
+  // phase) steps, software-pipelined DEPTH steps ahead (the gathers of
+  // steps i+1..i+DEPTH are in flight during the FMAs of step i).  A fence
+  // after each step keeps the compiler from hoisting every gather of the
+  // station at once, which would exhaust the registers: the empty asm
+  // statements take the step's accumulators (its FMAs complete there) and
   // the lane's LDS offset (later gathers depend on it, so they stay behind).
+  static constexpr int STEPS = KW * P;
+  static constexpr int DEPTH = BP_PIPE_DEPTH < STEPS ? BP_PIPE_DEPTH : STEPS - 1;
+
   __device__ __forceinline__ static void consume_station(const int (&sv)[KW * (P + 1)],
                                                          const unsigned char *buf, int lane,
                                                          float (&acc)[KW][NACC]) {
     unsigned lb = 16u * (unsigned)lane;
-    float4 v[2][P][R4];
-    gather(buf, lb, sv, v[0]);
+    uint4 v[DEPTH + 1][R];
 #pragma unroll
-    for (int kk = 0; kk < KW; ++kk) {
-      const int c = kk & 1;
-      if (kk + 1 < KW) gather(buf, lb, sv + (kk + 1) * (P + 1), v[c ^ 1]);
-      const float b = __int_as_float(sv[kk * (P + 1) + P]);
+    for (int i = 0; i < DEPTH; ++i)
+      gather(buf, lb, sv[(i / P) * (P + 1) + (i % P)], v[i % (DEPTH + 1)]);
 #pragma unroll
-      for (int p = 0; p < P; ++p)
+    for (int i = 0; i < STEPS; ++i) {
+      const int kk = i / P;
+      const int n = i + DEPTH;
+      if (n < STEPS) gather(buf, lb, sv[(n / P) * (P + 1) + (n % P)], v[n % (DEPTH + 1)]);
+      const int bw = sv[kk * (P + 1) + P];
 #pragma unroll
-        for (int r = 0; r < R4; ++r) {
-          acc[kk][4 * r + 0] = fmaf(b, v[c][p][r].x, acc[kk][4 * r + 0]);
-          acc[kk][4 * r + 1] = fmaf(b, v[c][p][r].y, acc[kk][4 * r + 1]);
-          acc[kk][4 * r + 2] = fmaf(b, v[c][p][r].z, acc[kk][4 * r + 2]);
-          acc[kk][4 * r + 3] = fmaf(b, v[c][p][r].w, acc[kk][4 * r + 3]);
-        }
+      for (int r = 0; r < R; ++r) F::mac(&acc[kk][SPA * r], v[i % (DEPTH + 1)][r], bw);
 #pragma unroll
       for (int j = 0; j < NACC; ++j) asm volatile("" : "+v"(acc[kk][j]));
       asm volatile("" : "+v"(lb));
@@ -431,9 +599,14 @@ struct StackLds {
     consume_station(sv, cur, lane, acc);
   }
 
+  // Sample of accumulator j of a lane: run j / SPA, position j % SPA.
+  __device__ __forceinline__ static int sample_of(int t0, int lane, int j) {
+    return t0 + SPA * lane + 64 * SPA * (j / SPA) + (j % SPA);
+  }
+
   __device__ __forceinline__ static void run(const StackArgs &a, unsigned char *smem) {
     const int Lw = a.Lw;
-    const int CS = copy_stride(Lw);
+    const int CB = copy_bytes(Lw);
     unsigned long long *red = reinterpret_cast<unsigned long long *>(smem);
     unsigned char *stage = smem + (NONE ? 0 : TT * 8);
     const int bbytes = (int)buf_bytes(Lw);
@@ -448,16 +621,16 @@ struct StackLds {
     const_int *tlo = as_const(a.tile_lo) + (size_t)tile * S * P;
     const_int *tall = as_const(a.tile_all) + 8 * tile;
     const int kbase = tile * KT + wave * KW;
-    // staged range of every slot: [t0 + lo, t0 + lo + Lw + 3)
+    // staged range of every slot: [t0 + lo, t0 + lo + Lw + SPA - 1)
     const bool edge = (long long)t0 + tall[0] < 0 ||
-                      (long long)t0 + tall[2] + Lw + 3 > (long long)a.T;
+                      (long long)t0 + tall[2] + Lw + SPA - 1 > (long long)a.T;
 
     if (!NONE)
       for (int i = tid; i < TT; i += NT) red[i] = 0ull;
     for (int i = tid; i < ZERO_BYTES / 16; i += NT) {
-      const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-      *reinterpret_cast<float4 *>(stage + P * 16 * CS + 16 * i) = z;
-      *reinterpret_cast<float4 *>(stage + bbytes + P * 16 * CS + 16 * i) = z;
+      const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+      *reinterpret_cast<uint4 *>(stage + P * NC * CB + 16 * i) = z;
+      *reinterpret_cast<uint4 *>(stage + bbytes + P * NC * CB + 16 * i) = z;
     }
 
     float acc[KW][NACC];
@@ -505,7 +678,7 @@ struct StackLds {
         float *row = a.beam + (size_t)(k - a.k_begin) * a.T;
 #pragma unroll
         for (int j = 0; j < NACC; ++j) {
-          const int t = t0 + 4 * lane + 256 * (j >> 2) + (j & 3);
+          const int t = sample_of(t0, lane, j);
           if (t >= vlo && t < vhi) row[t] = acc[kk][j];
         }
       }
@@ -527,7 +700,7 @@ struct StackLds {
                        vhi);
 #pragma unroll
         for (int j = 0; j < NACC; ++j) {
-          const int t = t0 + 4 * lane + 256 * (j >> 2) + (j & 3);
+          const int t = sample_of(t0, lane, j);
           const float v = acc[kk][j];
           if (t >= vlo && t < vhi && v > bv[j]) {
             bv[j] = v;
@@ -537,8 +710,7 @@ struct StackLds {
       }
 #pragma unroll
       for (int j = 0; j < NACC; ++j)
-        if (bv[j] > -FLT_MAX)
-          atomicMax(&red[4 * lane + 256 * (j >> 2) + (j & 3)], pack_key(bv[j], bk[j]));
+        if (bv[j] > -FLT_MAX) atomicMax(&red[sample_of(0, lane, j)], pack_key(bv[j], bk[j]));
       __syncthreads();
       for (int i = tid; i < TT; i += NT) {
         const unsigned long long key = red[i];
@@ -550,11 +722,14 @@ struct StackLds {
 
 // At least 4 waves per SIMD (VGPRs <= 128): two work-groups per CU, so one
 // computes while the other waits at its per-station barrier.
-template <int P, int R4, int KW, int W, int MAXLW, bool NONE>
-__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4)))
+#ifndef BP_WAVES_PER_EU
+#define BP_WAVES_PER_EU 4
+#endif
+template <int FMT, int P, int KW, int W, int MAXLW, bool NONE>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(BP_WAVES_PER_EU)))
 void stack_lds_kernel(StackArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  StackLds<P, R4, KW, W, MAXLW, NONE>::run(a, smem);
+  StackLds<FMT, P, KW, W, MAXLW, NONE>::run(a, smem);
 }
 
 // ---------------------------------------------------------------------------
@@ -564,7 +739,18 @@ void stack_lds_kernel(StackArgs a) {
 // One thread per time sample; blockIdx.y selects a listed source tile of
 // `chunk` sources.
 // ---------------------------------------------------------------------------
-template <bool NONE>
+// Feature element t of (s, p) as float: float32 planar, or the 16-bit
+// parity plane 0.
+template <int FMT>
+__device__ __forceinline__ float feature(const StackArgs &a, int s, int p, long long t) {
+  if (FMT == BP_F32) return static_cast<const float *>(a.F)[((size_t)s * a.P + p) * a.ldf + t];
+  const unsigned short h =
+      static_cast<const unsigned short *>(a.F)[((size_t)s * a.P + p) * 2 * a.ldf + t];
+  if (FMT == BP_F16) return (float)__builtin_bit_cast(_Float16, h);
+  return __uint_as_float((unsigned)h << 16);
+}
+
+template <int FMT, bool NONE>
 __global__ __launch_bounds__(256) void stack_direct_kernel(StackArgs a, int chunk) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long tile = a.tiles[blockIdx.y];
@@ -579,12 +765,14 @@ __global__ __launch_bounds__(256) void stack_direct_kernel(StackArgs a, int chun
     if (t >= T || t < vlo || t >= vhi) continue;
     float acc = 0.0f;
     for (int s = 0; s < S; ++s) {
-      const float b = a.beta[(size_t)k * S + s];
-      if (b == 0.0f) continue;
+      const float b0 = a.beta[(size_t)k * S + s];
+      if (b0 == 0.0f) continue;
+      float b = b0 * (a.scale ? a.scale[s] : 1.0f);
+      if (FMT == BP_BF16) b = __uint_as_float(bf16_bits(b) << 16);
       for (int p = 0; p < P; ++p) {
         const long long tg = t + a.tau[((size_t)k * S + s) * P + p];
         if (tg < 0 || tg >= T) continue;  // only reachable in flexible mode
-        acc = fmaf(b, a.F[((size_t)s * P + p) * T + tg], acc);
+        acc = fmaf(b, feature<FMT>(a, s, p, tg), acc);
       }
     }
     if (NONE) {

@@ -66,14 +66,25 @@ class Network:
 
     def beamform_max(self, features, out_of_bounds=0, beam_max=None, beam_argmax=None,
                      keys=None, stream=None):
-        """features: planar (S,P,T) float32 CUDA tensor.  Returns device
-        tensors (beam_max (T,), beam_argmax (T,)); asynchronous."""
+        """features: planar (S,P,T) float32 CUDA tensor, or a QuantizedFeatures
+        (16-bit storage).  Returns device tensors (beam_max (T,), beam_argmax
+        (T,)); asynchronous."""
         torch = _torch()
-        S, P, T = features.shape
-        if (S, P) != (self.n_stations, self.n_phases) or features.dtype != torch.float32:
-            raise ValueError("features must be float32 (n_stations, n_phases, n_samples)")
-        features = features.contiguous()
-        dev = features.device
+        fmt, scale = BP_F32, None
+        if isinstance(features, QuantizedFeatures):
+            fmt, T, dev = features.fmt, features.n_samples, features.data.device
+            S, P = features.data.shape[:2]
+            scale = features.weight_scale.ctypes.data
+            ptr = features.data.data_ptr()
+        else:
+            S, P, T = features.shape
+            if features.dtype != torch.float32:
+                raise ValueError("features must be float32 (n_stations, n_phases, n_samples)")
+            features = features.contiguous()
+            dev, ptr = features.device, features.data_ptr()
+        if (S, P) != (self.n_stations, self.n_phases):
+            raise ValueError(f"features are for {(S, P)} stations/phases, network for "
+                             f"{(self.n_stations, self.n_phases)}")
         if beam_max is None:
             beam_max = torch.zeros(T, dtype=torch.float32, device=dev)
         if beam_argmax is None:
@@ -81,7 +92,7 @@ class Network:
         if keys is None:
             keys = torch.empty(T, dtype=torch.int64, device=dev)
         check(self.lib.bp_network_beamform_max(
-            self._h, features.data_ptr(), BP_F32, None, T, int(out_of_bounds),
+            self._h, ptr, fmt, scale, T, int(out_of_bounds),
             beam_max.data_ptr(), beam_argmax.data_ptr(), keys.data_ptr(),
             _stream_ptr(stream)), "bp_network_beamform_max", self.lib)
         return beam_max, beam_argmax
@@ -101,6 +112,30 @@ class Network:
             int(source_begin), source_end, beam.data_ptr(), _stream_ptr(stream)),
             "bp_network_beamform", self.lib)
         return beam
+
+
+class QuantizedFeatures:
+    """16-bit feature storage: parity planes (S, P, 2, ldq) as uint16 on the
+    device (plane 0 x[t]*scale, plane 1 x[t+1]*scale) and the host weight
+    multipliers 1/scale per station."""
+
+    def __init__(self, data, weight_scale, fmt, n_samples):
+        self.data, self.weight_scale, self.fmt, self.n_samples = data, weight_scale, fmt, n_samples
+
+
+def quantize(F_spt, fmt=BP_BF16, stream=None):
+    """Planar (S,P,T) float32 CUDA tensor -> QuantizedFeatures in BP_F16 or
+    BP_BF16 (per-station power-of-two scales)."""
+    torch = _torch()
+    lib = load_library("gpu")
+    S, P, T = F_spt.shape
+    ldq = lib.bp_parity_stride(T)
+    out = torch.empty((S, P, 2, ldq), dtype=torch.int16, device=F_spt.device)
+    wscale = np.empty(S, np.float32)
+    check(lib.bp_quantize_planar(F_spt.contiguous().data_ptr(), S, P, T, int(fmt),
+                                 wscale.ctypes.data, out.data_ptr(), _stream_ptr(stream)),
+          "bp_quantize_planar", lib)
+    return QuantizedFeatures(out, wscale, int(fmt), T)
 
 
 def interleaved_to_planar(F_stp, stream=None):

@@ -167,6 +167,29 @@ def main():
     kernel_ms = kt.total_ms / max(1, kt.launches)
     log(f"step {ms_per_step:.2f} ms, stack kernel {kernel_ms:.2f} ms")
 
+    # reduced-precision feature storage on the same workload
+    precision = {}
+    a32 = barg.cpu().numpy().copy()
+    for name, fmt in (("bf16", bpd.BP_BF16), ("f16", bpd.BP_F16)):
+        Q = bpd.quantize(F, fmt)
+        for _ in range(args.warmup):
+            net.beamform_max(Q, oob, bmax, barg, keys)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            net.beamform_max(Q, oob, bmax, barg, keys)
+        torch.cuda.synchronize()
+        dtq = (time.perf_counter() - t1) / args.steps
+        aq = barg.cpu().numpy()
+        precision[name] = {"value": K * T / dtq, "ms_per_step": dtq * 1e3,
+                           "speedup_vs_f32": (dt / args.steps) / dtq,
+                           "argmax_equal_to_f32": float(np.mean(aq == a32))}
+        log(f"{name}: step {dtq * 1e3:.2f} ms")
+        del Q
+    # restore the FP32 result for the checks below
+    net.beamform_max(F, oob, bmax, barg, keys)
+    torch.cuda.synchronize()
+
     # known answer on the benchmark output itself
     a = barg.cpu().numpy()
     recovered = sum(int(np.array_equal(w.time_delays[a[t_e]], w.time_delays[k_e]))
@@ -218,6 +241,7 @@ def main():
         },
         "events_recovered": f"{recovered}/{len(w.events)}",
         "roofline": roofline,
+        "precision_forms": precision,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

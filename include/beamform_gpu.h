@@ -88,15 +88,22 @@ int bp_network_info(const bp_network *net, int *source_tile, int *max_tile_sprea
  * direct gathers) and the LDS window length of each class (0 = direct). */
 int bp_network_classes(const bp_network *net, int *tiles_per_class, int *window_per_class);
 
-/* reduce='max' on device-resident planar features.
- * features: DEVICE pointer, planar (n_stations, n_phases, n_samples) in the
- *   format `fmt` (BP_F32 here; BP_BF16 / BP_F16 with per-station scales).
- * station_scale: DEVICE pointer (n_stations) float32 multiplier applied to
- *   the station's weights (NULL = 1), used by the narrow formats.
+/* reduce='max' on device-resident features.
+ * features: DEVICE pointer in the format `fmt`:
+ *   BP_F32  planar float32 (n_stations, n_phases, n_samples);
+ *   BP_F16 / BP_BF16  16-bit parity planes as written by bp_quantize_planar:
+ *     (n_stations, n_phases, 2, bp_parity_stride(n_samples)), plane 0 holding
+ *     x[t] and plane 1 x[t+1].
+ * station_scale: HOST pointer (n_stations), 16-bit formats only: multiplier
+ *   of station s's weights, i.e. the inverse of the scale its features were
+ *   stored with (bp_quantize_planar returns it).  NULL = 1.  Powers of two
+ *   keep the folding exact.  Must be NULL for BP_F32.
  * beam_max, beam_argmax: DEVICE pointers (n_samples); beam_max keeps its
  *   previous value where no source is valid.
  * keys: DEVICE scratch of n_samples uint64 (caller-provided workspace).
- * stream: hipStream_t (NULL = default stream).  Asynchronous. */
+ * stream: hipStream_t (NULL = default stream).  Asynchronous, except that
+ *   the first call with a format, or with new station_scale values, builds
+ *   the network's schedule for it synchronously. */
 int bp_network_beamform_max(bp_network *net, const void *features, int fmt,
                             const float *station_scale, size_t n_samples,
                             int out_of_bounds, float *beam_max, int *beam_argmax,
@@ -108,6 +115,20 @@ int bp_network_beamform_max(bp_network *net, const void *features, int fmt,
 int bp_network_beamform(bp_network *net, const float *features, size_t n_samples,
                         int out_of_bounds, size_t source_begin, size_t source_end,
                         float *beam, void *stream);
+
+/* 16-bit feature storage.  bp_parity_stride(T) is the row stride (elements)
+ * of the parity planes.  bp_quantize_planar reads planar float32 features
+ * (n_stations, n_phases, n_samples) on the device, chooses per-station
+ * power-of-two scales (BP_F16: the station's max |F| maps into
+ * [2^14, 2^15); BP_BF16: 1), writes the scaled, rounded parity planes to
+ * `out` (DEVICE, 2 * n_stations * n_phases * bp_parity_stride(T) uint16) and
+ * the weight multipliers 1/scale to station_scale (HOST, n_stations).
+ * Non-finite features saturate to the format's largest finite value (NaN to
+ * 0).  Synchronous. */
+size_t bp_parity_stride(size_t n_samples);
+int bp_quantize_planar(const float *F_spt, size_t n_stations, size_t n_phases,
+                       size_t n_samples, int fmt, float *station_scale, void *out,
+                       void *stream);
 
 /* Layout / prestack kernels on device pointers (asynchronous on `stream`). */
 int bp_interleaved_to_planar(const float *F_stp, size_t n_stations, size_t n_samples,

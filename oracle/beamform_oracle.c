@@ -315,3 +315,45 @@ void oracle_beamform_f64(const double *F, const int *tau, const double *beta,
         }
     }
 }
+
+/* Max/argmax in double over [t_begin, t_end) (same candidate and tie rules
+ * as oracle_beamform_max), plus the runner-up value of each sample so that
+ * callers can tell ties from clear winners.  out_valid[t] = 0 where no
+ * source is a candidate. */
+void oracle_beamform_max_f64(const double *F, const int *tau, const double *beta,
+                             size_t n_samples, size_t n_sources, size_t n_stations,
+                             size_t n_phases, int out_of_bounds, int num_threads,
+                             long long t_begin, long long t_end, double *out_max,
+                             double *out_second, int *out_arg, unsigned char *out_valid,
+                             int *lohi_scratch)
+{
+    int nt = nthreads_or_default(num_threads);
+    delay_bounds_f64(tau, beta, n_sources, n_stations, n_phases, nt, lohi_scratch);
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 4)
+    for (long long t = t_begin; t < t_end; t++) {
+        double best = -DBL_MAX, second = -DBL_MAX;
+        int best_k = 0, any = 0;
+        for (size_t k = 0; k < n_sources; k++) {
+            if (!out_of_bounds &&
+                !strict_valid(t, lohi_scratch[2 * k], lohi_scratch[2 * k + 1], n_samples))
+                continue;
+            double b = stack_f64(F, tau + k * n_stations * n_phases, beta + k * n_stations, t,
+                                 n_samples, n_stations, n_phases, out_of_bounds);
+            if (!(b > -FLT_MAX))
+                continue;
+            any = 1;
+            if (b > best) {
+                second = best;
+                best = b;
+                best_k = (int)k;
+            } else if (b > second) {
+                second = b;
+            }
+        }
+        long long o = t - t_begin;
+        out_max[o] = best;
+        out_second[o] = second;
+        out_arg[o] = best_k;
+        out_valid[o] = (unsigned char)any;
+    }
+}

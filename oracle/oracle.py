@@ -64,8 +64,12 @@ def _oracle(flavour):
         lib.oracle_beamform_f64.argtypes = [_P_D, _P_I, _P_D, _SZ, _SZ, _SZ, _SZ,
                                             ct.c_int, ct.c_int, ct.c_longlong,
                                             ct.c_longlong, _P_D, _P_U8, _P_I]
+        lib.oracle_beamform_max_f64.argtypes = [_P_D, _P_I, _P_D, _SZ, _SZ, _SZ, _SZ,
+                                                ct.c_int, ct.c_int, ct.c_longlong,
+                                                ct.c_longlong, _P_D, _P_D, _P_I, _P_U8, _P_I]
         for name in ("oracle_delay_bounds", "oracle_prestack", "oracle_beamform",
-                     "oracle_beamform_max", "oracle_prestack_f64", "oracle_beamform_f64"):
+                     "oracle_beamform_max", "oracle_prestack_f64", "oracle_beamform_f64",
+                     "oracle_beamform_max_f64"):
             getattr(lib, name).restype = None
         lib._typed = True
     return lib
@@ -172,6 +176,29 @@ def beamform_f64(F, tau, beta, out_of_bounds=0, t_begin=0, t_end=None, num_threa
                                        int(t_begin), t_end, _ptr(beam, _P_D),
                                        _ptr(valid, _P_U8), _ptr(scratch, _P_I))
     return beam, valid.astype(bool)
+
+
+def beamform_max_f64(F, tau, beta, out_of_bounds=0, t_begin=0, t_end=None, num_threads=0):
+    """FP64 max/argmax over [t_begin, t_end) with the runner-up value:
+    returns (max, second, argmax, valid) arrays of length t_end - t_begin."""
+    F = _c(F, np.float64)
+    tau = _c(tau, np.int32)
+    beta = _c(beta, np.float64)
+    S, T, P = F.shape
+    K = tau.shape[0]
+    t_end = T if t_end is None else int(t_end)
+    W = t_end - int(t_begin)
+    mx = np.empty(W, np.float64)
+    sec = np.empty(W, np.float64)
+    arg = np.empty(W, np.int32)
+    valid = np.empty(W, np.uint8)
+    scratch = np.empty(2 * K, np.int32)
+    _oracle("fma").oracle_beamform_max_f64(_ptr(F, _P_D), _ptr(tau, _P_I), _ptr(beta, _P_D),
+                                           T, K, S, P, int(out_of_bounds), _nt(num_threads),
+                                           int(t_begin), t_end, _ptr(mx, _P_D), _ptr(sec, _P_D),
+                                           _ptr(arg, _P_I), _ptr(valid, _P_U8),
+                                           _ptr(scratch, _P_I))
+    return mx, sec, arg, valid.astype(bool)
 
 
 def max_from_beams(beam, valid):

@@ -194,3 +194,23 @@ def test_plain_known_answer():
         k = a[t_e]
         assert np.array_equal(tau[k], tau[k_e])
         assert k == np.flatnonzero((tau == tau[k_e]).all(axis=(1, 2)))[0]
+
+
+@pytest.mark.parametrize("oob", [0, 1])
+def test_fp64_max_vs_bruteforce(oob):
+    """FP64 max/argmax/runner-up over a time range agrees with the NumPy
+    brute force (reduction rule, bounds, and the range offset)."""
+    u, alpha, tau, beta = synthetic.random_case(77, S=5, C=3, P=2, T=90, K=23,
+                                                tau_range=(0, 20), negative_delays=bool(oob))
+    F = bf.prestack(u, alpha)
+    b, v = bf.beams(F, tau, beta, flexible=bool(oob))
+    best, arg, any_ = bf.reduce_max(b, v)
+    mx, sec, a, val = oracle.beamform_max_f64(F, tau, beta.astype(np.float64), oob,
+                                              t_begin=10, t_end=70)
+    assert np.array_equal(val, any_[10:70])
+    np.testing.assert_allclose(mx[val], best[10:70][val], rtol=1e-12)
+    assert np.array_equal(a[val], arg[10:70][val])
+    masked = np.where(v, b, -np.inf)[:, 10:70]
+    srt = np.sort(masked, axis=0)
+    ok = val & np.isfinite(srt[-2])
+    np.testing.assert_allclose(sec[ok], srt[-2][ok], rtol=1e-12)
