@@ -308,14 +308,30 @@ __global__ void finalize_max_kernel(const unsigned long long *__restrict__ keys,
 // schedule as wave-uniform scalar loads.
 // ---------------------------------------------------------------------------
 
-// Block -> (time tile, listed tile) with XCD locality: work-groups are
-// dispatched round-robin over the 8 XCDs, so block b runs on XCD b % 8;
-// XCD x is given the contiguous range [x*N/8, (x+1)*N/8) of the time-major
-// order, so the feature windows of a time tile stay in that XCD's L2.
+// Block -> (time tile, listed tile) with XCD and L2 locality.  Work-groups
+// are dispatched round-robin over the 8 XCDs, so block b runs on XCD b % 8;
+// XCD x is given the contiguous range [x*N/8, (x+1)*N/8) of an order that
+// walks groups of TG consecutive time tiles, all listed source tiles per
+// group and the group's time tiles innermost.  Work-groups running together
+// on an XCD then share both the feature windows of TG time tiles and each
+// source tile's schedule, which stay in that XCD's L2.
+#ifndef BP_TIME_GROUP
+#define BP_TIME_GROUP 8
+#endif
 __device__ __forceinline__ int xcd_remap(int b, int n) {
   const int n8 = n & ~7;
   if (b >= n8) return b;
   return (b & 7) * (n8 >> 3) + (b >> 3);
+}
+
+__device__ __forceinline__ void block_tile(int L, int n_time_tiles, int n_list, int &tt,
+                                           int &li) {
+  constexpr int TG = BP_TIME_GROUP;
+  const int g = L / (TG * n_list);
+  const int gsize = min(TG, n_time_tiles - g * TG);  // last group may be short
+  const int rem = L - g * TG * n_list;
+  li = rem / gsize;
+  tt = g * TG + (rem - li * gsize);
 }
 
 #ifndef BP_PIPE_DEPTH
@@ -614,8 +630,9 @@ struct StackLds {
     const int lane = tid & 63;
     const int wave = uniform(tid >> 6);
     const int L = xcd_remap(blockIdx.x, a.n_time_tiles * a.n_list);
-    const int tt = L / a.n_list;
-    const int tile = as_const(a.tiles)[L - tt * a.n_list];
+    int tt, li;
+    block_tile(L, a.n_time_tiles, a.n_list, tt, li);
+    const int tile = as_const(a.tiles)[li];
     const int t0 = tt * TT;
     const int S = a.S;
     const_int *tlo = as_const(a.tile_lo) + (size_t)tile * S * P;

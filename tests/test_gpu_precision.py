@@ -77,8 +77,11 @@ def test_quantizer_matches_numpy(gpu_lib, fmt):
 @pytest.mark.parametrize("fmt", [BP_F16, BP_BF16])
 @pytest.mark.parametrize("oob", [0, 1])
 def test_16bit_kernel_is_the_fp32_chain_on_dequantized_data(gpu_lib, fmt, oob):
-    """Bit-exact against the oracle on the dequantized features and folded
-    weights (for BF16 the weights are bf16-rounded as the kernel uses them)."""
+    """Against the oracle on the dequantized features and folded weights (for
+    BF16 the weights are bf16-rounded as the kernel uses them).  FP16
+    (v_fma_mix_f32) is the same FMA chain: bit-exact.  BF16 accumulates with
+    v_dot2_f32_bf16, whose rounding is not that of a fused FMA in every case:
+    there the bound is a few float32 ulps of the sum of |terms|."""
     u, alpha, tau, beta = synthetic.random_case(61, S=9, C=3, P=2, T=2500, K=700,
                                                 tau_range=(0, 400), negative_delays=bool(oob))
     F_stp = oracle.prestack(u, alpha)
@@ -90,8 +93,25 @@ def test_16bit_kernel_is_the_fp32_chain_on_dequantized_data(gpu_lib, fmt, oob):
     if fmt == BP_BF16:
         bw = dequant(bf16_round(bw), BP_BF16)
     mo, ao = oracle.beamform_max(Fdq, tau, bw, oob)
-    assert bits_equal(m, mo)
-    assert np.array_equal(a, ao)
+    if fmt == BP_F16:
+        assert bits_equal(m, mo)
+        assert np.array_equal(a, ao)
+        return
+    # sum of |terms| per source and sample bounds the accumulated rounding
+    mag = oracle.beamform(np.abs(Fdq), tau, np.abs(bw), 1).max(axis=0)
+    tol = 64 * np.finfo(np.float32).eps * (mag + 1e-30)
+    assert np.all(np.abs(m - mo) <= tol)
+    b = oracle.beamform(Fdq, tau, bw, oob)
+    T = b.shape[1]
+    if oob == 0:
+        lohi = oracle.delay_bounds(tau, beta).astype(np.int64)
+        t = np.arange(T)[None, :]
+        ok = (lohi[:, :1] <= lohi[:, 1:]) & (t + lohi[:, :1] >= 0) & (t + lohi[:, 1:] < T)
+        b = np.where(ok, b, -np.inf)
+    srt = np.sort(b, axis=0)
+    with np.errstate(invalid="ignore"):
+        clear = np.isfinite(srt[-1]) & ((srt[-1] - srt[-2]) > 2 * tol)
+    assert np.array_equal(a[clear], ao[clear])
 
 
 @pytest.mark.parametrize("fmt", [BP_F16, BP_BF16])
