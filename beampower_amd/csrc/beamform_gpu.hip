@@ -64,12 +64,13 @@ constexpr int kW = BP_W;    // waves per work-group -> source tile KW*W
 constexpr int kTT = 512;    // time tile (samples)
 constexpr int kTileSources = kKW * kW;
 constexpr int kSmallLw = 1024;   // windows of the common tiles
-constexpr int kLargeLw = 2048;   // largest window: 2 phases x copies x 2 buffers in 160 KiB
+constexpr int kLargeLw = bp::kMaxWindow;  // largest window
+constexpr int kLdsPerCU = 160 * 1024;
 constexpr double kSmallShare = 0.97;  // share of tiles the small window must cover
 constexpr int kDirectChunk = kTileSources;
 
-template <int FMT, int P, int MAXLW, bool NONE>
-using Lds = bp::StackLds<FMT, P, kKW, kW, MAXLW, NONE>;
+template <int FMT, int P, int NB, bool NONE>
+using Lds = bp::StackLds<FMT, P, kKW, kW, NB, NONE>;
 
 // Device buffer owned by RAII.
 struct DevBuf {
@@ -147,18 +148,18 @@ bool size_ok(size_t v) { return v <= (size_t)INT32_MAX - 4096; }
 std::once_flag g_attr_once;
 int g_attr_rc = BP_OK;
 
-template <int FMT, int P, int MAXLW, bool NONE>
+template <int FMT, int P, int NB, bool NONE>
 hipError_t set_lds_attr() {
   return hipFuncSetAttribute(
-      reinterpret_cast<const void *>(&bp::stack_lds_kernel<FMT, P, kKW, kW, MAXLW, NONE>),
+      reinterpret_cast<const void *>(&bp::stack_lds_kernel<FMT, P, kKW, kW, NB, NONE>),
       hipFuncAttributeMaxDynamicSharedMemorySize,
-      (int)Lds<FMT, P, MAXLW, NONE>::lds_bytes(MAXLW));
+      std::min(kLdsPerCU, Lds<FMT, P, NB, NONE>::lds_bytes(kLargeLw)));
 }
 
 template <int FMT, bool NONE>
 hipError_t set_lds_attrs() {
-  hipError_t r[] = {set_lds_attr<FMT, 1, kSmallLw, NONE>(), set_lds_attr<FMT, 2, kSmallLw, NONE>(),
-                    set_lds_attr<FMT, 1, kLargeLw, NONE>(), set_lds_attr<FMT, 2, kLargeLw, NONE>()};
+  hipError_t r[] = {set_lds_attr<FMT, 1, 2, NONE>(), set_lds_attr<FMT, 2, 2, NONE>(),
+                    set_lds_attr<FMT, 1, 3, NONE>(), set_lds_attr<FMT, 2, 3, NONE>()};
   for (hipError_t x : r)
     if (x != hipSuccess) return x;
   return hipSuccess;
@@ -254,14 +255,29 @@ int ensure_schedule(bp_network *net, int fmt, const float *scales, hipStream_t s
   return BP_OK;
 }
 
-template <int FMT, int MAXLW, bool NONE>
+// Staging buffers of the LDS kernel: 2 (copies of station s + 1 in flight
+// while station s is consumed) measured faster than 3 on the 50-station,
+// 192k-source, 1 h workload (91.8 vs 97.7 ms); 3 is taken only when asked
+// for at build time and two work-groups still fit a CU's LDS.
+#ifndef BP_STAGE_BUFFERS
+#define BP_STAGE_BUFFERS 2
+#endif
+template <int FMT, int P, bool NONE>
+void launch_lds_p(const bp::StackArgs &a, unsigned blocks, hipStream_t st) {
+  if (BP_STAGE_BUFFERS == 3 && Lds<FMT, P, 3, NONE>::lds_bytes(a.Lw) <= kLdsPerCU / 2)
+    hipLaunchKernelGGL((bp::stack_lds_kernel<FMT, P, kKW, kW, 3, NONE>), dim3(blocks),
+                       dim3(64 * kW), (Lds<FMT, P, 3, NONE>::lds_bytes(a.Lw)), st, a);
+  else
+    hipLaunchKernelGGL((bp::stack_lds_kernel<FMT, P, kKW, kW, 2, NONE>), dim3(blocks),
+                       dim3(64 * kW), (Lds<FMT, P, 2, NONE>::lds_bytes(a.Lw)), st, a);
+}
+
+template <int FMT, bool NONE>
 void launch_lds(int P, const bp::StackArgs &a, unsigned blocks, hipStream_t st) {
   if (P == 1)
-    hipLaunchKernelGGL((bp::stack_lds_kernel<FMT, 1, kKW, kW, MAXLW, NONE>), dim3(blocks),
-                       dim3(64 * kW), (Lds<FMT, 1, MAXLW, NONE>::lds_bytes(a.Lw)), st, a);
+    launch_lds_p<FMT, 1, NONE>(a, blocks, st);
   else
-    hipLaunchKernelGGL((bp::stack_lds_kernel<FMT, 2, kKW, kW, MAXLW, NONE>), dim3(blocks),
-                       dim3(64 * kW), (Lds<FMT, 2, MAXLW, NONE>::lds_bytes(a.Lw)), st, a);
+    launch_lds_p<FMT, 2, NONE>(a, blocks, st);
 }
 
 // Launch the stack kernels for sources [k_begin, k_end): every class's
@@ -285,15 +301,16 @@ int launch_stack(const bp_network *net, bp::StackArgs a, int k_begin, int k_end,
     a.tiles = net->dtiles[cls].as<int>() + (lo - v.begin());
     a.n_list = n;
     if (cls < 2) {
+      // staging offsets within a station are 31-bit
+      const long long planes = FMT == BP_F32 ? 1 : 2, es = FMT == BP_F32 ? 4 : 2;
+      if ((net->P * planes * a.ldf + kLargeLw + 16) * es >= (1LL << 31))
+        return fail(BP_EINVAL, "n_samples %d too large for the LDS kernel", a.T);
       BP_TRY(init_kernel_attrs());
       a.Lw = net->Lw[cls];
       a.n_time_tiles = (a.T + kTT - 1) / kTT;
       const long long blocks = (long long)a.n_time_tiles * n;
       if (blocks > INT32_MAX) return fail(BP_EINVAL, "grid too large (%lld blocks)", blocks);
-      if (cls == 0)
-        launch_lds<FMT, kSmallLw, NONE>(net->P, a, (unsigned)blocks, st);
-      else
-        launch_lds<FMT, kLargeLw, NONE>(net->P, a, (unsigned)blocks, st);
+      launch_lds<FMT, NONE>(net->P, a, (unsigned)blocks, st);
     } else {
       if (n > 65535) return fail(BP_EINVAL, "too many direct-kernel tiles (%d)", n);
       const unsigned gx = (unsigned)((a.T + 255) / 256);
@@ -330,9 +347,10 @@ int network_max(bp_network *net, const void *F, int fmt, const float *scales, si
 
 int round_up8(long long v) { return (int)((v + 7) / 8 * 8); }
 
-// Classify tiles by delay spread and choose the two LDS window lengths
-// (multiples of 8 samples, so a 16-byte row of 16-bit samples never reads
-// past the window's staged range).
+// Classify tiles by delay spread and choose the two LDS window lengths:
+// multiples of 8 samples (whole 16-byte words of 16-bit samples) of at
+// least TT + spread + 1 (the staging start of 16-bit samples is rounded
+// down to even, one sample before the least delay at most).
 void classify_tiles(bp_network *net, const std::vector<int> &tile_all) {
   const int n = net->n_tiles;
   std::vector<int> spreads;
@@ -345,15 +363,15 @@ void classify_tiles(bp_network *net, const std::vector<int> &tile_all) {
   const int q = sorted.empty() ? 0 : sorted[std::min<size_t>(sorted.size() - 1,
                                                    (size_t)(kSmallShare * sorted.size()))];
   // the small window covers kSmallShare of the tiles, or as many as fit
-  const int lw0 = std::min(kSmallLw, round_up8((long long)kTT + q));
-  int lw1 = round_up8((long long)kTT + net->max_spread);
+  const int lw0 = std::min(kSmallLw, round_up8((long long)kTT + q + 1));
+  int lw1 = round_up8((long long)kTT + net->max_spread + 1);
   lw1 = std::min(lw1, kLargeLw);
   if (lw0 > 0 && lw1 <= lw0) lw1 = 0;
   net->Lw[0] = lw0;
   net->Lw[1] = lw1;
   for (int c = 0; c < 3; ++c) net->tiles[c].clear();
   for (int j = 0; j < n; ++j) {
-    const long long need = (long long)kTT + spreads[j];
+    const long long need = (long long)kTT + spreads[j] + 1;
     // a tile with an infinite or NaN weight goes to the direct kernel, which
     // drops out-of-range terms explicitly (b * 0 would be NaN)
     const bool ok = lds && tile_all[8 * j + 4] == 0;

@@ -27,7 +27,7 @@ struct StackArgs {
   int n_list;             // number of entries of `tiles`
   int T, K, S, P;
   int oob;                // 0 strict, 1 flexible
-  int Lw;                 // LDS window length per (s,p) copy, multiple of 4
+  int Lw;                 // LDS window length (samples), multiple of 8
   int n_time_tiles;
   int k_begin, k_end;     // sources written: [k_begin, k_end)
   unsigned long long *keys;  // reduce=max: (T) packed keys
@@ -121,19 +121,27 @@ __host__ __device__ __forceinline__ unsigned bf16_bits(float x) {
   return u >> 16;
 }
 
+// LDS layout of a staged window of Lw samples (Lw a multiple of 8) of one
+// (station, phase): SPA copies shifted by 0..SPA-1 samples (SPA = 4 for
+// float32 features, 8 for 16-bit ones), back to back, each Lw/SPA 16-byte
+// words long: word m of copy c holds the SPA samples starting at
+// w0 + c + SPA*m, w0 = t0 + the staging start.  The Lw words of a phase are
+// rounded up to whole 1 KiB rows (64 lanes x 16 bytes, one staging
+// instruction per wave).
+constexpr int kMaxWindow = 2048;  // largest LDS window (samples)
+__host__ __device__ __forceinline__ int window_slot_bytes(int Lw) { return (Lw + 63) / 64 * 1024; }
+
 // Per-tile schedule of the LDS kernel.  For tile j, station s and source
-// slot kk (k = j*KT + kk): P LDS byte offsets, then the weight word.  A
-// window is kept as SPA copies shifted by 0..SPA-1 samples (SPA = 4 for
-// float32 features, 8 for 16-bit ones) with a copy stride of CB bytes; the
-// moveout d of phase p lies e = d - tile_lo[j,s,p] samples into the window,
+// slot kk (k = j*KT + kk): P LDS byte offsets, then the weight word.  The
+// moveout d of phase p lies e = d - lo samples into the window (lo: the
+// staging start, tile_lo[j,s,p] rounded down to even for 16-bit samples),
 // so the SPA samples starting at (lane sample + e) sit 16-byte aligned in
-// copy e % SPA, 16*(e / SPA) bytes past the lane's own offset; slot p holds
-// copies p*SPA .. p*SPA+SPA-1.  The weight word is beta[k,s] * scale[s] as
-// float32 bits, or for bfloat16 features its bf16 rounding in the high half.
-// Sources past K and unused stations get weight 0 and the offset of the
-// all-zero block that follows the slots, so the kernel needs no branch for
-// them: fma(0, 0, acc) == acc.  Tiles served by the direct kernel (Lw = 0)
-// get offsets 0.
+// copy e % SPA, e / SPA words past the lane's own word.  The weight word is
+// beta[k,s] * scale[s] as float32 bits, or for bfloat16 features its bf16
+// rounding in the high half.  Sources past K and unused stations get weight
+// 0 and the offset of the all-zero block that follows the phase slots, so
+// the kernel needs no branch for them: fma(0, 0, acc) == acc.  Tiles served
+// by the direct kernel (Lw = 0) get offsets 0.
 __global__ void schedule_kernel(const int *__restrict__ tau, const float *__restrict__ beta,
                                 const int *__restrict__ tile_lo,
                                 const int *__restrict__ tile_lw,
@@ -147,19 +155,19 @@ __global__ void schedule_kernel(const int *__restrict__ tau, const float *__rest
   const int tile = (int)(js / S);
   const int k = tile * KT + kk;
   const int Lw = tile_lw[tile];
-  const int spa = fmt == BP_F32 ? 4 : 8, es = fmt == BP_F32 ? 4 : 2;
+  const int spa = fmt == BP_F32 ? 4 : 8;
   const float b = k < K ? beta[(size_t)k * S + s] : 0.0f;
   int *out = sched + idx * (P + 1);
   for (int p = 0; p < P; ++p) {
     int off = 0;
     if (Lw > 0) {
-      const int row = 64 * spa;
-      const int CB = (Lw + row - 1) / row * row * es;  // copy stride in bytes
-      off = P * spa * CB;
+      const int SB = window_slot_bytes(Lw);
+      off = P * SB;
       if (b != 0.0f) {
-        const int e =
-            tau[((size_t)k * S + s) * P + p] - tile_lo[((size_t)tile * S + s) * P + p];
-        off = (p * spa + e % spa) * CB + 16 * (e / spa);
+        int lo = tile_lo[((size_t)tile * S + s) * P + p];
+        lo = fmt == BP_F32 ? lo : (lo & ~1);
+        const int e = tau[((size_t)k * S + s) * P + p] - lo;
+        off = p * SB + 16 * ((e % spa) * (Lw / spa) + e / spa);
       }
     }
     out[p] = off;
@@ -300,9 +308,9 @@ __global__ void finalize_max_kernel(const unsigned long long *__restrict__ keys,
 // Stations are visited one at a time (ascending, so every (k,t) FMA chain
 // runs in ascending (s, p) order).  For each of the station's P phases the
 // window [t0 + lo, t0 + lo + Lw + SPA - 1) of planar features is staged into
-// LDS as SPA copies shifted by 0..SPA-1 samples, double-buffered: the next
-// station's global->LDS copies are in flight while the current one is
-// consumed.  Positions outside [0, T) are staged as exact zeros.
+// LDS as SPA copies shifted by 0..SPA-1 samples, in a ring of NB buffers:
+// the global->LDS copies of station s + NB - 1 are in flight while station
+// s is consumed.  Positions outside [0, T) are staged as exact zeros.
 // Accumulators acc[KW][8] stay in registers for the whole tile.  The
 // per-(source, station) LDS offsets and weights come from the precomputed
 // schedule as wave-uniform scalar loads.
@@ -312,11 +320,12 @@ __global__ void finalize_max_kernel(const unsigned long long *__restrict__ keys,
 // are dispatched round-robin over the 8 XCDs, so block b runs on XCD b % 8;
 // XCD x is given the contiguous range [x*N/8, (x+1)*N/8) of an order that
 // walks groups of TG consecutive time tiles, all listed source tiles per
-// group and the group's time tiles innermost.  Work-groups running together
-// on an XCD then share both the feature windows of TG time tiles and each
-// source tile's schedule, which stay in that XCD's L2.
+// group and the group's time tiles innermost.  With TG = 1 the work-groups
+// running together on an XCD share one time tile, so every station's
+// feature windows are fetched from HBM once and then hit in that XCD's L2;
+// larger groups trade that for reuse of each source tile's schedule.
 #ifndef BP_TIME_GROUP
-#define BP_TIME_GROUP 8
+#define BP_TIME_GROUP 1
 #endif
 __device__ __forceinline__ int xcd_remap(int b, int n) {
   const int n8 = n & ~7;
@@ -338,7 +347,43 @@ __device__ __forceinline__ void block_tile(int L, int n_time_tiles, int n_list, 
 #define BP_PIPE_DEPTH 2  // (source, phase) gather steps in flight
 #endif
 
-typedef __attribute__((address_space(1))) void global_void;
+// Work-group barrier without the memory-model fence of __syncthreads(),
+// which waits for every outstanding global->LDS copy of the wave (the
+// fence makes those copies' LDS writes visible).  Callers order their own
+// LDS traffic with explicit s_waitcnt; the empty asm statements keep the
+// compiler from moving memory accesses across the barrier.
+__device__ __forceinline__ void barrier_no_fence() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+// s_waitcnt vmcnt(n) (expcnt and lgkmcnt left alone) for a wave-uniform n
+// in [0, 15]: the counter field is an immediate, so one branch per value.
+__device__ __forceinline__ void wait_vmem(int n) {
+  switch (n) {
+#define BP_WAIT_CASE(k) \
+  case k:               \
+    __builtin_amdgcn_s_waitcnt(0x0F70 | k); \
+    break;
+    BP_WAIT_CASE(1) BP_WAIT_CASE(2) BP_WAIT_CASE(3) BP_WAIT_CASE(4) BP_WAIT_CASE(5)
+    BP_WAIT_CASE(6) BP_WAIT_CASE(7) BP_WAIT_CASE(8) BP_WAIT_CASE(9) BP_WAIT_CASE(10)
+    BP_WAIT_CASE(11) BP_WAIT_CASE(12) BP_WAIT_CASE(13) BP_WAIT_CASE(14) BP_WAIT_CASE(15)
+#undef BP_WAIT_CASE
+    default:
+      __builtin_amdgcn_s_waitcnt(0x0F70);
+      break;
+  }
+}
+
+typedef __attribute__((address_space(1))) char global_char;
+
+// Raw buffer descriptor (stride 0, dword data format) based at `base`,
+// covering 2 GiB: staging windows lie within it by construction.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t window_rsrc(const global_char *base) {
+  return __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, 0x7fffffff, 0x00020000);
+}
+
 typedef __attribute__((address_space(3))) void lds_void;
 // Read-only per-network tables are read through the constant address space:
 // wave-uniform loads from it are always scalar (s_load), whatever the
@@ -411,110 +456,120 @@ struct Fmt<BP_BF16> {
   }
 };
 
-// Copy stride (elements) of a window of Lw samples: whole 1 KiB rows of
-// 64 lanes x 16 bytes, so staging always moves complete rows.
-__host__ __device__ __forceinline__ int copy_stride(int Lw, int spa) {
-  const int row = 64 * spa;
-  return (Lw + row - 1) / row * row;
-}
-
-template <int FMT, int P, int KW, int W, int MAXLW, bool NONE>
+template <int FMT, int P, int KW, int W, int NB, bool NONE>
 struct StackLds {
   static_assert(P == 1 || P == 2, "the LDS kernel takes one or two phases");
+  static_assert(NB == 2 || NB == 3, "two or three staging buffers");
   using F = Fmt<FMT>;
   static constexpr int ES = F::ES;
   static constexpr int SPA = F::SPA;
   static constexpr int NC = SPA;          // shifted copies of a window
   static constexpr int R = 8 / SPA;       // 16-byte words per lane per term
+  static constexpr int PL = ES == 4 ? 1 : 2;  // feature planes per (s, p) in HBM
   static constexpr int TT = 512;
   static constexpr int KT = KW * W;
   static constexpr int NT = 64 * W;
   static constexpr int NACC = 8;
   static constexpr int ZERO_BYTES = 1024 * R;  // all-zero block read by unused terms
 
-  // LDS of one buffer: P slots of NC copies (stride copy_stride(Lw) elements),
-  // then the zero block.
-  __host__ __device__ static int copy_bytes(int Lw) { return copy_stride(Lw, SPA) * ES; }
-  __host__ __device__ static size_t buf_bytes(int Lw) {
-    return (size_t)P * NC * copy_bytes(Lw) + ZERO_BYTES;
+  // LDS of one buffer: P phase slots (window_slot_bytes each), then the
+  // zero block.  NB buffers form a ring.
+  __host__ __device__ static int buf_bytes(int Lw) {
+    return P * window_slot_bytes(Lw) + ZERO_BYTES;
   }
-  static size_t lds_bytes(int Lw) { return (NONE ? 0 : (size_t)TT * 8) + 2 * buf_bytes(Lw); }
+  __host__ __device__ static int lds_bytes(int Lw) {
+    return (NONE ? 0 : TT * 8) + NB * buf_bytes(Lw);
+  }
 
-  // Global->LDS staging of interior tiles (global_load_lds_dwordx4, no
-  // registers).  Row (p, c, j) is copy c of phase p, 16-byte words
-  // [64j, 64j + 64): lane l moves the SPA samples x[w0 + c + SPA*m ..],
-  // m = 64j + l (clamped to the window's last word, so the padding past the
-  // window gets a harmless duplicate), to LDS at slot p + copy c + 16m.
-  // For 16-bit samples the features are kept twice in HBM, as x[t] and as
-  // x[t+1] ("parity planes"), so every row starts 4-byte aligned: an odd
-  // start w0 + c reads plane 1 at w0 + c - 1.  Row r belongs to wave r % W;
-  // a wave's rows are the same for every station, so their LDS offsets and
-  // per-lane byte offsets are computed once.
-  static constexpr int NR = (P * NC * ((MAXLW + 64 * SPA - 1) / (64 * SPA)) + W - 1) / W;
+  // Global->LDS staging of interior tiles (buffer_load_dwordx4 ... lds, no
+  // registers).  A phase slot is ceil(Lw / 64) rows of 64 lanes x 16 bytes;
+  // row j is slot words [64j, 64j + 64), and lane l moves word
+  // u = 64j + l (clamped to the last word Lw - 1, so the slot padding gets
+  // a harmless duplicate).  Wave w moves rows j = w, w + W, ... of every
+  // phase.  A row's lane offsets from the window start do not depend on
+  // the phase or the station, so they are computed once; a row then costs
+  // one instruction with the station's descriptor, the phase's scalar
+  // offset and the lane's offset.  For 16-bit samples the features are kept
+  // twice in HBM, as x[t] and as x[t+1] ("parity planes"), and the window
+  // start is even: copy c then reads plane (c & 1) at the even element
+  // c - (c & 1) + SPA*m, so every row is 4-byte aligned.
+  static constexpr int NRMAX = (kMaxWindow / 64 + W - 1) / W;  // rows per phase and wave
 
   struct RowPlan {
-    int n;              // rows of this wave
-    int p[NR];          // phase of the row (wave-uniform)
-    int c[NR];          // copy (shift) of the row (wave-uniform)
-    int lds[NR];        // LDS byte offset of the row in a buffer (wave-uniform)
-    unsigned loff[NR];  // this lane's byte offset 16*m within the copy
+    int n;                 // rows of this wave per phase
+    unsigned voff[NRMAX];  // this lane's byte offset from the window start, row wave + i*W
   };
 
-  __device__ __forceinline__ static void plan_rows(int Lw, int wave, int lane, RowPlan &rp) {
-    const int CB = copy_bytes(Lw);
-    const int nch = CB >> 10;
-    const int last_word = (Lw + SPA - 1) / SPA - 1;
-    const int rows = P * NC * nch;
-    rp.n = 0;
+  __device__ __forceinline__ static void plan_rows(int Lw, long long ldf, int wave, int lane,
+                                                   RowPlan &rp) {
+    const int nrp = (Lw + 63) / 64;
+    const int nw = Lw / SPA;
+    rp.n = (nrp - wave + W - 1) / W;
 #pragma unroll
-    for (int i = 0; i < NR; ++i) {
-      const int row = wave + i * W;
-      const int rr = row < rows ? row : 0;
-      const int p = rr / (NC * nch);
-      const int rem = rr - p * NC * nch;
-      const int c = rem / nch;
-      const int j = rem - c * nch;
-      rp.p[i] = p;
-      rp.c[i] = c;
-      rp.lds[i] = (p * NC + c) * CB + 1024 * j;
-      rp.loff[i] = 16u * (unsigned)min(64 * j + lane, last_word);
-      rp.n += row < rows ? 1 : 0;
+    for (int i = 0; i < NRMAX; ++i) {
+      const int j = wave + i * W;
+      const int u = min(64 * j + lane, Lw - 1);
+      const int c = u / nw, m = u - c * nw;
+      if (ES == 4) {
+        rp.voff[i] = 4u * (unsigned)(c + SPA * m);
+      } else {
+        const int par = c & 1;
+        rp.voff[i] = (unsigned)par * 2u * (unsigned)ldf + 2u * (unsigned)(c - par + SPA * m);
+      }
     }
   }
 
-  // Byte address of feature element t of (s, p) in the staging source:
-  // F32: planar (S, P, ldf); 16-bit: parity planes (S, P, 2, ldf), where an
-  // odd t is read as element t - 1 of plane 1 (ldf is even, so the address
-  // is 4-byte aligned).
-  __device__ __forceinline__ static const char *elem_ptr(const StackArgs &a, int s, int p,
-                                                         long long t) {
-    const char *base = reinterpret_cast<const char *>(a.F);
-    if (ES == 4) return base + (((size_t)s * P + p) * a.ldf + t) * 4;
-    const int par = (int)(t & 1);
-    return base + ((((size_t)s * P + p) * 2 + par) * a.ldf + (t - par)) * 2;
+  // Staging start of (s, p) relative to t0: the least delay of the tile's
+  // users of the phase at the station (the tile's least delay when no
+  // source uses the station, which keeps the copies in range), rounded down
+  // to even for 16-bit samples.
+  __device__ __forceinline__ static int stage_lo(const_int *tlo, int tall0, int s, int p) {
+    int lo = tlo[s * P + p];
+    lo = lo == INT_MAX ? tall0 : lo;
+    return ES == 4 ? lo : (lo & ~1);
   }
 
-  __device__ __forceinline__ static void dma_station(const StackArgs &a, const_int *tlo, int s,
-                                                     int t0, const RowPlan &rp,
+  // A station's staging sources: a buffer descriptor based at plane 0 of
+  // phase 0 at t0 + the tile's least staging start, and each phase's byte
+  // offset from there (non-negative, below 2^31 as the host checks).
+  struct StationSrc {
+    __amdgpu_buffer_rsrc_t rs;
+    unsigned soff[P];
+  };
+
+  __device__ __forceinline__ static void station_src(const StackArgs &a, const_int *tlo,
+                                                     int tall0, int s, int t0,
+                                                     StationSrc &src) {
+    const int lo0 = ES == 4 ? tall0 : (tall0 & ~1);
+    src.rs = window_rsrc((const global_char *)(a.F) +
+                         (((long long)s * P * PL) * a.ldf + t0 + lo0) * ES);
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+      src.soff[p] = (unsigned)(((long long)p * PL * a.ldf + stage_lo(tlo, tall0, s, p) - lo0) * ES);
+  }
+
+  // The copies are buffer loads to LDS (MUBUF): unlike the FLAT-encoded
+  // global load to LDS, the compiler then keeps its waits for the gathers
+  // exact (lgkmcnt(n)) while copies are in flight instead of draining all
+  // outstanding LDS reads at every wait.
+  __device__ __forceinline__ static void dma_station(const StationSrc &src, const RowPlan &rp,
+                                                     int SB, int wave,
                                                      unsigned char *__restrict__ buf) {
-    int lo[P];
 #pragma unroll
-    for (int p = 0; p < P; ++p) lo[p] = tlo[s * P + p];
+    for (int p = 0; p < P; ++p) {
 #pragma unroll
-    for (int i = 0; i < NR; ++i) {
-      if (i >= rp.n) break;
-      const int p = rp.p[i];
-      const int lop = P == 1 ? lo[0] : (p == 0 ? lo[0] : lo[P - 1]);
-      if (lop == INT_MAX) continue;  // no source of the tile uses station s
-      const char *base = elem_ptr(a, s, p, (long long)t0 + lop + rp.c[i]);
-      __builtin_amdgcn_global_load_lds((global_void *)(base + rp.loff[i]),
-                                       (lds_void *)(buf + rp.lds[i]), 16, 0, 0);
+      for (int i = 0; i < NRMAX; ++i) {
+        if (i >= rp.n) break;
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(
+            src.rs, (lds_void *)(buf + p * SB + 1024 * (wave + i * W)), 16, rp.voff[i],
+            src.soff[p], 0, 0);
+      }
     }
   }
 
   // Edge tiles (a window crosses [0, T)): element-wise guarded copies
-  // through registers, zeros outside [0, T).  Unit u = c*words + m of slot p
-  // fills copy c's 16-byte word m with x[w0 + c + SPA*m .. + SPA-1].
+  // through registers, zeros outside [0, T).  Slot word u = c*(Lw/SPA) + m
+  // of phase p holds x[w0 + c + SPA*m .. + SPA-1].
   __device__ __forceinline__ static unsigned load_elem(const StackArgs &a, int s, int p,
                                                        long long t) {
     if (t < 0 || t >= a.T) return 0u;
@@ -528,14 +583,14 @@ struct StackLds {
   __device__ __forceinline__ static void stage_station_guarded(const StackArgs &a,
                                                                const_int *tlo, int s, int t0,
                                                                int tid, unsigned char *buf) {
-    const int Lw = a.Lw, CB = copy_bytes(Lw), words = (Lw + SPA - 1) / SPA;
+    const int Lw = a.Lw, nw = Lw / SPA, SB = window_slot_bytes(Lw);
 #pragma unroll
     for (int p = 0; p < P; ++p) {
-      const int lo = tlo[s * P + p];
-      if (lo == INT_MAX) continue;
-      const long long w0 = (long long)t0 + lo;
-      for (int u = tid; u < NC * words; u += NT) {
-        const int c = u / words, m = u - c * words;
+      const int lo0 = tlo[s * P + p];
+      if (lo0 == INT_MAX) continue;  // no source of the tile uses (s, p)
+      const long long w0 = (long long)t0 + (ES == 4 ? lo0 : (lo0 & ~1));
+      for (int u = tid; u < Lw; u += NT) {
+        const int c = u / nw, m = u - c * nw;
         const long long t = w0 + c + (long long)SPA * m;
         unsigned q[4];
 #pragma unroll
@@ -546,8 +601,7 @@ struct StackLds {
             q[d] = load_elem(a, s, p, t + 2 * d) | (load_elem(a, s, p, t + 2 * d + 1) << 16);
           }
         }
-        *reinterpret_cast<uint4 *>(buf + (p * NC + c) * CB + 16 * m) =
-            make_uint4(q[0], q[1], q[2], q[3]);
+        *reinterpret_cast<uint4 *>(buf + p * SB + 16 * u) = make_uint4(q[0], q[1], q[2], q[3]);
       }
     }
   }
@@ -600,18 +654,17 @@ struct StackLds {
     }
   }
 
-  // One pipeline step: start the next station's global->LDS copies into
+  // One pipeline step: start a later station's global->LDS copies into
   // `nxt`, then consume the current station from `cur`.  Both buffers are
   // restrict-qualified in one scope, so the compiler knows the copies cannot
   // alias the gathers and does not wait for them before reading `cur`.
-  __device__ __forceinline__ static void station_step(const StackArgs &a, const_int *tlo,
-                                                      int s_next, int t0, bool dma,
-                                                      const RowPlan &rp,
+  __device__ __forceinline__ static void station_step(bool dma, const StationSrc &src,
+                                                      const RowPlan &rp, int SB, int wave,
                                                       const int (&sv)[KW * (P + 1)],
                                                       const unsigned char *__restrict__ cur,
                                                       unsigned char *__restrict__ nxt,
                                                       int lane, float (&acc)[KW][NACC]) {
-    if (dma) dma_station(a, tlo, s_next, t0, rp, nxt);
+    if (dma) dma_station(src, rp, SB, wave, nxt);
     consume_station(sv, cur, lane, acc);
   }
 
@@ -622,10 +675,10 @@ struct StackLds {
 
   __device__ __forceinline__ static void run(const StackArgs &a, unsigned char *smem) {
     const int Lw = a.Lw;
-    const int CB = copy_bytes(Lw);
+    const int SB = window_slot_bytes(Lw);
     unsigned long long *red = reinterpret_cast<unsigned long long *>(smem);
     unsigned char *stage = smem + (NONE ? 0 : TT * 8);
-    const int bbytes = (int)buf_bytes(Lw);
+    const int bbytes = buf_bytes(Lw);
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = uniform(tid >> 6);
@@ -637,17 +690,18 @@ struct StackLds {
     const int S = a.S;
     const_int *tlo = as_const(a.tile_lo) + (size_t)tile * S * P;
     const_int *tall = as_const(a.tile_all) + 8 * tile;
+    const int tall0 = tall[0];
     const int kbase = tile * KT + wave * KW;
-    // staged range of every slot: [t0 + lo, t0 + lo + Lw + SPA - 1)
-    const bool edge = (long long)t0 + tall[0] < 0 ||
+    // staged range of every slot: [t0 + lo, t0 + lo + Lw + SPA - 1), with
+    // lo one lower at most for 16-bit samples (even start)
+    const bool edge = (long long)t0 + tall0 - 1 < 0 ||
                       (long long)t0 + tall[2] + Lw + SPA - 1 > (long long)a.T;
 
     if (!NONE)
       for (int i = tid; i < TT; i += NT) red[i] = 0ull;
-    for (int i = tid; i < ZERO_BYTES / 16; i += NT) {
-      const uint4 z = make_uint4(0u, 0u, 0u, 0u);
-      *reinterpret_cast<uint4 *>(stage + P * NC * CB + 16 * i) = z;
-      *reinterpret_cast<uint4 *>(stage + bbytes + P * NC * CB + 16 * i) = z;
+    for (int i = tid; i < NB * (ZERO_BYTES / 16); i += NT) {
+      const int b = i / (ZERO_BYTES / 16), w = i - b * (ZERO_BYTES / 16);
+      *reinterpret_cast<uint4 *>(stage + b * bbytes + P * SB + 16 * w) = make_uint4(0u, 0u, 0u, 0u);
     }
 
     float acc[KW][NACC];
@@ -659,28 +713,57 @@ struct StackLds {
     // schedule rows of this wave: station s at sched_w + s * KT * (P + 1)
     const_int *sched_w = as_const(a.sched) + ((size_t)tile * S * KT + wave * KW) * (P + 1);
     int sv[KW * (P + 1)];
-    load_sched(sched_w, sv);
-    RowPlan rp;
-    plan_rows(Lw, wave, lane, rp);
-    if (edge)
-      stage_station_guarded(a, tlo, 0, t0, tid, stage);
-    else
-      dma_station(a, tlo, 0, t0, rp, stage);
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    for (int s = 0; s < S; ++s) {
-      const unsigned char *cur = stage + (s & 1) * bbytes;
-      unsigned char *nxt = stage + ((s + 1) & 1) * bbytes;
-      const bool more = s + 1 < S;
-      station_step(a, tlo, s + 1, t0, more && !edge, rp, sv, cur, nxt, lane, acc);
-      if (more) {
-        // next station's operands: the scalar loads overlap the barrier
-        load_sched(sched_w + (size_t)(s + 1) * KT * (P + 1), sv);
-        if (edge) stage_station_guarded(a, tlo, s + 1, t0, tid, nxt);
+    if (edge) {
+      // synchronous: stage, barrier, consume, barrier
+      for (int s = 0; s < S; ++s) {
+        load_sched(sched_w + (size_t)s * KT * (P + 1), sv);
+        stage_station_guarded(a, tlo, s, t0, tid, stage);
+        __syncthreads();
+        consume_station(sv, stage, lane, acc);
+        __syncthreads();
       }
-      // the station's LDS-DMA writes must land before the barrier
-      __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
-      __syncthreads();
+    } else {
+      // Ring of NB buffers: the copies of station s + NB - 1 are issued
+      // while station s is consumed.  Every wave issues P * rp.n rows per
+      // station, so at the end of station s waiting until at most that
+      // many rows per station issued after s + 1 are in flight means the
+      // wave's rows of station s + 1 have landed; the barrier then makes
+      // every wave's rows visible and frees buffer s for reuse.
+      RowPlan rp;
+      plan_rows(Lw, a.ldf, wave, lane, rp);
+      const int rows = P * rp.n;
+      StationSrc src;
+      const int npro = min(NB - 1, S);
+      __syncthreads();  // zero blocks written
+      for (int s = 0; s < npro; ++s) {
+        station_src(a, tlo, tall0, s, t0, src);
+        dma_station(src, rp, SB, wave, stage + s * bbytes);
+      }
+      wait_vmem(rows * (npro - 1));
+      barrier_no_fence();
+      load_sched(sched_w, sv);
+      if (NB - 1 < S) station_src(a, tlo, tall0, NB - 1, t0, src);
+      int ci = 0;  // ring index of station s
+      for (int s = 0; s < S; ++s) {
+        const int sd = s + NB - 1;
+        const int di = ci == 0 ? NB - 1 : ci - 1;  // ring index of station sd
+        station_step(sd < S, src, rp, SB, wave, sv, stage + ci * bbytes, stage + di * bbytes,
+                     lane, acc);
+        if (s + 1 < S) {
+          // next station's operands and staging sources: the scalar loads
+          // overlap the barrier
+          load_sched(sched_w + (size_t)(s + 1) * KT * (P + 1), sv);
+          if (sd + 1 < S) station_src(a, tlo, tall0, sd + 1, t0, src);
+        }
+        // every gather of station s has been consumed by its FMAs, so only
+        // the copies need waiting for
+        if (NB == 2)
+          wait_vmem(0);
+        else
+          wait_vmem(rows * max(0, min(sd, S - 1) - (s + 1)));
+        barrier_no_fence();
+        ci = ci + 1 == NB ? 0 : ci + 1;
+      }
     }
 
     if (NONE) {
@@ -742,11 +825,11 @@ struct StackLds {
 #ifndef BP_WAVES_PER_EU
 #define BP_WAVES_PER_EU 4
 #endif
-template <int FMT, int P, int KW, int W, int MAXLW, bool NONE>
+template <int FMT, int P, int KW, int W, int NB, bool NONE>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(BP_WAVES_PER_EU)))
 void stack_lds_kernel(StackArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  StackLds<FMT, P, KW, W, MAXLW, NONE>::run(a, smem);
+  StackLds<FMT, P, KW, W, NB, NONE>::run(a, smem);
 }
 
 // ---------------------------------------------------------------------------
