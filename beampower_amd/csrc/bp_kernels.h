@@ -379,9 +379,16 @@ __device__ __forceinline__ void wait_vmem(int n) {
 typedef __attribute__((address_space(1))) char global_char;
 
 // Raw buffer descriptor (stride 0, dword data format) based at `base`,
-// covering 2 GiB: staging windows lie within it by construction.
+// covering 2 GiB: staging windows lie within it by construction.  The base
+// is wave-uniform; passing it through readfirstlane states so, which keeps
+// the descriptor in scalar registers (a descriptor the compiler believes
+// divergent costs a loop over lanes at every load).
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t window_rsrc(const global_char *base) {
-  return __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, 0x7fffffff, 0x00020000);
+  const unsigned long long b = (unsigned long long)base;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+  return __builtin_amdgcn_make_buffer_rsrc((void *)(((unsigned long long)hi << 32) | lo),
+                                           (short)0, 0x7fffffff, 0x00020000);
 }
 
 typedef __attribute__((address_space(3))) void lds_void;
@@ -545,7 +552,8 @@ struct StackLds {
                          (((long long)s * P * PL) * a.ldf + t0 + lo0) * ES);
 #pragma unroll
     for (int p = 0; p < P; ++p)
-      src.soff[p] = (unsigned)(((long long)p * PL * a.ldf + stage_lo(tlo, tall0, s, p) - lo0) * ES);
+      src.soff[p] = __builtin_amdgcn_readfirstlane(
+          (unsigned)(((long long)p * PL * a.ldf + stage_lo(tlo, tall0, s, p) - lo0) * ES));
   }
 
   // The copies are buffer loads to LDS (MUBUF): unlike the FLAT-encoded
