@@ -54,7 +54,15 @@ GPU_SYMBOLS = (
     "bp_interleaved_to_planar", "bp_prestack_planar",
     "bp_parity_stride", "bp_quantize_planar",
     "bp_profiling", "bp_profiling_read",
+    "bp_batch_create", "bp_batch_launch", "bp_batch_graph_launch", "bp_batch_tune",
+    "bp_batch_get_config", "bp_batch_set_config", "bp_batch_destroy",
 )
+
+
+class BatchConfig(ct.Structure):
+    """bp_batch_config of include/beamform_gpu.h."""
+    _fields_ = [("time_group", ct.c_int), ("stage_buffers", ct.c_int),
+                ("blocks_per_cu", ct.c_int), ("ms", ct.c_float)]
 
 
 def _declare_gpu_extension(lib):
@@ -76,6 +84,15 @@ def _declare_gpu_extension(lib):
     lib.bp_quantize_planar.argtypes = [vp, _SZ, _SZ, _SZ, ct.c_int, vp, vp, vp]
     lib.bp_profiling.argtypes = [ct.c_int]
     lib.bp_profiling_read.argtypes = [ct.POINTER(ct.c_double), ct.POINTER(ct.c_longlong)]
+    lib.bp_batch_create.argtypes = [ct.c_int, vp, vp, ct.c_int, vp, vp, vp, vp, vp, vp,
+                                    ct.POINTER(vp)]
+    lib.bp_batch_launch.argtypes = [vp, vp]
+    lib.bp_batch_graph_launch.argtypes = [vp, vp]
+    lib.bp_batch_tune.argtypes = [vp, ct.c_int, ct.POINTER(BatchConfig), ct.c_int, _P_I,
+                                  ct.POINTER(BatchConfig), vp]
+    lib.bp_batch_get_config.argtypes = [vp, ct.POINTER(BatchConfig)]
+    lib.bp_batch_set_config.argtypes = [vp, ct.POINTER(BatchConfig)]
+    lib.bp_batch_destroy.argtypes = [vp]
     for name in GPU_SYMBOLS:
         if name not in ("bp_last_error", "bp_version", "bp_parity_stride"):
             getattr(lib, name).restype = ct.c_int

@@ -7,6 +7,9 @@
 // engine on device-resident buffers.
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <tuple>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
@@ -66,6 +69,8 @@ constexpr int kTileSources = kKW * kW;
 constexpr int kSmallLw = 1024;   // windows of the common tiles
 constexpr int kLargeLw = bp::kMaxWindow;  // largest window
 constexpr int kLdsPerCU = 160 * 1024;
+constexpr int kNumCU = 256;        // MI355X
+constexpr int kTimeGroup = 1;      // block order: time-major (see bp_kernels.h, block_tile)
 constexpr double kSmallShare = 0.97;  // share of tiles the small window must cover
 constexpr int kDirectChunk = kTileSources;
 
@@ -221,6 +226,7 @@ bp::StackArgs make_args(const bp_network *net, const void *F, int fmt, int T, in
   a.S = net->S;
   a.P = net->P;
   a.oob = oob;
+  a.time_group = kTimeGroup;
   return a;
 }
 
@@ -447,6 +453,160 @@ void degenerate_max(size_t T, size_t K, int oob, float *bmax, int *barg) {
     if (oob == 1 && K > 0) bmax[t] = 0.0f;
     barg[t] = 0;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Batched form: many windows (network, features, length, outputs) in one
+// persistent launch per LDS window class.
+// ---------------------------------------------------------------------------
+
+template <int FMT, int P, int NB>
+hipError_t set_batch_attr() {
+  return hipFuncSetAttribute(
+      reinterpret_cast<const void *>(&bp::stack_batch_kernel<FMT, P, kKW, kW, NB>),
+      hipFuncAttributeMaxDynamicSharedMemorySize,
+      std::min(kLdsPerCU, Lds<FMT, P, NB, false>::lds_bytes(kLargeLw)));
+}
+
+std::once_flag g_batch_attr_once;
+int g_batch_attr_rc = BP_OK;
+
+int init_batch_attrs() {
+  std::call_once(g_batch_attr_once, [] {
+    hipError_t r[] = {
+        set_batch_attr<BP_F32, 1, 2>(),  set_batch_attr<BP_F32, 2, 2>(),
+        set_batch_attr<BP_F32, 1, 3>(),  set_batch_attr<BP_F32, 2, 3>(),
+        set_batch_attr<BP_F16, 1, 2>(),  set_batch_attr<BP_F16, 2, 2>(),
+        set_batch_attr<BP_F16, 1, 3>(),  set_batch_attr<BP_F16, 2, 3>(),
+        set_batch_attr<BP_BF16, 1, 2>(), set_batch_attr<BP_BF16, 2, 2>(),
+        set_batch_attr<BP_BF16, 1, 3>(), set_batch_attr<BP_BF16, 2, 3>()};
+    for (hipError_t e : r)
+      if (e != hipSuccess) {
+        g_batch_attr_rc = fail(BP_EHIP, "hipFuncSetAttribute: %s", hipGetErrorString(e));
+        break;
+      }
+  });
+  return g_batch_attr_rc;
+}
+
+}  // namespace
+
+struct bp_batch {
+  int fmt = BP_F32, P = 0, n = 0, max_T = 0;
+  bp_batch_config cfg{kTimeGroup, 2, 2, 0.0f};
+  // per LDS class: window descriptors (host copy and device), item count, largest window
+  std::vector<bp::BatchWindow> wins[2];
+  DevBuf dwins[2];
+  long long n_items[2] = {0, 0};
+  int max_lw[2] = {0, 0};
+  std::vector<bp::StackArgs> direct;  // per window with direct-kernel tiles
+  std::vector<int> direct_n;
+  DevBuf dkeys;
+  hipStream_t cap = nullptr;  // private stream for graph capture
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~bp_batch() { drop_graph(); if (cap) (void)hipStreamDestroy(cap); }
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+};
+
+namespace {
+
+// Tuning results of this process, keyed by the batch's shape.
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, long long, long long, int, int>, bp_batch_config> g_tune_cache;
+
+std::tuple<int, int, long long, long long, int, int> tune_key(const bp_batch *b) {
+  return std::make_tuple(b->fmt, b->P, b->n_items[0], b->n_items[1], b->max_lw[0], b->max_lw[1]);
+}
+
+// Stage buffers usable for class c: 3 only when two work-groups still fit a CU.
+template <int FMT, int P>
+int class_nb(const bp_batch *b, int c) {
+  return (b->cfg.stage_buffers == 3 &&
+          Lds<FMT, P, 3, false>::lds_bytes(b->max_lw[c]) <= kLdsPerCU / 2) ? 3 : 2;
+}
+
+template <int FMT, int P>
+int launch_batch_class(bp_batch *b, int c, hipStream_t st) {
+  if (b->n_items[c] == 0) return BP_OK;
+  const int nb = class_nb<FMT, P>(b, c);
+  const int lds = nb == 3 ? Lds<FMT, P, 3, false>::lds_bytes(b->max_lw[c])
+                          : Lds<FMT, P, 2, false>::lds_bytes(b->max_lw[c]);
+  const int per_cu = std::max(1, std::min(b->cfg.blocks_per_cu, kLdsPerCU / lds));
+  const long long want = (long long)kNumCU * per_cu;
+  const unsigned grid = (unsigned)std::max<long long>(8, std::min(want, (b->n_items[c] + 7) / 8 * 8));
+  if (nb == 3)
+    hipLaunchKernelGGL((bp::stack_batch_kernel<FMT, P, kKW, kW, 3>), dim3(grid), dim3(64 * kW),
+                       lds, st, b->dwins[c].as<bp::BatchWindow>(), (int)b->wins[c].size(),
+                       b->n_items[c]);
+  else
+    hipLaunchKernelGGL((bp::stack_batch_kernel<FMT, P, kKW, kW, 2>), dim3(grid), dim3(64 * kW),
+                       lds, st, b->dwins[c].as<bp::BatchWindow>(), (int)b->wins[c].size(),
+                       b->n_items[c]);
+  HIP_TRY(hipGetLastError());
+  return BP_OK;
+}
+
+template <int FMT>
+int launch_batch_fmt(bp_batch *b, hipStream_t st) {
+  for (int c = 0; c < 2; ++c) {
+    if (b->P == 1)
+      BP_TRY((launch_batch_class<FMT, 1>(b, c, st)));
+    else if (b->P == 2)
+      BP_TRY((launch_batch_class<FMT, 2>(b, c, st)));
+  }
+  for (size_t i = 0; i < b->direct.size(); ++i) {
+    const bp::StackArgs &a = b->direct[i];
+    const unsigned gx = (unsigned)((a.T + 255) / 256);
+    hipLaunchKernelGGL((bp::stack_direct_kernel<FMT, false>), dim3(gx, (unsigned)b->direct_n[i]),
+                       dim3(256), 0, st, a, kDirectChunk);
+    HIP_TRY(hipGetLastError());
+  }
+  return BP_OK;
+}
+
+// Enqueue the whole batch: keys, stack launches, decode.  No host
+// synchronisation and no allocation, so it can be captured in a graph.
+int batch_enqueue(bp_batch *b, hipStream_t st, bool profile) {
+  const unsigned gx = (unsigned)std::min(1024, (b->max_T + 255) / 256);
+  hipLaunchKernelGGL(bp::fill_keys_batch_kernel, dim3(std::max(1u, gx), (unsigned)b->n),
+                     dim3(256), 0, st, b->dkeys.as<bp::BatchKeys>());
+  HIP_TRY(hipGetLastError());
+  hipEvent_t ev[2];
+  if (profile) BP_TRY(prof_begin(st, ev));
+  if (b->fmt == BP_F32)
+    BP_TRY(launch_batch_fmt<BP_F32>(b, st));
+  else if (b->fmt == BP_F16)
+    BP_TRY(launch_batch_fmt<BP_F16>(b, st));
+  else
+    BP_TRY(launch_batch_fmt<BP_BF16>(b, st));
+  if (profile) BP_TRY(prof_end(st, ev));
+  hipLaunchKernelGGL(bp::finalize_max_batch_kernel, dim3(std::max(1u, gx), (unsigned)b->n),
+                     dim3(256), 0, st, b->dkeys.as<bp::BatchKeys>());
+  HIP_TRY(hipGetLastError());
+  return BP_OK;
+}
+
+// Apply a configuration: the block order lives in the window descriptors.
+int batch_configure(bp_batch *b, const bp_batch_config &c) {
+  if (c.time_group < 1 || c.time_group > 64 || (c.stage_buffers != 2 && c.stage_buffers != 3) ||
+      c.blocks_per_cu < 1 || c.blocks_per_cu > 4)
+    return fail(BP_EINVAL, "invalid batch configuration (%d, %d, %d)", c.time_group,
+                c.stage_buffers, c.blocks_per_cu);
+  b->cfg = c;
+  for (int k = 0; k < 2; ++k) {
+    if (b->wins[k].empty()) continue;
+    for (auto &w : b->wins[k]) w.a.time_group = c.time_group;
+    HIP_TRY(hipMemcpy(b->dwins[k].p, b->wins[k].data(),
+                      b->wins[k].size() * sizeof(bp::BatchWindow), hipMemcpyHostToDevice));
+  }
+  b->drop_graph();
+  return BP_OK;
 }
 
 }  // namespace
@@ -750,6 +910,182 @@ int prestack_waveform_features(float *waveform_features, float *weights_phases,
   HIP_TRY(hipMemcpyAsync(prestacked, dout.p, S * T * P * sizeof(float),
                          hipMemcpyDeviceToHost, st.s));
   HIP_TRY(hipStreamSynchronize(st.s));
+  return BP_OK;
+}
+
+int bp_batch_create(int n_windows, bp_network *const *networks, const void *const *features,
+                    int fmt, const float *const *station_scales, const size_t *n_samples,
+                    const int *out_of_bounds, float *const *beam_max, int *const *beam_argmax,
+                    unsigned long long *const *keys, bp_batch **out) {
+  if (!out) return fail(BP_EINVAL, "out is NULL");
+  *out = nullptr;
+  if (n_windows <= 0 || !networks || !features || !n_samples || !out_of_bounds || !beam_max ||
+      !beam_argmax || !keys)
+    return fail(BP_EINVAL, "a batch needs n_windows > 0 and non-NULL arrays");
+  if (n_windows > 65535) return fail(BP_EINVAL, "too many windows (%d)", n_windows);
+  if (fmt != BP_F32 && fmt != BP_F16 && fmt != BP_BF16)
+    return fail(BP_EINVAL, "unknown feature format %d", fmt);
+  BP_TRY(check_device());
+  auto b = std::make_unique<bp_batch>();
+  b->fmt = fmt;
+  b->n = n_windows;
+  Stream st;
+  BP_TRY(st.create());
+  std::map<const bp_network *, std::vector<float>> scales_of;
+  std::vector<bp::BatchKeys> kd;
+  for (int i = 0; i < n_windows; ++i) {
+    bp_network *net = networks[i];
+    if (!net || !features[i] || !beam_max[i] || !beam_argmax[i] || !keys[i])
+      return fail(BP_EINVAL, "window %d: NULL network, features or output", i);
+    if (i == 0) b->P = net->P;
+    if (net->P != b->P)
+      return fail(BP_EINVAL, "window %d has %d phases, window 0 has %d", i, net->P, b->P);
+    const int oob = out_of_bounds[i];
+    if (oob != 0 && oob != 1)
+      return fail(BP_EINVAL, "window %d: out_of_bounds must be 0 or 1, not %d", i, oob);
+    BP_TRY(check_shapes(n_samples[i], net->K, net->S, net->P));
+    const float *sc = station_scales ? station_scales[i] : nullptr;
+    if (fmt == BP_F32 && sc) return fail(BP_EINVAL, "station_scale applies to 16-bit formats only");
+    std::vector<float> want(net->S, 1.0f);
+    if (sc) want.assign(sc, sc + net->S);
+    auto it = scales_of.find(net);
+    if (it != scales_of.end() && it->second != want)
+      return fail(BP_EINVAL, "window %d: a network's windows in one batch must share station scales", i);
+    scales_of[net] = want;
+    BP_TRY(ensure_schedule(net, fmt, sc, st.s));
+    const int T = (int)n_samples[i];
+    kd.push_back(bp::BatchKeys{keys[i], beam_max[i], beam_argmax[i], T});
+    b->max_T = std::max(b->max_T, T);
+    if (T == 0) continue;
+    bp::StackArgs a = make_args(net, features[i], fmt, T, oob);
+    a.keys = keys[i];
+    a.k_begin = 0;
+    a.k_end = net->K;
+    a.n_time_tiles = (T + kTT - 1) / kTT;
+    const long long planes = fmt == BP_F32 ? 1 : 2, es = fmt == BP_F32 ? 4 : 2;
+    const bool lds_ok = (net->P * planes * a.ldf + kLargeLw + 16) * es < (1LL << 31);
+    for (int c = 0; c < 2; ++c) {
+      if (net->tiles[c].empty()) continue;
+      if (!lds_ok) return fail(BP_EINVAL, "window %d: n_samples %d too large for the LDS kernel", i, T);
+      a.tiles = net->dtiles[c].as<int>();
+      a.n_list = (int)net->tiles[c].size();
+      a.Lw = net->Lw[c];
+      b->wins[c].push_back(bp::BatchWindow{a, b->n_items[c]});
+      b->n_items[c] += (long long)a.n_time_tiles * a.n_list;
+      b->max_lw[c] = std::max(b->max_lw[c], a.Lw);
+    }
+    if (!net->tiles[2].empty()) {
+      const int nd = (int)net->tiles[2].size();
+      if (nd > 65535) return fail(BP_EINVAL, "window %d: too many direct-kernel tiles (%d)", i, nd);
+      a.tiles = net->dtiles[2].as<int>();
+      a.n_list = nd;
+      b->direct.push_back(a);
+      b->direct_n.push_back(nd);
+    }
+  }
+  for (int c = 0; c < 2; ++c) {
+    if (b->wins[c].empty()) continue;
+    BP_TRY(b->dwins[c].alloc(b->wins[c].size() * sizeof(bp::BatchWindow)));
+  }
+  BP_TRY(b->dkeys.alloc(kd.size() * sizeof(bp::BatchKeys)));
+  HIP_TRY(hipMemcpy(b->dkeys.p, kd.data(), kd.size() * sizeof(bp::BatchKeys),
+                    hipMemcpyHostToDevice));
+  BP_TRY(init_batch_attrs());
+  bp_batch_config cfg = b->cfg;
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    auto it = g_tune_cache.find(tune_key(b.get()));
+    if (it != g_tune_cache.end()) cfg = it->second;
+  }
+  BP_TRY(batch_configure(b.get(), cfg));
+  HIP_TRY(hipStreamSynchronize(st.s));
+  *out = b.release();
+  return BP_OK;
+}
+
+int bp_batch_launch(bp_batch *b, void *stream) {
+  if (!b) return fail(BP_EINVAL, "NULL batch");
+  return batch_enqueue(b, static_cast<hipStream_t>(stream), true);
+}
+
+int bp_batch_graph_launch(bp_batch *b, void *stream) {
+  if (!b) return fail(BP_EINVAL, "NULL batch");
+  if (!b->exec) {
+    if (!b->cap) HIP_TRY(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
+    HIP_TRY(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+    const int rc = batch_enqueue(b, b->cap, false);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->cap, &g);
+    if (rc != BP_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    HIP_TRY(e);
+    b->graph = g;
+    HIP_TRY(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+  }
+  HIP_TRY(hipGraphLaunch(b->exec, static_cast<hipStream_t>(stream)));
+  return BP_OK;
+}
+
+int bp_batch_get_config(const bp_batch *b, bp_batch_config *cfg) {
+  if (!b || !cfg) return fail(BP_EINVAL, "NULL argument");
+  *cfg = b->cfg;
+  return BP_OK;
+}
+
+int bp_batch_set_config(bp_batch *b, const bp_batch_config *cfg) {
+  if (!b || !cfg) return fail(BP_EINVAL, "NULL argument");
+  return batch_configure(b, *cfg);
+}
+
+int bp_batch_tune(bp_batch *b, int reps, bp_batch_config *tried, int max_tried, int *n_tried,
+                  bp_batch_config *best, void *stream) {
+  if (!b) return fail(BP_EINVAL, "NULL batch");
+  if (reps < 1) reps = 1;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  std::vector<bp_batch_config> cands;
+  for (int tg : {1, 2, 4, 8})
+    for (int nb : {2, 3})
+      for (int bpc : {1, 2}) cands.push_back(bp_batch_config{tg, nb, bpc, 0.0f});
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  int count = 0, rc = BP_OK;
+  bp_batch_config win{};
+  win.ms = -1.0f;
+  for (bp_batch_config &c : cands) {
+    if ((rc = batch_configure(b, c)) != BP_OK) break;
+    if ((rc = batch_enqueue(b, st, false)) != BP_OK) break;  // warm-up
+    if (hipEventRecord(e0, st) != hipSuccess) { rc = fail(BP_EHIP, "hipEventRecord"); break; }
+    for (int r = 0; r < reps && rc == BP_OK; ++r) rc = batch_enqueue(b, st, false);
+    if (rc != BP_OK) break;
+    if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+      rc = fail(BP_EHIP, "timing events failed");
+      break;
+    }
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    c.ms = ms / reps;
+    if (tried && count < max_tried) tried[count] = c;
+    ++count;
+    if (win.ms < 0 || c.ms < win.ms) win = c;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (n_tried) *n_tried = count;
+  if (rc != BP_OK) return rc;
+  BP_TRY(batch_configure(b, win));
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tune_cache[tune_key(b)] = win;
+  }
+  if (best) *best = win;
+  return BP_OK;
+}
+
+int bp_batch_destroy(bp_batch *b) {
+  delete b;
   return BP_OK;
 }
 

@@ -29,6 +29,7 @@ struct StackArgs {
   int oob;                // 0 strict, 1 flexible
   int Lw;                 // LDS window length (samples), multiple of 8
   int n_time_tiles;
+  int time_group;         // block order: time tiles per group (>= 1)
   int k_begin, k_end;     // sources written: [k_begin, k_end)
   unsigned long long *keys;  // reduce=max: (T) packed keys
   float *beam;               // reduce=none: beam[(k - k_begin) * T + t]
@@ -324,18 +325,14 @@ __global__ void finalize_max_kernel(const unsigned long long *__restrict__ keys,
 // running together on an XCD share one time tile, so every station's
 // feature windows are fetched from HBM once and then hit in that XCD's L2;
 // larger groups trade that for reuse of each source tile's schedule.
-#ifndef BP_TIME_GROUP
-#define BP_TIME_GROUP 1
-#endif
 __device__ __forceinline__ int xcd_remap(int b, int n) {
   const int n8 = n & ~7;
   if (b >= n8) return b;
   return (b & 7) * (n8 >> 3) + (b >> 3);
 }
 
-__device__ __forceinline__ void block_tile(int L, int n_time_tiles, int n_list, int &tt,
+__device__ __forceinline__ void block_tile(int L, int n_time_tiles, int n_list, int TG, int &tt,
                                            int &li) {
-  constexpr int TG = BP_TIME_GROUP;
   const int g = L / (TG * n_list);
   const int gsize = min(TG, n_time_tiles - g * TG);  // last group may be short
   const int rem = L - g * TG * n_list;
@@ -581,11 +578,12 @@ struct StackLds {
   __device__ __forceinline__ static unsigned load_elem(const StackArgs &a, int s, int p,
                                                        long long t) {
     if (t < 0 || t >= a.T) return 0u;
-    const char *base = reinterpret_cast<const char *>(a.F);
+    const global_char *base = (const global_char *)(a.F);
     if (ES == 4)
-      return *reinterpret_cast<const unsigned *>(base + (((size_t)s * P + p) * a.ldf + t) * 4);
-    return *reinterpret_cast<const unsigned short *>(base +
-                                                     (((size_t)s * P + p) * 2 * a.ldf + t) * 2);
+      return *(const __attribute__((address_space(1))) unsigned *)(
+          base + (((size_t)s * P + p) * a.ldf + t) * 4);
+    return *(const __attribute__((address_space(1))) unsigned short *)(
+        base + (((size_t)s * P + p) * 2 * a.ldf + t) * 2);
   }
 
   __device__ __forceinline__ static void stage_station_guarded(const StackArgs &a,
@@ -681,7 +679,10 @@ struct StackLds {
     return t0 + SPA * lane + 64 * SPA * (j / SPA) + (j % SPA);
   }
 
-  __device__ __forceinline__ static void run(const StackArgs &a, unsigned char *smem) {
+  // One work-group's tile: time tile tt (samples [512 tt, 512 tt + 512)) of
+  // source tile `tile`, all stations.
+  __device__ __forceinline__ static void tile_job(const StackArgs &a, int tt, int tile,
+                                                  unsigned char *smem) {
     const int Lw = a.Lw;
     const int SB = window_slot_bytes(Lw);
     unsigned long long *red = reinterpret_cast<unsigned long long *>(smem);
@@ -690,10 +691,6 @@ struct StackLds {
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = uniform(tid >> 6);
-    const int L = xcd_remap(blockIdx.x, a.n_time_tiles * a.n_list);
-    int tt, li;
-    block_tile(L, a.n_time_tiles, a.n_list, tt, li);
-    const int tile = as_const(a.tiles)[li];
     const int t0 = tt * TT;
     const int S = a.S;
     const_int *tlo = as_const(a.tile_lo) + (size_t)tile * S * P;
@@ -826,6 +823,15 @@ struct StackLds {
       }
     }
   }
+
+  // One launch over the listed tiles of one network: block -> (time tile,
+  // listed tile) with XCD and L2 locality.
+  __device__ __forceinline__ static void run(const StackArgs &a, unsigned char *smem) {
+    const int L = xcd_remap(blockIdx.x, a.n_time_tiles * a.n_list);
+    int tt, li;
+    block_tile(L, a.n_time_tiles, a.n_list, a.time_group, tt, li);
+    tile_job(a, tt, as_const(a.tiles)[li], smem);
+  }
 };
 
 // At least 4 waves per SIMD (VGPRs <= 128): two work-groups per CU, so one
@@ -838,6 +844,85 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(BP_WAVES
 void stack_lds_kernel(StackArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   StackLds<FMT, P, KW, W, NB, NONE>::run(a, smem);
+}
+
+// ---------------------------------------------------------------------------
+// Batched form: one persistent launch over the LDS-class tiles of many
+// windows (each its own network, length and feature buffer), reduce='max'.
+// Window w contributes n_time_tiles(w) * n_list(w) work items, numbered from
+// item_begin(w).  The grid (a multiple of 8 work-groups) stays resident:
+// work-group b runs on XCD b % 8, and XCD x walks the contiguous item range
+// [x*N8, (x+1)*N8), N8 = ceil(N/8), so that within an XCD consecutive items
+// are consecutive time tiles of one window (the same L2 locality as one
+// launch per window).  Items cost different amounts (S varies per window);
+// the static round-robin within an XCD balances them over many items.
+// ---------------------------------------------------------------------------
+struct BatchWindow {
+  StackArgs a;           // the window's arguments: its tiles of one LDS class
+  long long item_begin;  // first work item of the window
+};
+
+typedef const __attribute__((address_space(4))) BatchWindow const_batch_window;
+
+template <int FMT, int P, int KW, int W, int NB>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(BP_WAVES_PER_EU)))
+void stack_batch_kernel(const BatchWindow *wins, int n_wins, long long n_items) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const_batch_window *cw = (const_batch_window *)wins;
+  const long long n8 = (n_items + 7) / 8;
+  const int x = blockIdx.x & 7;
+  const long long end = min(n_items, (long long)(x + 1) * n8);
+  const int step = gridDim.x >> 3;
+  for (long long L = (long long)x * n8 + (blockIdx.x >> 3); L < end; L += step) {
+    // the window holding item L: the last with item_begin <= L
+    int lo = 0, hi = n_wins - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (cw[mid].item_begin <= L)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    // the window's arguments, copied word by word through scalar loads
+    StackArgs a;
+    static_assert(sizeof(StackArgs) % 4 == 0, "StackArgs is copied as words");
+    const_int *src = (const_int *)&cw[lo].a;
+#pragma unroll
+    for (int w = 0; w < (int)(sizeof(StackArgs) / 4); ++w) reinterpret_cast<int *>(&a)[w] = src[w];
+    int tt, li;
+    block_tile((int)(L - cw[lo].item_begin), a.n_time_tiles, a.n_list, a.time_group, tt, li);
+    StackLds<FMT, P, KW, W, NB, false>::tile_job(a, tt, as_const(a.tiles)[li], smem);
+    __syncthreads();  // the LDS is reused by the next item
+  }
+}
+
+// Batched key initialisation and decoding: blockIdx.y selects the window.
+struct BatchKeys {
+  unsigned long long *keys;
+  float *beam_max;
+  int *beam_argmax;
+  int T;
+};
+
+__global__ void fill_keys_batch_kernel(const BatchKeys *w) {
+  const BatchKeys b = w[blockIdx.y];
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < b.T;
+       i += (long long)gridDim.x * blockDim.x)
+    b.keys[i] = 0ull;
+}
+
+__global__ void finalize_max_batch_kernel(const BatchKeys *w) {
+  const BatchKeys b = w[blockIdx.y];
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < b.T;
+       t += (long long)gridDim.x * blockDim.x) {
+    const unsigned long long key = b.keys[t];
+    if (key) {
+      b.beam_max[t] = float_unorder((uint32_t)(key >> 32));
+      b.beam_argmax[t] = (int)(0xffffffffu - (uint32_t)key);
+    } else {
+      b.beam_argmax[t] = 0;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -10,7 +10,7 @@ import ctypes as ct
 
 import numpy as np
 
-from .core import check, load_library
+from .core import BatchConfig, check, load_library
 
 BP_F32, BP_BF16, BP_F16 = 0, 1, 2
 
@@ -161,6 +161,104 @@ def prestack_planar(u, weights_phases, stream=None):
                                  S, C, T, P, out.data_ptr(), _stream_ptr(stream)),
           "bp_prestack_planar", lib)
     return out
+
+
+class Batch:
+    """Batched reduce='max' over windows of resident networks
+    (bp_batch_*): one persistent launch per LDS window class for all windows.
+
+    windows: sequence of (network, features, out_of_bounds), features being
+    a planar float32 CUDA tensor or QuantizedFeatures (one format for the
+    whole batch).  Outputs are allocated here: `beam_max[i]` (zeros) and
+    `beam_argmax[i]` CUDA tensors of the window's length."""
+
+    def __init__(self, windows):
+        torch = _torch()
+        self.lib = load_library("gpu")
+        n = len(windows)
+        if n == 0:
+            raise ValueError("a batch needs at least one window")
+        self.windows = list(windows)  # keeps networks and features alive
+        fmts, ptrs, scales, lengths = [], [], [], []
+        self.beam_max, self.beam_argmax, self._keys = [], [], []
+        for net, feats, _ in self.windows:
+            if isinstance(feats, QuantizedFeatures):
+                fmts.append(feats.fmt)
+                ptrs.append(feats.data.data_ptr())
+                scales.append(feats.weight_scale.ctypes.data)
+                T, dev = feats.n_samples, feats.data.device
+                S, P = feats.data.shape[:2]
+            else:
+                if feats.dtype != torch.float32 or not feats.is_contiguous():
+                    raise ValueError("features must be contiguous float32 (S, P, T)")
+                fmts.append(BP_F32)
+                ptrs.append(feats.data_ptr())
+                scales.append(None)
+                S, P, T = feats.shape
+                dev = feats.device
+            if (S, P) != (net.n_stations, net.n_phases):
+                raise ValueError("features do not match their network")
+            lengths.append(T)
+            self.beam_max.append(torch.zeros(T, dtype=torch.float32, device=dev))
+            self.beam_argmax.append(torch.empty(T, dtype=torch.int32, device=dev))
+            self._keys.append(torch.empty(max(T, 1), dtype=torch.int64, device=dev))
+        if len(set(fmts)) != 1:
+            raise ValueError("one feature format per batch")
+        vp = ct.c_void_p
+        arr = lambda xs: (vp * n)(*xs)
+        self._args = dict(
+            nets=arr([net._h.value for net, _, _ in self.windows]), feats=arr(ptrs),
+            scales=arr(scales) if fmts[0] != BP_F32 else None,
+            lengths=(ct.c_size_t * n)(*lengths),
+            oob=(ct.c_int * n)(*[int(o) for _, _, o in self.windows]),
+            bmax=arr([t.data_ptr() for t in self.beam_max]),
+            barg=arr([t.data_ptr() for t in self.beam_argmax]),
+            keys=arr([t.data_ptr() for t in self._keys]))
+        h = vp()
+        a = self._args
+        check(self.lib.bp_batch_create(n, a["nets"], a["feats"], fmts[0], a["scales"],
+                                       a["lengths"], a["oob"], a["bmax"], a["barg"],
+                                       a["keys"], ct.byref(h)), "bp_batch_create", self.lib)
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.bp_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def launch(self, stream=None):
+        check(self.lib.bp_batch_launch(self._h, _stream_ptr(stream)), "bp_batch_launch",
+              self.lib)
+
+    def graph_launch(self, stream=None):
+        check(self.lib.bp_batch_graph_launch(self._h, _stream_ptr(stream)),
+              "bp_batch_graph_launch", self.lib)
+
+    def config(self):
+        c = BatchConfig()
+        check(self.lib.bp_batch_get_config(self._h, ct.byref(c)), "bp_batch_get_config",
+              self.lib)
+        return dict(time_group=c.time_group, stage_buffers=c.stage_buffers,
+                    blocks_per_cu=c.blocks_per_cu, ms=c.ms)
+
+    def set_config(self, time_group=1, stage_buffers=2, blocks_per_cu=2):
+        c = BatchConfig(time_group, stage_buffers, blocks_per_cu, 0.0)
+        check(self.lib.bp_batch_set_config(self._h, ct.byref(c)), "bp_batch_set_config",
+              self.lib)
+
+    def tune(self, reps=2, stream=None):
+        """Time every configuration on this batch, keep the fastest.  Returns
+        (best, tried) as dicts."""
+        tried = (BatchConfig * 64)()
+        n, best = ct.c_int(), BatchConfig()
+        check(self.lib.bp_batch_tune(self._h, int(reps), tried, 64, ct.byref(n), ct.byref(best),
+                                     _stream_ptr(stream)), "bp_batch_tune", self.lib)
+        conv = lambda c: dict(time_group=c.time_group, stage_buffers=c.stage_buffers,
+                              blocks_per_cu=c.blocks_per_cu, ms=c.ms)
+        return conv(best), [conv(tried[i]) for i in range(n.value)]
 
 
 class KernelTimer:

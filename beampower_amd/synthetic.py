@@ -117,6 +117,22 @@ def precision_c3(seed=3, n_samples=2_160_000):
                          n_samples, 200)
 
 
+def batched_c4_window(seed, minutes=(1.0, 30.0), stations=(10, 80), sources=(1_000, 200_000)):
+    """One window of the batched form: length uniform in `minutes` at 25 Hz,
+    station count uniform in `stations`, a source grid of about K sources (K
+    uniform in `sources`; nz ~ K^(1/3)/2 depth levels, a square horizontal
+    grid), plain-method geometry (50 km box, depth 0-20 km), seeded per
+    window."""
+    rng = np.random.default_rng(seed)
+    n_samples = int(rng.uniform(*minutes) * 60 * SAMPLING_RATE_HZ)
+    n_stations = int(rng.integers(stations[0], stations[1] + 1))
+    k = float(rng.uniform(*sources))
+    nz = max(2, int(round(k ** (1.0 / 3.0) / 2)))
+    nxy = max(2, int(round((k / nz) ** 0.5)))
+    return make_workload(f"batch-{seed}", seed, n_stations, 50.0, (nxy, nxy, nz), n_samples,
+                         max(1, n_samples // 3000))
+
+
 def subset_sources(n_sources, n_subset, seed):
     """Seeded sorted subset of source indices (the reduce='none' case)."""
     rng = np.random.default_rng(seed)

@@ -137,6 +137,43 @@ int bp_prestack_planar(const float *u_sct, const float *weights_phases,
                        size_t n_stations, size_t n_channels, size_t n_samples,
                        size_t n_phases, float *F_spt, void *stream);
 
+/* Batched reduce='max' (no counterpart in the reference, which takes one
+ * window per call): n_windows windows, each with its own network,
+ * device-resident features, length and DEVICE outputs (beam_max,
+ * beam_argmax and keys scratch of n_samples[i] elements, as for
+ * bp_network_beamform_max), all in one feature format and with one number of
+ * phases.  The LDS-class tiles of every window run in one persistent launch
+ * per window class; results are bit-identical to bp_network_beamform_max on
+ * each window alone.  station_scales: NULL, or per window a HOST pointer
+ * (n_stations) or NULL, as for bp_network_beamform_max; windows of one
+ * network must share it.  Creation builds missing schedules synchronously
+ * and uploads the window table; the networks, features and outputs must
+ * outlive the batch.  bp_batch_launch enqueues the batch on `stream`
+ * (no allocation, no host synchronisation); bp_batch_graph_launch captures
+ * that sequence into a hipGraph on first use and replays the graph after.
+ * bp_batch_tune times every configuration of the search space (time tiles
+ * per block-order group 1/2/4/8, staging buffers 2/3, resident work-groups
+ * per CU 1/2) on this batch, keeps the fastest and remembers it for
+ * batches of the same shape in this process. */
+typedef struct bp_batch bp_batch;
+typedef struct {
+  int time_group;     /* time tiles per block-order group (1 = time-major) */
+  int stage_buffers;  /* 2 or 3 (3 only where two work-groups still fit a CU) */
+  int blocks_per_cu;  /* resident work-groups per CU of the persistent grid */
+  float ms;           /* measured time of the whole batch (tuning output) */
+} bp_batch_config;
+int bp_batch_create(int n_windows, bp_network *const *networks, const void *const *features,
+                    int fmt, const float *const *station_scales, const size_t *n_samples,
+                    const int *out_of_bounds, float *const *beam_max, int *const *beam_argmax,
+                    unsigned long long *const *keys, bp_batch **out);
+int bp_batch_launch(bp_batch *batch, void *stream);
+int bp_batch_graph_launch(bp_batch *batch, void *stream);
+int bp_batch_tune(bp_batch *batch, int reps, bp_batch_config *tried, int max_tried,
+                  int *n_tried, bp_batch_config *best, void *stream);
+int bp_batch_get_config(const bp_batch *batch, bp_batch_config *config);
+int bp_batch_set_config(bp_batch *batch, const bp_batch_config *config);
+int bp_batch_destroy(bp_batch *batch);
+
 /* Kernel timing: when enabled, the dominant (stack) kernel of every
  * bp_network_* call is bracketed with HIP events on its own stream and the
  * elapsed milliseconds are accumulated per thread. */

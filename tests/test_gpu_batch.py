@@ -1,0 +1,120 @@
+"""Batched form on the GPU: one persistent launch over windows of several
+resident networks (bp_batch_*), against each window computed alone with
+bp_network_beamform_max.
+
+Bar: bit-identical beam_max and beam_argmax per window (the batch runs the
+same tile code on the same tiles; the max is an order-independent packed-key
+atomicMax), for the plain launch, every hipGraph replay and every tuned
+configuration.
+"""
+import numpy as np
+import pytest
+
+from conftest import bits_equal
+from beampower_amd import synthetic
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def _planar(torch, bpd, u, alpha):
+    return bpd.prestack_planar(torch.from_numpy(np.ascontiguousarray(u)).cuda(),
+                               torch.from_numpy(np.ascontiguousarray(alpha)).cuda())
+
+
+def make_windows(fmt, seed0=100):
+    """Windows of plain-method networks with uneven stations, sources and
+    lengths, plus random networks whose tiles need the large LDS window and
+    the direct kernel, plus a second window on the first network."""
+    torch = _torch()
+    from beampower_amd import device as bpd
+    wins = []
+    for i in range(6):
+        w = synthetic.batched_c4_window(seed0 + i, minutes=(1.0, 4.0), stations=(10, 30),
+                                        sources=(500, 6000))
+        F = _planar(torch, bpd, w.waveform_features, w.weights_phases)
+        wins.append([bpd.Network(w.time_delays, w.weights_sources), F, i % 2])
+    for seed, spread, T in ((7, 1200, 5000), (8, 3000, 7000)):
+        u, alpha, tau, beta = synthetic.random_case(seed, S=9, C=3, P=2, T=T, K=300,
+                                                    tau_range=(0, spread))
+        wins.append([bpd.Network(tau, beta), _planar(torch, bpd, u, alpha), 1])
+    u0 = synthetic.boxcar_envelopes(np.random.default_rng(5), wins[0][0].n_stations, 3, 2100)
+    F0 = _planar(torch, bpd, u0, synthetic.tutorial_phase_weights(wins[0][0].n_stations))
+    wins.append([wins[0][0], F0, 0])
+    if fmt != bpd.BP_F32:
+        for w in wins:
+            w[1] = bpd.quantize(w[1], fmt)
+    return [tuple(w) for w in wins]
+
+
+def alone(wins):
+    out = []
+    for net, feats, oob in wins:
+        m, a = net.beamform_max(feats, oob)
+        out.append((m.cpu().numpy(), a.cpu().numpy()))
+    return out
+
+
+def assert_batch_equals(batch, ref, what):
+    for i, (m, a) in enumerate(ref):
+        assert bits_equal(batch.beam_max[i].cpu().numpy(), m), (what, i)
+        assert np.array_equal(batch.beam_argmax[i].cpu().numpy(), a), (what, i)
+
+
+@pytest.mark.parametrize("fmt_name", ["f32", "bf16"])
+def test_batch_matches_windows_alone(fmt_name):
+    torch = _torch()
+    from beampower_amd import device as bpd
+    fmt = {"f32": bpd.BP_F32, "bf16": bpd.BP_BF16}[fmt_name]
+    wins = make_windows(fmt)
+    classes = [net.info()["tiles_per_class"] for net, _, _ in wins]
+    assert any(c[1] for c in classes) and any(c[2] for c in classes), classes
+    ref = alone(wins)
+    batch = bpd.Batch(wins)
+    batch.launch()
+    torch.cuda.synchronize()
+    assert_batch_equals(batch, ref, "launch")
+    for rep in range(3):
+        for m in batch.beam_max:
+            m.zero_()
+        batch.graph_launch()
+        torch.cuda.synchronize()
+        assert_batch_equals(batch, ref, f"graph replay {rep}")
+    best, tried = batch.tune(reps=1)
+    assert len(tried) == 16 and all(t["ms"] > 0 for t in tried)
+    assert best["ms"] == min(t["ms"] for t in tried)
+    assert batch.config()["time_group"] == best["time_group"]
+    for m in batch.beam_max:
+        m.zero_()
+    batch.graph_launch()
+    torch.cuda.synchronize()
+    assert_batch_equals(batch, ref, f"tuned {best}")
+
+
+def test_batch_rejects_f16_windows_with_different_scales_on_one_network():
+    _torch()
+    from beampower_amd import device as bpd
+    wins = make_windows(bpd.BP_F16)
+    shared = [w for w in wins if w[0] is wins[0][0]]
+    assert len(shared) == 2
+    if np.array_equal(shared[0][1].weight_scale, shared[1][1].weight_scale):
+        pytest.skip("the two windows happen to share scales")
+    with pytest.raises(RuntimeError, match="share station scales"):
+        bpd.Batch(wins)
+
+
+def test_batch_f16_without_shared_networks():
+    torch = _torch()
+    from beampower_amd import device as bpd
+    wins = make_windows(bpd.BP_F16)[:-1]
+    ref = alone(wins)
+    batch = bpd.Batch(wins)
+    batch.launch()
+    torch.cuda.synchronize()
+    assert_batch_equals(batch, ref, "f16")
