@@ -497,6 +497,7 @@ struct bp_batch {
   // per LDS class: window descriptors (host copy and device), item count, largest window
   std::vector<bp::BatchWindow> wins[2];
   DevBuf dwins[2];
+  DevBuf dsplit[2];  // per class: 9 item indices, the XCD ranges of the persistent grid
   long long n_items[2] = {0, 0};
   int max_lw[2] = {0, 0};
   std::vector<bp::StackArgs> direct;  // per window with direct-kernel tiles
@@ -543,11 +544,11 @@ int launch_batch_class(bp_batch *b, int c, hipStream_t st) {
   if (nb == 3)
     hipLaunchKernelGGL((bp::stack_batch_kernel<FMT, P, kKW, kW, 3>), dim3(grid), dim3(64 * kW),
                        lds, st, b->dwins[c].as<bp::BatchWindow>(), (int)b->wins[c].size(),
-                       b->n_items[c]);
+                       b->dsplit[c].as<long long>());
   else
     hipLaunchKernelGGL((bp::stack_batch_kernel<FMT, P, kKW, kW, 2>), dim3(grid), dim3(64 * kW),
                        lds, st, b->dwins[c].as<bp::BatchWindow>(), (int)b->wins[c].size(),
-                       b->n_items[c]);
+                       b->dsplit[c].as<long long>());
   HIP_TRY(hipGetLastError());
   return BP_OK;
 }
@@ -986,6 +987,37 @@ int bp_batch_create(int n_windows, bp_network *const *networks, const void *cons
   for (int c = 0; c < 2; ++c) {
     if (b->wins[c].empty()) continue;
     BP_TRY(b->dwins[c].alloc(b->wins[c].size() * sizeof(bp::BatchWindow)));
+    // XCD ranges at equal shares of items x stations
+    const std::vector<bp::BatchWindow> &ws = b->wins[c];
+    double total = 0.0;
+    for (size_t w = 0; w < ws.size(); ++w) {
+      const long long items = (w + 1 < ws.size() ? ws[w + 1].item_begin : b->n_items[c]) -
+                              ws[w].item_begin;
+      total += (double)items * ws[w].a.S;
+    }
+    long long split[9];
+    split[0] = 0;
+    split[8] = b->n_items[c];
+    size_t w = 0;
+    double before = 0.0;  // cost of the windows before w
+    for (int x = 1; x < 8; ++x) {
+      const double target = total * x / 8.0;
+      for (;;) {
+        const long long items = (w + 1 < ws.size() ? ws[w + 1].item_begin : b->n_items[c]) -
+                                ws[w].item_begin;
+        const double cost = (double)items * ws[w].a.S;
+        if (before + cost >= target || w + 1 == ws.size()) {
+          const long long k = (long long)std::ceil((target - before) / ws[w].a.S);
+          split[x] = std::min(b->n_items[c], ws[w].item_begin + std::max(0LL, std::min(items, k)));
+          break;
+        }
+        before += cost;
+        ++w;
+      }
+      split[x] = std::max(split[x], split[x - 1]);
+    }
+    BP_TRY(b->dsplit[c].alloc(sizeof(split)));
+    HIP_TRY(hipMemcpy(b->dsplit[c].p, split, sizeof(split), hipMemcpyHostToDevice));
   }
   BP_TRY(b->dkeys.alloc(kd.size() * sizeof(bp::BatchKeys)));
   HIP_TRY(hipMemcpy(b->dkeys.p, kd.data(), kd.size() * sizeof(bp::BatchKeys),

@@ -852,10 +852,12 @@ void stack_lds_kernel(StackArgs a) {
 // Window w contributes n_time_tiles(w) * n_list(w) work items, numbered from
 // item_begin(w).  The grid (a multiple of 8 work-groups) stays resident:
 // work-group b runs on XCD b % 8, and XCD x walks the contiguous item range
-// [x*N8, (x+1)*N8), N8 = ceil(N/8), so that within an XCD consecutive items
+// [split[x], split[x+1]), so that within an XCD consecutive items
 // are consecutive time tiles of one window (the same L2 locality as one
-// launch per window).  Items cost different amounts (S varies per window);
-// the static round-robin within an XCD balances them over many items.
+// launch per window).  Items cost different amounts (S varies per window),
+// so the host places the XCD boundaries `split` at equal shares of the total
+// cost (items x stations) rather than of the item count; within an XCD the
+// round-robin over its work-groups mixes the windows of its range.
 // ---------------------------------------------------------------------------
 struct BatchWindow {
   StackArgs a;           // the window's arguments: its tiles of one LDS class
@@ -866,14 +868,14 @@ typedef const __attribute__((address_space(4))) BatchWindow const_batch_window;
 
 template <int FMT, int P, int KW, int W, int NB>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(BP_WAVES_PER_EU)))
-void stack_batch_kernel(const BatchWindow *wins, int n_wins, long long n_items) {
+void stack_batch_kernel(const BatchWindow *wins, int n_wins, const long long *split) {
   extern __shared__ __align__(16) unsigned char smem[];
   const_batch_window *cw = (const_batch_window *)wins;
-  const long long n8 = (n_items + 7) / 8;
+  typedef const __attribute__((address_space(4))) long long const_ll;
   const int x = blockIdx.x & 7;
-  const long long end = min(n_items, (long long)(x + 1) * n8);
+  const long long end = ((const_ll *)split)[x + 1];
   const int step = gridDim.x >> 3;
-  for (long long L = (long long)x * n8 + (blockIdx.x >> 3); L < end; L += step) {
+  for (long long L = ((const_ll *)split)[x] + (blockIdx.x >> 3); L < end; L += step) {
     // the window holding item L: the last with item_begin <= L
     int lo = 0, hi = n_wins - 1;
     while (lo < hi) {

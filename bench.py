@@ -108,6 +108,84 @@ def profile_traffic():
     return None
 
 
+def bench_batched(args, log):
+    """Batched form: uneven windows (1-30 min, 10-80 stations, 1k-200k
+    sources, plain-method geometry seeded per window) of as many networks in
+    one persistent launch per window class; FP32 and BF16 features; hipGraph
+    replay; optional autotuning; compared with the windows launched one by
+    one (bit equality and time)."""
+    import numpy as np
+    import torch
+    from beampower_amd import device as bpd
+    from beampower_amd import synthetic
+
+    n = args.batch_windows
+    log(f"batched form: generating {n} windows")
+    wins, work = [], 0
+    for i in range(n):
+        w = synthetic.batched_c4_window(1000 + i)
+        F = bpd.prestack_planar(torch.from_numpy(w.waveform_features).cuda(),
+                                torch.from_numpy(w.weights_phases).cuda())
+        net = bpd.Network(w.time_delays, w.weights_sources)
+        K, S, P = w.time_delays.shape
+        work += K * F.shape[2]
+        wins.append((net, F, 0))
+        del w
+    torch.cuda.synchronize()
+    out = {"windows": n, "source_samples": work,
+           "stations": [wi[0].n_stations for wi in wins][:8] + ["..."]}
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    def alone():
+        for net, F, oob in wins:
+            net.beamform_max(F, oob)
+
+    reps = max(1, args.steps // 2)
+    t_alone = timed(alone, reps)
+    ref = []
+    for net, F, oob in wins:
+        m, a = net.beamform_max(F, oob)
+        ref.append((m.cpu().numpy(), a.cpu().numpy()))
+    b32 = bpd.Batch(wins)
+    if args.tune:
+        log("batched form: tuning")
+        best, tried = b32.tune(reps=1)
+        out["tuned"] = best
+        out["tried"] = tried
+    out["config"] = b32.config()
+    t_launch = timed(b32.launch, reps)
+    t_graph = timed(b32.graph_launch, reps)
+    same = all(np.array_equal(b32.beam_max[i].cpu().numpy().view(np.uint32),
+                              ref[i][0].view(np.uint32))
+               and np.array_equal(b32.beam_argmax[i].cpu().numpy(), ref[i][1])
+               for i in range(n))
+    out["f32"] = {"ms": t_launch * 1e3, "graph_ms": t_graph * 1e3, "value": work / t_graph,
+                  "windows_one_by_one_ms": t_alone * 1e3,
+                  "identical_to_windows_alone": bool(same)}
+    log(f"batched f32: {t_graph * 1e3:.1f} ms (graph), {t_alone * 1e3:.1f} ms one by one")
+    del b32
+    q = [(net, bpd.quantize(F, bpd.BP_BF16), oob) for net, F, oob in wins]
+    b16 = bpd.Batch(q)
+    t16 = timed(b16.graph_launch, reps)
+    agree = float(np.mean(np.concatenate(
+        [b16.beam_argmax[i].cpu().numpy() == ref[i][1] for i in range(n)])))
+    out["bf16"] = {"graph_ms": t16 * 1e3, "value": work / t16,
+                   "speedup_vs_f32": t_graph / t16, "argmax_equal_to_f32": agree}
+    log(f"batched bf16: {t16 * 1e3:.1f} ms (graph)")
+    del b16, q, wins
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +194,9 @@ def main():
     ap.add_argument("--oob", choices=["strict", "flexible"], default="strict")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-windows", type=int, default=32,
+                    help="windows of the batched form (256 = the full batched workload; 0 = skip)")
+    ap.add_argument("--tune", action="store_true", help="autotune the batched form first")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs on a single MI355X "
@@ -215,6 +296,11 @@ def main():
                                 / (kernel_ms * 1e-3)) / HBM_PEAK,
     }
 
+    del net, F, bmax, barg, keys
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    batched = bench_batched(args, log) if args.batch_windows > 0 else None
+
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline")
@@ -242,6 +328,7 @@ def main():
         "events_recovered": f"{recovered}/{len(w.events)}",
         "roofline": roofline,
         "precision_forms": precision,
+        "batched_form": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
