@@ -54,9 +54,17 @@ GPU_SYMBOLS = (
     "bp_interleaved_to_planar", "bp_prestack_planar",
     "bp_parity_stride", "bp_quantize_planar",
     "bp_profiling", "bp_profiling_read",
+    "bp_network_beamform_max_view", "bp_network_padding", "bp_pack_features",
     "bp_batch_create", "bp_batch_launch", "bp_batch_graph_launch", "bp_batch_tune",
     "bp_batch_get_config", "bp_batch_set_config", "bp_batch_destroy",
 )
+
+
+class FeatureViewC(ct.Structure):
+    """bp_feature_view of include/beamform_gpu.h."""
+    _fields_ = [("data", ct.c_void_p), ("fmt", ct.c_int), ("n_samples", ct.c_size_t),
+                ("row_stride", ct.c_size_t), ("pad_before", ct.c_size_t),
+                ("station_scale", ct.c_void_p)]
 
 
 class BatchConfig(ct.Structure):
@@ -84,7 +92,11 @@ def _declare_gpu_extension(lib):
     lib.bp_quantize_planar.argtypes = [vp, _SZ, _SZ, _SZ, ct.c_int, vp, vp, vp]
     lib.bp_profiling.argtypes = [ct.c_int]
     lib.bp_profiling_read.argtypes = [ct.POINTER(ct.c_double), ct.POINTER(ct.c_longlong)]
-    lib.bp_batch_create.argtypes = [ct.c_int, vp, vp, ct.c_int, vp, vp, vp, vp, vp, vp,
+    lib.bp_network_beamform_max_view.argtypes = [vp, ct.POINTER(FeatureViewC), ct.c_int, vp,
+                                                 vp, vp, vp]
+    lib.bp_network_padding.argtypes = [vp, ct.POINTER(_SZ), ct.POINTER(_SZ)]
+    lib.bp_pack_features.argtypes = [vp, _SZ, _SZ, _SZ, ct.c_int, _SZ, _SZ, vp, vp, vp]
+    lib.bp_batch_create.argtypes = [ct.c_int, vp, ct.POINTER(FeatureViewC), vp, vp, vp, vp,
                                     ct.POINTER(vp)]
     lib.bp_batch_launch.argtypes = [vp, vp]
     lib.bp_batch_graph_launch.argtypes = [vp, vp]

@@ -200,6 +200,7 @@ struct bp_network {
   int n_tiles = 0;
   int max_spread = 0;
   int Lw[2] = {0, 0};
+  int pad_lo = 0, pad_hi = 0;  // feature row padding that makes every LDS tile interior
   std::vector<int> tiles[3];  // ascending tile ids per class (host copy)
   DevBuf tau, beta, lohi, tile_lo, tile_all, tile_lw, dtiles[3];
   DevBuf sched[3], scale[3];
@@ -228,6 +229,36 @@ bp::StackArgs make_args(const bp_network *net, const void *F, int fmt, int T, in
   a.oob = oob;
   a.time_group = kTimeGroup;
   return a;
+}
+
+// Arguments for a feature view: rows of row_stride elements, sample 0 at
+// element pad_before, readable zeros around [0, T).
+bp::StackArgs make_args_view(const bp_network *net, const bp_feature_view &v, int oob) {
+  bp::StackArgs a = make_args(net, nullptr, v.fmt, (int)v.n_samples, oob);
+  const size_t es = v.fmt == BP_F32 ? 4 : 2;
+  a.F = static_cast<const char *>(v.data) + v.pad_before * es;
+  a.ldf = (long long)v.row_stride;
+  a.pad_lo = (int)v.pad_before;
+  a.pad_hi = (int)(v.row_stride - v.pad_before - v.n_samples);
+  return a;
+}
+
+int check_shapes(size_t T, size_t K, size_t S, size_t P);
+
+int check_view(const bp_network *net, const bp_feature_view &v) {
+  if (!v.data) return fail(BP_EINVAL, "NULL features");
+  if (v.fmt != BP_F32 && v.fmt != BP_F16 && v.fmt != BP_BF16)
+    return fail(BP_EINVAL, "unknown feature format %d", v.fmt);
+  if (v.fmt == BP_F32 && v.station_scale)
+    return fail(BP_EINVAL, "station_scale applies to the 16-bit formats only");
+  BP_TRY(check_shapes(v.n_samples, net->K, net->S, net->P));
+  if (!size_ok(v.row_stride) || !size_ok(v.pad_before) ||
+      v.row_stride < v.pad_before + v.n_samples)
+    return fail(BP_EINVAL, "row_stride %zu cannot hold pad_before %zu + %zu samples",
+                v.row_stride, v.pad_before, v.n_samples);
+  if (v.fmt != BP_F32 && ((v.row_stride | v.pad_before) & 1))
+    return fail(BP_EINVAL, "16-bit rows need an even row_stride and pad_before");
+  return BP_OK;
 }
 
 bool has_lds_tiles(const bp_network *net) {
@@ -329,15 +360,16 @@ int launch_stack(const bp_network *net, bp::StackArgs a, int k_begin, int k_end,
   return BP_OK;
 }
 
-int network_max(bp_network *net, const void *F, int fmt, const float *scales, size_t n_samples,
-                int oob, float *bmax, int *barg, unsigned long long *keys, hipStream_t st) {
-  const int T = (int)n_samples;
-  BP_TRY(ensure_schedule(net, fmt, fmt == BP_F32 ? nullptr : scales, st));
+int network_max(bp_network *net, const bp_feature_view &v, int oob, float *bmax, int *barg,
+                unsigned long long *keys, hipStream_t st) {
+  const int T = (int)v.n_samples;
+  const int fmt = v.fmt;
+  BP_TRY(ensure_schedule(net, fmt, fmt == BP_F32 ? nullptr : v.station_scale, st));
   const unsigned nb = (unsigned)((T + 255) / 256);
   hipLaunchKernelGGL(bp::fill_u64_kernel, dim3(nb), dim3(256), 0, st, keys, (long long)T,
                      0ull);
   HIP_TRY(hipGetLastError());
-  bp::StackArgs a = make_args(net, F, fmt, T, oob);
+  bp::StackArgs a = make_args_view(net, v, oob);
   a.keys = keys;
   if (fmt == BP_F32)
     BP_TRY((launch_stack<BP_F32, false>(net, a, 0, net->K, st)));
@@ -352,6 +384,17 @@ int network_max(bp_network *net, const void *F, int fmt, const float *scales, si
 }
 
 int round_up8(long long v) { return (int)((v + 7) / 8 * 8); }
+
+bp_feature_view plain_view(const void *F, int fmt, const float *scale, size_t T) {
+  bp_feature_view v{};
+  v.data = F;
+  v.fmt = fmt;
+  v.n_samples = T;
+  v.row_stride = fmt == BP_F32 ? T : parity_stride(T);
+  v.pad_before = 0;
+  v.station_scale = scale;
+  return v;
+}
 
 // Classify tiles by delay spread and choose the two LDS window lengths:
 // multiples of 8 samples (whole 16-byte words of 16-bit samples) of at
@@ -388,6 +431,17 @@ void classify_tiles(bp_network *net, const std::vector<int> &tile_all) {
       cls = 1;
     net->tiles[cls].push_back(j);
   }
+  // Padding of the feature rows under which no LDS tile needs the guarded
+  // edge path: a tile stages [t0 + lo, t0 + maxlo + Lw + 7) with lo rounded
+  // down to even, t0 >= 0 and t0 <= T - 1.
+  int lo_min = 0, hi_max = 0;
+  for (int c = 0; c < 2; ++c)
+    for (int j : net->tiles[c]) {
+      lo_min = std::min(lo_min, tile_all[8 * j] & ~1);
+      hi_max = std::max(hi_max, tile_all[8 * j + 2] + net->Lw[c] + 6);
+    }
+  net->pad_lo = round_up8(-(long long)lo_min);
+  net->pad_hi = round_up8(hi_max);
 }
 
 int network_create(const int *tau, const float *beta, size_t K, size_t S, size_t P,
@@ -453,6 +507,30 @@ void degenerate_max(size_t T, size_t K, int oob, float *bmax, int *barg) {
     if (oob == 1 && K > 0) bmax[t] = 0.0f;
     barg[t] = 0;
   }
+}
+
+// Host (S, T, P) features of the drop-in entry points -> device planar rows
+// padded for the network (no LDS tile takes the guarded edge path).
+int stage_dropin_features(const bp_network *net, const float *host_stp, size_t S, size_t T,
+                          size_t P, DevBuf &f_stp, DevBuf &f_pad, bp_feature_view &v,
+                          hipStream_t st) {
+  const size_t nf = S * T * P;
+  const size_t ld = (size_t)net->pad_lo + T + (size_t)net->pad_hi;
+  BP_TRY(f_stp.alloc(nf * sizeof(float)));
+  BP_TRY(f_pad.alloc(S * P * ld * sizeof(float)));
+  HIP_TRY(hipMemcpyAsync(f_stp.p, host_stp, nf * sizeof(float), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(bp::interleaved_to_planar_kernel,
+                     dim3((unsigned)((ld + 255) / 256), (unsigned)S), dim3(256), 0, st,
+                     f_stp.as<float>(), (int)S, (int)T, (int)P, (long long)ld,
+                     (long long)net->pad_lo, f_pad.as<float>());
+  HIP_TRY(hipGetLastError());
+  v = bp_feature_view{};
+  v.data = f_pad.p;
+  v.fmt = BP_F32;
+  v.n_samples = T;
+  v.row_stride = ld;
+  v.pad_before = (size_t)net->pad_lo;
+  return BP_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -697,11 +775,89 @@ int bp_network_beamform_max(bp_network *net, const void *features, int fmt,
     return fail(BP_EINVAL, "station_scale applies to the 16-bit formats only");
   BP_TRY(check_shapes(n_samples, net->K, net->S, net->P));
   if (n_samples == 0) return BP_OK;
-  return network_max(net, features, fmt, station_scale, n_samples, out_of_bounds, beam_max,
-                     beam_argmax, keys, static_cast<hipStream_t>(stream));
+  return network_max(net, plain_view(features, fmt, station_scale, n_samples), out_of_bounds,
+                     beam_max, beam_argmax, keys, static_cast<hipStream_t>(stream));
+}
+
+int bp_network_beamform_max_view(bp_network *net, const bp_feature_view *features,
+                                 int out_of_bounds, float *beam_max, int *beam_argmax,
+                                 unsigned long long *keys, void *stream) {
+  if (!net || !features || !beam_max || !beam_argmax || !keys)
+    return fail(BP_EINVAL, "NULL argument");
+  if (out_of_bounds != 0 && out_of_bounds != 1)
+    return fail(BP_EINVAL, "out_of_bounds must be 0 (strict) or 1 (flexible), not %d",
+                out_of_bounds);
+  BP_TRY(check_view(net, *features));
+  if (features->n_samples == 0) return BP_OK;
+  return network_max(net, *features, out_of_bounds, beam_max, beam_argmax, keys,
+                     static_cast<hipStream_t>(stream));
+}
+
+int bp_network_padding(const bp_network *net, size_t *pad_before, size_t *pad_after) {
+  if (!net) return fail(BP_EINVAL, "NULL network");
+  if (pad_before) *pad_before = (size_t)net->pad_lo;
+  if (pad_after) *pad_after = (size_t)net->pad_hi;
+  return BP_OK;
 }
 
 size_t bp_parity_stride(size_t n_samples) { return parity_stride(n_samples); }
+
+int bp_pack_features(const float *F_spt, size_t n_stations, size_t n_phases, size_t n_samples,
+                     int fmt, size_t pad_before, size_t row_stride, float *station_scale,
+                     void *out, void *stream) {
+  if (!F_spt || !out) return fail(BP_EINVAL, "NULL argument");
+  if (fmt != BP_F32 && fmt != BP_F16 && fmt != BP_BF16)
+    return fail(BP_EINVAL, "unknown feature format %d", fmt);
+  if (fmt != BP_F32 && !station_scale)
+    return fail(BP_EINVAL, "16-bit packing returns station scales: station_scale is NULL");
+  BP_TRY(check_shapes(n_samples, 0, n_stations, n_phases));
+  if (!size_ok(row_stride) || !size_ok(pad_before) || row_stride < pad_before + n_samples)
+    return fail(BP_EINVAL, "row_stride %zu cannot hold pad_before %zu + %zu samples", row_stride,
+                pad_before, n_samples);
+  if (fmt != BP_F32 && ((row_stride | pad_before) & 1))
+    return fail(BP_EINVAL, "16-bit rows need an even row_stride and pad_before");
+  const size_t S = n_stations, P = n_phases, T = n_samples;
+  if (S == 0 || P == 0 || row_stride == 0) return BP_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  DevBuf dscale;
+  if (fmt != BP_F32) {
+    DevBuf maxbits;
+    BP_TRY(maxbits.alloc(S * sizeof(unsigned)));
+    BP_TRY(dscale.alloc(S * sizeof(float)));
+    std::vector<unsigned> mb(S, 0u);
+    HIP_TRY(hipMemcpyAsync(maxbits.p, mb.data(), S * sizeof(unsigned), hipMemcpyHostToDevice,
+                           st));
+    if (T > 0) {
+      hipLaunchKernelGGL(bp::station_maxabs_kernel, dim3(64, (unsigned)S), dim3(256), 0, st,
+                         F_spt, (int)P, (long long)T, maxbits.as<unsigned>());
+      HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipMemcpyAsync(mb.data(), maxbits.p, S * sizeof(unsigned), hipMemcpyDeviceToHost,
+                           st));
+    HIP_TRY(hipStreamSynchronize(st));
+    // power-of-two scales: FP16 maps the station's max |F| into [2^14, 2^15)
+    // (headroom below 65504); BF16 keeps the float32 exponent range: scale 1.
+    // The caller gets the weight multipliers 1/scale (exact).
+    std::vector<float> sigma(S, 1.0f);
+    for (size_t s = 0; s < S; ++s) {
+      const float m = __builtin_bit_cast(float, mb[s]);
+      if (fmt == BP_F16 && m > 0.0f && m <= 3.0e38f) {
+        int e = 0;
+        std::frexp(m, &e);  // m = f * 2^e, f in [0.5, 1)
+        sigma[s] = std::ldexp(1.0f, 15 - e);
+      }
+      station_scale[s] = 1.0f / sigma[s];
+    }
+    HIP_TRY(hipMemcpyAsync(dscale.p, sigma.data(), S * sizeof(float), hipMemcpyHostToDevice, st));
+  }
+  hipLaunchKernelGGL(bp::pack_features_kernel,
+                     dim3((unsigned)((row_stride + 255) / 256), (unsigned)(S * P)), dim3(256), 0,
+                     st, F_spt, (int)P, (long long)T, dscale.as<float>(), fmt,
+                     (long long)row_stride, (long long)pad_before, out);
+  HIP_TRY(hipGetLastError());
+  if (fmt != BP_F32) HIP_TRY(hipStreamSynchronize(st));  // dscale is freed on return
+  return BP_OK;
+}
 
 int bp_quantize_planar(const float *F_spt, size_t n_stations, size_t n_phases, size_t n_samples,
                        int fmt, float *station_scale, void *out, void *stream) {
@@ -709,40 +865,9 @@ int bp_quantize_planar(const float *F_spt, size_t n_stations, size_t n_phases, s
   if (fmt != BP_F16 && fmt != BP_BF16)
     return fail(BP_EINVAL, "quantization targets BP_F16 or BP_BF16, not %d", fmt);
   BP_TRY(check_shapes(n_samples, 0, n_stations, n_phases));
-  const size_t S = n_stations, P = n_phases, T = n_samples;
-  if (S == 0 || P == 0 || T == 0) return BP_OK;
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  DevBuf maxbits, dscale;
-  BP_TRY(maxbits.alloc(S * sizeof(unsigned)));
-  BP_TRY(dscale.alloc(S * sizeof(float)));
-  std::vector<unsigned> mb(S, 0u);
-  HIP_TRY(hipMemcpyAsync(maxbits.p, mb.data(), S * sizeof(unsigned), hipMemcpyHostToDevice, st));
-  hipLaunchKernelGGL(bp::station_maxabs_kernel, dim3(64, (unsigned)S), dim3(256), 0, st, F_spt,
-                     (int)P, (long long)T, maxbits.as<unsigned>());
-  HIP_TRY(hipGetLastError());
-  HIP_TRY(hipMemcpyAsync(mb.data(), maxbits.p, S * sizeof(unsigned), hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipStreamSynchronize(st));
-  // power-of-two scales: FP16 maps the station's max |F| into [2^14, 2^15)
-  // (headroom below 65504); BF16 keeps the float32 exponent range: scale 1.
-  // The caller gets the weight multipliers 1/scale (exact).
-  std::vector<float> sigma(S, 1.0f);
-  for (size_t s = 0; s < S; ++s) {
-    const float m = __builtin_bit_cast(float, mb[s]);
-    if (fmt == BP_F16 && m > 0.0f && m <= 3.0e38f) {
-      int e = 0;
-      std::frexp(m, &e);  // m = f * 2^e, f in [0.5, 1)
-      sigma[s] = std::ldexp(1.0f, 15 - e);
-    }
-    station_scale[s] = 1.0f / sigma[s];
-  }
-  HIP_TRY(hipMemcpyAsync(dscale.p, sigma.data(), S * sizeof(float), hipMemcpyHostToDevice, st));
-  const size_t ldq = parity_stride(T);
-  hipLaunchKernelGGL(bp::quantize_kernel, dim3((unsigned)((ldq + 255) / 256), (unsigned)(S * P)),
-                     dim3(256), 0, st, F_spt, (int)P, (long long)T, (long long)ldq,
-                     dscale.as<float>(), fmt, static_cast<unsigned short *>(out));
-  HIP_TRY(hipGetLastError());
-  HIP_TRY(hipStreamSynchronize(st));
-  return BP_OK;
+  if (n_stations == 0 || n_phases == 0 || n_samples == 0) return BP_OK;
+  return bp_pack_features(F_spt, n_stations, n_phases, n_samples, fmt, 0,
+                          parity_stride(n_samples), station_scale, out, stream);
 }
 
 int bp_network_beamform(bp_network *net, const float *features, size_t n_samples,
@@ -771,7 +896,7 @@ int bp_interleaved_to_planar(const float *F_stp, size_t n_stations, size_t n_sam
   hipLaunchKernelGGL(bp::interleaved_to_planar_kernel,
                      dim3((unsigned)((n_samples + 255) / 256), (unsigned)n_stations),
                      dim3(256), 0, static_cast<hipStream_t>(stream), F_stp, (int)n_stations,
-                     (int)n_samples, (int)n_phases, F_spt);
+                     (int)n_samples, (int)n_phases, (long long)n_samples, 0LL, F_spt);
   HIP_TRY(hipGetLastError());
   return BP_OK;
 }
@@ -815,20 +940,18 @@ int beamform_max(float *waveform_features, int *time_delays, float *weights_sour
   BP_TRY(network_create(time_delays, weights_sources, n_sources, n_stations, n_phases, st.s,
                         net));
   const size_t T = n_samples, nf = n_stations * n_samples * n_phases;
-  DevBuf f_stp, f_spt, keys, bmax, barg;
-  BP_TRY(f_stp.alloc(nf * sizeof(float)));
-  BP_TRY(f_spt.alloc(nf * sizeof(float)));
+  bp_feature_view v{};
+  DevBuf f_stp, f_pad, keys, bmax, barg;
+  BP_TRY(stage_dropin_features(net.get(), waveform_features, n_stations, T, n_phases, f_stp,
+                               f_pad, v, st.s));
   BP_TRY(keys.alloc(T * sizeof(unsigned long long)));
   BP_TRY(bmax.alloc(T * sizeof(float)));
   BP_TRY(barg.alloc(T * sizeof(int)));
-  HIP_TRY(hipMemcpyAsync(f_stp.p, waveform_features, nf * sizeof(float),
-                         hipMemcpyHostToDevice, st.s));
   // the caller's beam_max is kept where no source is valid (beamform.c:332-333)
   HIP_TRY(hipMemcpyAsync(bmax.p, beam_max, T * sizeof(float), hipMemcpyHostToDevice, st.s));
-  BP_TRY(bp_interleaved_to_planar(f_stp.as<float>(), n_stations, n_samples, n_phases,
-                                  f_spt.as<float>(), st.s));
-  BP_TRY(network_max(net.get(), f_spt.as<float>(), BP_F32, nullptr, T, out_of_bounds,
-                     bmax.as<float>(), barg.as<int>(), keys.as<unsigned long long>(), st.s));
+  (void)nf;
+  BP_TRY(network_max(net.get(), v, out_of_bounds, bmax.as<float>(), barg.as<int>(),
+                     keys.as<unsigned long long>(), st.s));
   HIP_TRY(hipMemcpyAsync(beam_max, bmax.p, T * sizeof(float), hipMemcpyDeviceToHost, st.s));
   HIP_TRY(hipMemcpyAsync(beam_argmax, barg.p, T * sizeof(int), hipMemcpyDeviceToHost, st.s));
   HIP_TRY(hipStreamSynchronize(st.s));
@@ -858,21 +981,17 @@ int beamform(float *waveform_features, int *time_delays, float *weights_sources,
   std::unique_ptr<bp_network> net;
   BP_TRY(network_create(time_delays, weights_sources, n_sources, n_stations, n_phases, st.s,
                         net));
-  const size_t nf = n_stations * n_samples * n_phases;
-  DevBuf f_stp, f_spt, dbeam;
-  BP_TRY(f_stp.alloc(nf * sizeof(float)));
-  BP_TRY(f_spt.alloc(nf * sizeof(float)));
+  bp_feature_view v{};
+  DevBuf f_stp, f_pad, dbeam;
+  BP_TRY(stage_dropin_features(net.get(), waveform_features, n_stations, T, n_phases, f_stp,
+                               f_pad, v, st.s));
   BP_TRY(dbeam.alloc(K * T * sizeof(float)));
-  HIP_TRY(hipMemcpyAsync(f_stp.p, waveform_features, nf * sizeof(float),
-                         hipMemcpyHostToDevice, st.s));
-  BP_TRY(bp_interleaved_to_planar(f_stp.as<float>(), n_stations, n_samples, n_phases,
-                                  f_spt.as<float>(), st.s));
   // invalid strict cells read back as 0, as the wrapper's zero-filled buffer
   const long long n = (long long)(K * T);
   hipLaunchKernelGGL(bp::fill_f32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                      st.s, dbeam.as<float>(), n, 0.0f);
   HIP_TRY(hipGetLastError());
-  bp::StackArgs a = make_args(net.get(), f_spt.as<float>(), BP_F32, (int)T, out_of_bounds);
+  bp::StackArgs a = make_args_view(net.get(), v, out_of_bounds);
   a.beam = dbeam.as<float>();
   BP_TRY((launch_stack<BP_F32, true>(net.get(), a, 0, (int)K, st.s)));
   HIP_TRY(hipMemcpyAsync(beam, dbeam.p, K * T * sizeof(float), hipMemcpyDeviceToHost, st.s));
@@ -914,18 +1033,17 @@ int prestack_waveform_features(float *waveform_features, float *weights_phases,
   return BP_OK;
 }
 
-int bp_batch_create(int n_windows, bp_network *const *networks, const void *const *features,
-                    int fmt, const float *const *station_scales, const size_t *n_samples,
-                    const int *out_of_bounds, float *const *beam_max, int *const *beam_argmax,
+int bp_batch_create(int n_windows, bp_network *const *networks,
+                    const bp_feature_view *features, const int *out_of_bounds,
+                    float *const *beam_max, int *const *beam_argmax,
                     unsigned long long *const *keys, bp_batch **out) {
   if (!out) return fail(BP_EINVAL, "out is NULL");
   *out = nullptr;
-  if (n_windows <= 0 || !networks || !features || !n_samples || !out_of_bounds || !beam_max ||
+  if (n_windows <= 0 || !networks || !features || !out_of_bounds || !beam_max ||
       !beam_argmax || !keys)
     return fail(BP_EINVAL, "a batch needs n_windows > 0 and non-NULL arrays");
   if (n_windows > 65535) return fail(BP_EINVAL, "too many windows (%d)", n_windows);
-  if (fmt != BP_F32 && fmt != BP_F16 && fmt != BP_BF16)
-    return fail(BP_EINVAL, "unknown feature format %d", fmt);
+  const int fmt = features[0].fmt;
   BP_TRY(check_device());
   auto b = std::make_unique<bp_batch>();
   b->fmt = fmt;
@@ -936,17 +1054,20 @@ int bp_batch_create(int n_windows, bp_network *const *networks, const void *cons
   std::vector<bp::BatchKeys> kd;
   for (int i = 0; i < n_windows; ++i) {
     bp_network *net = networks[i];
-    if (!net || !features[i] || !beam_max[i] || !beam_argmax[i] || !keys[i])
-      return fail(BP_EINVAL, "window %d: NULL network, features or output", i);
+    if (!net || !beam_max[i] || !beam_argmax[i] || !keys[i])
+      return fail(BP_EINVAL, "window %d: NULL network or output", i);
+    const bp_feature_view &fv = features[i];
+    if (fv.fmt != fmt)
+      return fail(BP_EINVAL, "window %d: format %d, window 0 has %d (one format per batch)", i,
+                  fv.fmt, fmt);
+    BP_TRY(check_view(net, fv));
     if (i == 0) b->P = net->P;
     if (net->P != b->P)
       return fail(BP_EINVAL, "window %d has %d phases, window 0 has %d", i, net->P, b->P);
     const int oob = out_of_bounds[i];
     if (oob != 0 && oob != 1)
       return fail(BP_EINVAL, "window %d: out_of_bounds must be 0 or 1, not %d", i, oob);
-    BP_TRY(check_shapes(n_samples[i], net->K, net->S, net->P));
-    const float *sc = station_scales ? station_scales[i] : nullptr;
-    if (fmt == BP_F32 && sc) return fail(BP_EINVAL, "station_scale applies to 16-bit formats only");
+    const float *sc = fv.station_scale;
     std::vector<float> want(net->S, 1.0f);
     if (sc) want.assign(sc, sc + net->S);
     auto it = scales_of.find(net);
@@ -954,11 +1075,11 @@ int bp_batch_create(int n_windows, bp_network *const *networks, const void *cons
       return fail(BP_EINVAL, "window %d: a network's windows in one batch must share station scales", i);
     scales_of[net] = want;
     BP_TRY(ensure_schedule(net, fmt, sc, st.s));
-    const int T = (int)n_samples[i];
+    const int T = (int)fv.n_samples;
     kd.push_back(bp::BatchKeys{keys[i], beam_max[i], beam_argmax[i], T});
     b->max_T = std::max(b->max_T, T);
     if (T == 0) continue;
-    bp::StackArgs a = make_args(net, features[i], fmt, T, oob);
+    bp::StackArgs a = make_args_view(net, fv, oob);
     a.keys = keys[i];
     a.k_begin = 0;
     a.k_end = net->K;

@@ -13,9 +13,10 @@
 namespace bp {
 
 struct StackArgs {
-  const void *F;          // features: float32 planar (S, P, T), or for the 16-bit
-                          // formats parity planes (S, P, 2, ldf): x[t] and x[t+1]
-  long long ldf;          // feature row stride in elements (T, or the parity-plane stride)
+  const void *F;          // features at sample 0 of row 0: float32 rows (S, P) or for
+                          // the 16-bit formats parity-plane rows (S, P, 2), x[t] and x[t+1]
+  long long ldf;          // feature row stride in elements
+  int pad_lo, pad_hi;     // readable zero samples before 0 and from T on, in every row
   const float *scale;     // per-station weight multiplier of the 16-bit formats (or NULL)
   const int *tau;         // (K, S, P) moveouts
   const float *beta;      // (K, S) source weights
@@ -199,46 +200,59 @@ __global__ void station_maxabs_kernel(const float *__restrict__ F, int P, long l
   if ((threadIdx.x & 63) == 0) atomicMax(maxbits + s, m);
 }
 
-// Scaled 16-bit parity planes: plane 0 holds q(x[t] * scale[s]), plane 1
-// holds q(x[t+1] * scale[s]) (0 past the end), row stride ldq elements.
+// Planar float32 features (S*P rows of T) -> rows of `ldout` elements in
+// the storage format, sample 0 at element pad_out, zeros outside [0, T):
+//   F32   : row (s,p) holds x[t];
+//   16-bit: parity planes, row (s,p,0) holds q(x[t] * scale[s]) and row
+//           (s,p,1) q(x[t+1] * scale[s]).
 // Non-finite inputs saturate to the format's largest finite value (NaN to 0).
-__global__ void quantize_kernel(const float *__restrict__ F, int P, long long T, long long ldq,
-                                const float *__restrict__ scale, int fmt,
-                                unsigned short *__restrict__ out) {
+__device__ __forceinline__ unsigned short quantize16(float x, int fmt) {
+  if (x != x) x = 0.0f;
+  if (fmt == BP_F16) {
+    x = fminf(fmaxf(x, -65504.0f), 65504.0f);
+    return __builtin_bit_cast(unsigned short, (_Float16)x);
+  }
+  x = fminf(fmaxf(x, -3.3895313892515355e38f), 3.3895313892515355e38f);
+  return (unsigned short)bf16_bits(x);
+}
+
+__global__ void pack_features_kernel(const float *__restrict__ F, int P, long long T,
+                                     const float *__restrict__ scale, int fmt, long long ldout,
+                                     long long pad_out, void *__restrict__ out) {
   const int sp = blockIdx.y;
-  const int s = sp / P;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ldq) return;
-  const float sc = scale[s];
-  unsigned short q[2];
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ldout) return;
+  const long long t = i - pad_out;
+  const float *row = F + (size_t)sp * T;
+  if (fmt == BP_F32) {
+    static_cast<float *>(out)[(size_t)sp * ldout + i] = (t >= 0 && t < T) ? row[t] : 0.0f;
+    return;
+  }
+  const float sc = scale[sp / P];
+  unsigned short *o = static_cast<unsigned short *>(out);
   for (int par = 0; par < 2; ++par) {
     const long long tt = t + par;
-    float x = tt < T ? F[(size_t)sp * T + tt] * sc : 0.0f;
-    if (x != x) x = 0.0f;
-    if (fmt == BP_F16) {
-      x = fminf(fmaxf(x, -65504.0f), 65504.0f);
-      q[par] = __builtin_bit_cast(unsigned short, (_Float16)x);
-    } else {
-      x = fminf(fmaxf(x, -3.3895313892515355e38f), 3.3895313892515355e38f);
-      q[par] = (unsigned short)bf16_bits(x);
-    }
+    const float x = (tt >= 0 && tt < T) ? row[tt] * sc : 0.0f;
+    o[((size_t)sp * 2 + par) * ldout + i] = quantize16(x, fmt);
   }
-  out[((size_t)sp * 2 + 0) * ldq + t] = q[0];
-  out[((size_t)sp * 2 + 1) * ldq + t] = q[1];
 }
 
 // ---------------------------------------------------------------------------
 // Layout and prestack.
 // ---------------------------------------------------------------------------
 
-// (S,T,P) phase-interleaved -> (S,P,T) planar.
+// (S,T,P) phase-interleaved -> planar rows (S,P) of `ld` elements, sample 0
+// at element `pad`, zeros outside [0, T).
 __global__ void interleaved_to_planar_kernel(const float *__restrict__ in, int S, int T,
-                                             int P, float *__restrict__ out) {
+                                             int P, long long ld, long long pad,
+                                             float *__restrict__ out) {
   const int s = blockIdx.y;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= T) return;
-  const float *src = in + ((size_t)s * T + t) * P;
-  for (int p = 0; p < P; ++p) out[((size_t)s * P + p) * T + t] = src[p];
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ld) return;
+  const long long t = i - pad;
+  const bool in_range = t >= 0 && t < T;
+  const float *src = in + ((size_t)s * T + (in_range ? t : 0)) * P;
+  for (int p = 0; p < P; ++p) out[((size_t)s * P + p) * ld + i] = in_range ? src[p] : 0.0f;
 }
 
 // beamform.c:108-152: F[s,.,p] = sum_c alpha[s,c,p] * u[s,c,t], accumulated
@@ -698,9 +712,10 @@ struct StackLds {
     const int tall0 = tall[0];
     const int kbase = tile * KT + wave * KW;
     // staged range of every slot: [t0 + lo, t0 + lo + Lw + SPA - 1), with
-    // lo one lower at most for 16-bit samples (even start)
-    const bool edge = (long long)t0 + tall0 - 1 < 0 ||
-                      (long long)t0 + tall[2] + Lw + SPA - 1 > (long long)a.T;
+    // lo rounded down to even for 16-bit samples; rows are readable (and
+    // zero) from -pad_lo to T + pad_hi
+    const bool edge = (long long)t0 + (ES == 4 ? tall0 : (tall0 & ~1)) < -(long long)a.pad_lo ||
+                      (long long)t0 + tall[2] + Lw + SPA - 1 > (long long)a.T + a.pad_hi;
 
     if (!NONE)
       for (int i = tid; i < TT; i += NT) red[i] = 0ull;

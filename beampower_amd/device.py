@@ -10,7 +10,7 @@ import ctypes as ct
 
 import numpy as np
 
-from .core import BatchConfig, check, load_library
+from .core import BatchConfig, FeatureViewC, check, load_library
 
 BP_F32, BP_BF16, BP_F16 = 0, 1, 2
 
@@ -64,12 +64,36 @@ class Network:
         return dict(source_tile=tile.value, max_tile_spread=spread.value, n_tiles=n.value,
                     tiles_per_class=list(counts), window_per_class=list(windows))
 
+    def padding(self):
+        """(pad_before, pad_after): zero padding of feature rows under which
+        no LDS tile of this network takes the guarded edge path."""
+        pb, pa = ct.c_size_t(), ct.c_size_t()
+        check(self.lib.bp_network_padding(self._h, ct.byref(pb), ct.byref(pa)),
+              "bp_network_padding", self.lib)
+        return pb.value, pa.value
+
     def beamform_max(self, features, out_of_bounds=0, beam_max=None, beam_argmax=None,
                      keys=None, stream=None):
-        """features: planar (S,P,T) float32 CUDA tensor, or a QuantizedFeatures
-        (16-bit storage).  Returns device tensors (beam_max (T,), beam_argmax
-        (T,)); asynchronous."""
+        """features: planar (S,P,T) float32 CUDA tensor, a QuantizedFeatures
+        (16-bit storage) or a FeatureView (padded rows, any format).  Returns
+        device tensors (beam_max (T,), beam_argmax (T,)); asynchronous."""
         torch = _torch()
+        if isinstance(features, FeatureView):
+            if features.n_stations_phases != (self.n_stations, self.n_phases):
+                raise ValueError("features do not match the network")
+            T, dev = features.n_samples, features.data.device
+            if beam_max is None:
+                beam_max = torch.zeros(T, dtype=torch.float32, device=dev)
+            if beam_argmax is None:
+                beam_argmax = torch.empty(T, dtype=torch.int32, device=dev)
+            if keys is None:
+                keys = torch.empty(T, dtype=torch.int64, device=dev)
+            view = features.c_view()
+            check(self.lib.bp_network_beamform_max_view(
+                self._h, ct.byref(view), int(out_of_bounds), beam_max.data_ptr(),
+                beam_argmax.data_ptr(), keys.data_ptr(), _stream_ptr(stream)),
+                "bp_network_beamform_max_view", self.lib)
+            return beam_max, beam_argmax
         fmt, scale = BP_F32, None
         if isinstance(features, QuantizedFeatures):
             fmt, T, dev = features.fmt, features.n_samples, features.data.device
@@ -121,6 +145,59 @@ class QuantizedFeatures:
 
     def __init__(self, data, weight_scale, fmt, n_samples):
         self.data, self.weight_scale, self.fmt, self.n_samples = data, weight_scale, fmt, n_samples
+
+
+class FeatureView:
+    """Device features with zero-padded rows (bp_feature_view): `data` holds
+    S*P rows (F32) or S*P*2 parity-plane rows (16-bit) of `row_stride`
+    elements, sample 0 at element `pad_before`."""
+
+    def __init__(self, data, fmt, n_stations, n_phases, n_samples, row_stride, pad_before,
+                 weight_scale=None):
+        self.data, self.fmt = data, int(fmt)
+        self.n_stations_phases = (int(n_stations), int(n_phases))
+        self.n_samples, self.row_stride, self.pad_before = int(n_samples), int(row_stride), int(pad_before)
+        self.weight_scale = weight_scale
+
+    @classmethod
+    def of(cls, features):
+        """A view of a planar float32 tensor (S,P,T) or a QuantizedFeatures."""
+        if isinstance(features, FeatureView):
+            return features
+        if isinstance(features, QuantizedFeatures):
+            S, P, _, ldq = features.data.shape
+            return cls(features.data, features.fmt, S, P, features.n_samples, ldq, 0,
+                       features.weight_scale)
+        S, P, T = features.shape
+        return cls(features.contiguous(), BP_F32, S, P, T, T, 0)
+
+    def c_view(self):
+        scale = None if self.weight_scale is None else self.weight_scale.ctypes.data
+        return FeatureViewC(self.data.data_ptr(), self.fmt, self.n_samples, self.row_stride,
+                            self.pad_before, scale)
+
+
+def pack(F_spt, fmt=BP_F32, pad_before=0, pad_after=0, stream=None):
+    """Planar (S,P,T) float32 CUDA tensor -> FeatureView in `fmt` with
+    `pad_before` / `pad_after` zero samples around every row (16-bit rows are
+    padded to even lengths)."""
+    torch = _torch()
+    lib = load_library("gpu")
+    S, P, T = F_spt.shape
+    pad_before, pad_after = int(pad_before), int(pad_after)
+    if fmt != BP_F32:
+        pad_before += pad_before & 1
+    row_stride = pad_before + T + pad_after
+    if fmt != BP_F32:
+        row_stride += row_stride & 1
+    rows = S * P * (1 if fmt == BP_F32 else 2)
+    dtype = torch.float32 if fmt == BP_F32 else torch.int16
+    data = torch.empty(rows * row_stride, dtype=dtype, device=F_spt.device)
+    scale = None if fmt == BP_F32 else np.empty(S, np.float32)
+    check(lib.bp_pack_features(F_spt.contiguous().data_ptr(), S, P, T, int(fmt), pad_before,
+                               row_stride, None if scale is None else scale.ctypes.data,
+                               data.data_ptr(), _stream_ptr(stream)), "bp_pack_features", lib)
+    return FeatureView(data, fmt, S, P, T, row_stride, pad_before, scale)
 
 
 def quantize(F_spt, fmt=BP_BF16, stream=None):
@@ -179,45 +256,35 @@ class Batch:
         if n == 0:
             raise ValueError("a batch needs at least one window")
         self.windows = list(windows)  # keeps networks and features alive
-        fmts, ptrs, scales, lengths = [], [], [], []
+        views = []
         self.beam_max, self.beam_argmax, self._keys = [], [], []
         for net, feats, _ in self.windows:
-            if isinstance(feats, QuantizedFeatures):
-                fmts.append(feats.fmt)
-                ptrs.append(feats.data.data_ptr())
-                scales.append(feats.weight_scale.ctypes.data)
-                T, dev = feats.n_samples, feats.data.device
-                S, P = feats.data.shape[:2]
-            else:
-                if feats.dtype != torch.float32 or not feats.is_contiguous():
-                    raise ValueError("features must be contiguous float32 (S, P, T)")
-                fmts.append(BP_F32)
-                ptrs.append(feats.data_ptr())
-                scales.append(None)
-                S, P, T = feats.shape
-                dev = feats.device
-            if (S, P) != (net.n_stations, net.n_phases):
+            if not isinstance(feats, (FeatureView, QuantizedFeatures)) and (
+                    feats.dtype != torch.float32 or not feats.is_contiguous()):
+                raise ValueError("features must be contiguous float32 (S, P, T)")
+            v = FeatureView.of(feats)
+            if v.n_stations_phases != (net.n_stations, net.n_phases):
                 raise ValueError("features do not match their network")
-            lengths.append(T)
+            views.append(v)
+            T, dev = v.n_samples, v.data.device
             self.beam_max.append(torch.zeros(T, dtype=torch.float32, device=dev))
             self.beam_argmax.append(torch.empty(T, dtype=torch.int32, device=dev))
             self._keys.append(torch.empty(max(T, 1), dtype=torch.int64, device=dev))
-        if len(set(fmts)) != 1:
+        if len({v.fmt for v in views}) != 1:
             raise ValueError("one feature format per batch")
+        self._views = views
         vp = ct.c_void_p
         arr = lambda xs: (vp * n)(*xs)
         self._args = dict(
-            nets=arr([net._h.value for net, _, _ in self.windows]), feats=arr(ptrs),
-            scales=arr(scales) if fmts[0] != BP_F32 else None,
-            lengths=(ct.c_size_t * n)(*lengths),
+            nets=arr([net._h.value for net, _, _ in self.windows]),
+            views=(FeatureViewC * n)(*[v.c_view() for v in views]),
             oob=(ct.c_int * n)(*[int(o) for _, _, o in self.windows]),
             bmax=arr([t.data_ptr() for t in self.beam_max]),
             barg=arr([t.data_ptr() for t in self.beam_argmax]),
             keys=arr([t.data_ptr() for t in self._keys]))
         h = vp()
         a = self._args
-        check(self.lib.bp_batch_create(n, a["nets"], a["feats"], fmts[0], a["scales"],
-                                       a["lengths"], a["oob"], a["bmax"], a["barg"],
+        check(self.lib.bp_batch_create(n, a["nets"], a["views"], a["oob"], a["bmax"], a["barg"],
                                        a["keys"], ct.byref(h)), "bp_batch_create", self.lib)
         self._h = h
 

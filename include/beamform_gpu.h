@@ -109,6 +109,45 @@ int bp_network_beamform_max(bp_network *net, const void *features, int fmt,
                             int out_of_bounds, float *beam_max, int *beam_argmax,
                             unsigned long long *keys, void *stream);
 
+/* A device feature buffer in a storage format, with optional zero padding
+ * around every row.  Row r (r = s*P + p for BP_F32; r = (s*P + p)*2 + plane
+ * for the 16-bit parity planes) starts at element r * row_stride of `data`;
+ * its sample 0 is element pad_before of the row.  The elements before
+ * sample 0 and from sample n_samples to the end of the row must read as
+ * zero features (bp_pack_features writes them so).  Padding at least
+ * bp_network_padding() on both sides lets every LDS tile of the network
+ * stage its windows without the guarded edge path.  16-bit rows need even
+ * row_stride and pad_before.  station_scale: HOST (n_stations) weight
+ * multipliers of 16-bit features (from bp_pack_features), or NULL. */
+typedef struct {
+  const void *data;
+  int fmt;
+  size_t n_samples;
+  size_t row_stride;
+  size_t pad_before;
+  const float *station_scale;
+} bp_feature_view;
+
+/* reduce='max' on a feature view; otherwise as bp_network_beamform_max. */
+int bp_network_beamform_max_view(bp_network *net, const bp_feature_view *features,
+                                 int out_of_bounds, float *beam_max, int *beam_argmax,
+                                 unsigned long long *keys, void *stream);
+
+/* Zero padding (samples) a feature view of this network needs before and
+ * after every row so that no LDS tile takes the guarded edge path. */
+int bp_network_padding(const bp_network *net, size_t *pad_before, size_t *pad_after);
+
+/* Planar float32 features (n_stations, n_phases, n_samples) on the device ->
+ * a feature view's buffer `out` (DEVICE, rows of row_stride elements: S*P
+ * rows for BP_F32, S*P*2 for the 16-bit formats) with sample 0 at element
+ * pad_before and zeros elsewhere.  16-bit formats choose per-station
+ * power-of-two scales as bp_quantize_planar does, write their weight
+ * multipliers to station_scale (HOST, n_stations) and return after the
+ * packing; BP_F32 ignores station_scale and is asynchronous. */
+int bp_pack_features(const float *F_spt, size_t n_stations, size_t n_phases, size_t n_samples,
+                     int fmt, size_t pad_before, size_t row_stride, float *station_scale,
+                     void *out, void *stream);
+
 /* reduce='none' on device-resident planar f32 features for the sources
  * [source_begin, source_end) of the network.  beam: DEVICE (n, n_samples);
  * invalid cells are left untouched. */
@@ -138,15 +177,14 @@ int bp_prestack_planar(const float *u_sct, const float *weights_phases,
                        size_t n_phases, float *F_spt, void *stream);
 
 /* Batched reduce='max' (no counterpart in the reference, which takes one
- * window per call): n_windows windows, each with its own network,
- * device-resident features, length and DEVICE outputs (beam_max,
- * beam_argmax and keys scratch of n_samples[i] elements, as for
- * bp_network_beamform_max), all in one feature format and with one number of
- * phases.  The LDS-class tiles of every window run in one persistent launch
- * per window class; results are bit-identical to bp_network_beamform_max on
- * each window alone.  station_scales: NULL, or per window a HOST pointer
- * (n_stations) or NULL, as for bp_network_beamform_max; windows of one
- * network must share it.  Creation builds missing schedules synchronously
+ * window per call): n_windows windows, each with its own network, feature
+ * view, bounds mode and DEVICE outputs (beam_max, beam_argmax and keys
+ * scratch of n_samples elements, as for bp_network_beamform_max), all in one
+ * feature format and with one number of phases.  The LDS-class tiles of
+ * every window run in one persistent launch per window class; results are
+ * bit-identical to bp_network_beamform_max_view on each window alone.
+ * Windows of one network must share station scales.  Creation builds
+ * missing schedules synchronously
  * and uploads the window table; the networks, features and outputs must
  * outlive the batch.  bp_batch_launch enqueues the batch on `stream`
  * (no allocation, no host synchronisation); bp_batch_graph_launch captures
@@ -162,9 +200,9 @@ typedef struct {
   int blocks_per_cu;  /* resident work-groups per CU of the persistent grid */
   float ms;           /* measured time of the whole batch (tuning output) */
 } bp_batch_config;
-int bp_batch_create(int n_windows, bp_network *const *networks, const void *const *features,
-                    int fmt, const float *const *station_scales, const size_t *n_samples,
-                    const int *out_of_bounds, float *const *beam_max, int *const *beam_argmax,
+int bp_batch_create(int n_windows, bp_network *const *networks,
+                    const bp_feature_view *features, const int *out_of_bounds,
+                    float *const *beam_max, int *const *beam_argmax,
                     unsigned long long *const *keys, bp_batch **out);
 int bp_batch_launch(bp_batch *batch, void *stream);
 int bp_batch_graph_launch(bp_batch *batch, void *stream);

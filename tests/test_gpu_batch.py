@@ -118,3 +118,35 @@ def test_batch_f16_without_shared_networks():
     batch.launch()
     torch.cuda.synchronize()
     assert_batch_equals(batch, ref, "f16")
+
+
+@pytest.mark.parametrize("fmt_name", ["f32", "bf16", "f16"])
+def test_padded_views_match_the_plain_layout(fmt_name):
+    """Rows padded by bp_network_padding put every LDS tile on the staged
+    (interior) path; the sums are the same FMA chains, so the results are
+    bit-identical to the unpadded layout, alone and batched."""
+    torch = _torch()
+    from beampower_amd import device as bpd
+    fmt = {"f32": bpd.BP_F32, "bf16": bpd.BP_BF16, "f16": bpd.BP_F16}[fmt_name]
+    wins = make_windows(bpd.BP_F32)
+    if fmt == bpd.BP_F16:
+        wins = wins[:-1]  # one F16 window per network (scales differ per window)
+    padded, ref = [], []
+    for net, F, oob in wins:
+        plain = F if fmt == bpd.BP_F32 else bpd.quantize(F, fmt)
+        m0, a0 = net.beamform_max(plain, oob)
+        pb, pa = net.padding()
+        lds_tiles = sum(net.info()["tiles_per_class"][:2])
+        assert pb % 8 == 0 and (pa >= 512 or lds_tiles == 0), (pb, pa)
+        v = bpd.pack(F, fmt, pb, pa)
+        if fmt != bpd.BP_F32:
+            assert np.array_equal(v.weight_scale, plain.weight_scale)
+        m1, a1 = net.beamform_max(v, oob)
+        m0, a0, m1, a1 = (x.cpu().numpy() for x in (m0, a0, m1, a1))
+        assert bits_equal(m1, m0) and np.array_equal(a1, a0), net.info()
+        padded.append((net, v, oob))
+        ref.append((m0, a0))
+    batch = bpd.Batch(padded)
+    batch.graph_launch()
+    torch.cuda.synchronize()
+    assert_batch_equals(batch, ref, f"padded {fmt_name}")

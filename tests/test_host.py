@@ -88,8 +88,7 @@ def test_c_abi_rejects_bad_arguments_without_a_device():
     assert rc == -22 and "too large" in core.last_error(lib)
     out = ct.c_void_p()
     assert lib.bp_network_create(None, None, 0, 1, 1, ct.byref(out)) == -22
-    assert lib.bp_batch_create(0, None, None, 0, None, None, None, None, None, None,
-                               ct.byref(out)) == -22
+    assert lib.bp_batch_create(0, None, None, None, None, None, None, ct.byref(out)) == -22
     assert "n_windows > 0" in core.last_error(lib)
     assert lib.bp_batch_launch(None, None) == -22
     assert lib.bp_batch_tune(None, 1, None, 0, None, None, None) == -22
