@@ -131,6 +131,8 @@ def bench_batched(args, log):
         work += K * F.shape[2]
         wins.append((net, F, 0))
         del w
+    planar = wins
+    wins = [(net, bpd.pack(F, bpd.BP_F32, *net.padding()), oob) for net, F, oob in planar]
     torch.cuda.synchronize()
     out = {"windows": n, "source_samples": work,
            "stations": [wi[0].n_stations for wi in wins][:8] + ["..."]}
@@ -172,7 +174,7 @@ def bench_batched(args, log):
                   "identical_to_windows_alone": bool(same)}
     log(f"batched f32: {t_graph * 1e3:.1f} ms (graph), {t_alone * 1e3:.1f} ms one by one")
     del b32
-    q = [(net, bpd.quantize(F, bpd.BP_BF16), oob) for net, F, oob in wins]
+    q = [(net, bpd.pack(F, bpd.BP_BF16, *net.padding()), oob) for net, F, oob in planar]
     b16 = bpd.Batch(q)
     t16 = timed(b16.graph_launch, reps)
     agree = float(np.mean(np.concatenate(
@@ -180,7 +182,7 @@ def bench_batched(args, log):
     out["bf16"] = {"graph_ms": t16 * 1e3, "value": work / t16,
                    "speedup_vs_f32": t_graph / t16, "argmax_equal_to_f32": agree}
     log(f"batched bf16: {t16 * 1e3:.1f} ms (graph)")
-    del b16, q, wins
+    del b16, q, wins, planar
     torch.cuda.synchronize()
     torch.cuda.empty_cache()
     return out
@@ -218,10 +220,14 @@ def main():
     K, _, P = w.time_delays.shape
     u = torch.from_numpy(w.waveform_features).cuda()
     alpha = torch.from_numpy(w.weights_phases).cuda()
-    F = bpd.prestack_planar(u, alpha)
+    F_plain = bpd.prestack_planar(u, alpha)
     del u
     net = bpd.Network(w.time_delays, w.weights_sources)
     info = net.info()
+    # rows zero-padded for this network (as the drop-in entry points stage
+    # them): every LDS tile takes the staged path
+    pad = net.padding()
+    F = bpd.pack(F_plain, bpd.BP_F32, *pad)
     bmax = torch.zeros(T, dtype=torch.float32, device="cuda")
     barg = torch.empty(T, dtype=torch.int32, device="cuda")
     keys = torch.empty(T, dtype=torch.int64, device="cuda")
@@ -252,7 +258,7 @@ def main():
     precision = {}
     a32 = barg.cpu().numpy().copy()
     for name, fmt in (("bf16", bpd.BP_BF16), ("f16", bpd.BP_F16)):
-        Q = bpd.quantize(F, fmt)
+        Q = bpd.pack(F_plain, fmt, *pad)
         for _ in range(args.warmup):
             net.beamform_max(Q, oob, bmax, barg, keys)
         torch.cuda.synchronize()
@@ -296,7 +302,7 @@ def main():
                                 / (kernel_ms * 1e-3)) / HBM_PEAK,
     }
 
-    del net, F, bmax, barg, keys
+    del net, F, F_plain, bmax, barg, keys
     torch.cuda.synchronize()
     torch.cuda.empty_cache()
     batched = bench_batched(args, log) if args.batch_windows > 0 else None
@@ -324,6 +330,7 @@ def main():
             "sources": K, "samples": T, "reduce": "max", "out_of_bounds": args.oob,
             "source_tile": info["source_tile"], "max_tile_spread": info["max_tile_spread"],
             "tiles_per_class": info["tiles_per_class"], "window_per_class": info["window_per_class"],
+            "row_padding": list(pad),
         },
         "events_recovered": f"{recovered}/{len(w.events)}",
         "roofline": roofline,
