@@ -571,11 +571,12 @@ int init_batch_attrs() {
 
 struct bp_batch {
   int fmt = BP_F32, P = 0, n = 0, max_T = 0;
-  bp_batch_config cfg{kTimeGroup, 2, 2, 0.0f};
+  bp_batch_config cfg{kTimeGroup, 2, 0, 0.0f};
   // per LDS class: window descriptors (host copy and device), item count, largest window
   std::vector<bp::BatchWindow> wins[2];
   DevBuf dwins[2];
-  DevBuf dsplit[2];  // per class: 9 item indices, the XCD ranges of the persistent grid
+  DevBuf dsplit[2];  // per class: 9 item indices, the XCD ranges of the grid
+  long long max_range[2] = {0, 0};  // largest XCD range (items)
   long long n_items[2] = {0, 0};
   int max_lw[2] = {0, 0};
   std::vector<bp::StackArgs> direct;  // per window with direct-kernel tiles
@@ -616,9 +617,19 @@ int launch_batch_class(bp_batch *b, int c, hipStream_t st) {
   const int nb = class_nb<FMT, P>(b, c);
   const int lds = nb == 3 ? Lds<FMT, P, 3, false>::lds_bytes(b->max_lw[c])
                           : Lds<FMT, P, 2, false>::lds_bytes(b->max_lw[c]);
-  const int per_cu = std::max(1, std::min(b->cfg.blocks_per_cu, kLdsPerCU / lds));
-  const long long want = (long long)kNumCU * per_cu;
-  const unsigned grid = (unsigned)std::max<long long>(8, std::min(want, (b->n_items[c] + 7) / 8 * 8));
+  // blocks_per_cu > 0: a persistent grid of that many work-groups per CU;
+  // 0: one work-group per item (8 x the largest XCD range; the hardware
+  // then refills CUs as work-groups finish, which keeps co-resident
+  // work-groups out of phase)
+  long long want;
+  if (b->cfg.blocks_per_cu == 0) {
+    want = 8 * b->max_range[c];
+  } else {
+    const int per_cu = std::max(1, std::min(b->cfg.blocks_per_cu, kLdsPerCU / lds));
+    want = std::min((long long)kNumCU * per_cu, (b->n_items[c] + 7) / 8 * 8);
+  }
+  if (want > INT32_MAX) return fail(BP_EINVAL, "batch grid too large (%lld)", want);
+  const unsigned grid = (unsigned)std::max<long long>(8, want);
   if (nb == 3)
     hipLaunchKernelGGL((bp::stack_batch_kernel<FMT, P, kKW, kW, 3>), dim3(grid), dim3(64 * kW),
                        lds, st, b->dwins[c].as<bp::BatchWindow>(), (int)b->wins[c].size(),
@@ -674,7 +685,7 @@ int batch_enqueue(bp_batch *b, hipStream_t st, bool profile) {
 // Apply a configuration: the block order lives in the window descriptors.
 int batch_configure(bp_batch *b, const bp_batch_config &c) {
   if (c.time_group < 1 || c.time_group > 64 || (c.stage_buffers != 2 && c.stage_buffers != 3) ||
-      c.blocks_per_cu < 1 || c.blocks_per_cu > 4)
+      c.blocks_per_cu < 0 || c.blocks_per_cu > 4)
     return fail(BP_EINVAL, "invalid batch configuration (%d, %d, %d)", c.time_group,
                 c.stage_buffers, c.blocks_per_cu);
   b->cfg = c;
@@ -1137,6 +1148,7 @@ int bp_batch_create(int n_windows, bp_network *const *networks,
       }
       split[x] = std::max(split[x], split[x - 1]);
     }
+    for (int x = 0; x < 8; ++x) b->max_range[c] = std::max(b->max_range[c], split[x + 1] - split[x]);
     BP_TRY(b->dsplit[c].alloc(sizeof(split)));
     HIP_TRY(hipMemcpy(b->dsplit[c].p, split, sizeof(split), hipMemcpyHostToDevice));
   }
@@ -1200,7 +1212,7 @@ int bp_batch_tune(bp_batch *b, int reps, bp_batch_config *tried, int max_tried, 
   std::vector<bp_batch_config> cands;
   for (int tg : {1, 2, 4, 8})
     for (int nb : {2, 3})
-      for (int bpc : {1, 2}) cands.push_back(bp_batch_config{tg, nb, bpc, 0.0f});
+      for (int bpc : {0, 1, 2}) cands.push_back(bp_batch_config{tg, nb, bpc, 0.0f});
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));

@@ -555,7 +555,8 @@ struct StackLds {
     unsigned soff[P];
   };
 
-  __device__ __forceinline__ static void station_src(const StackArgs &a, const_int *tlo,
+  template <class A>
+  __device__ __forceinline__ static void station_src(const A &a, const_int *tlo,
                                                      int tall0, int s, int t0,
                                                      StationSrc &src) {
     const int lo0 = ES == 4 ? tall0 : (tall0 & ~1);
@@ -589,7 +590,8 @@ struct StackLds {
   // Edge tiles (a window crosses [0, T)): element-wise guarded copies
   // through registers, zeros outside [0, T).  Slot word u = c*(Lw/SPA) + m
   // of phase p holds x[w0 + c + SPA*m .. + SPA-1].
-  __device__ __forceinline__ static unsigned load_elem(const StackArgs &a, int s, int p,
+  template <class A>
+  __device__ __forceinline__ static unsigned load_elem(const A &a, int s, int p,
                                                        long long t) {
     if (t < 0 || t >= a.T) return 0u;
     const global_char *base = (const global_char *)(a.F);
@@ -600,7 +602,8 @@ struct StackLds {
         base + (((size_t)s * P + p) * 2 * a.ldf + t) * 2);
   }
 
-  __device__ __forceinline__ static void stage_station_guarded(const StackArgs &a,
+  template <class A>
+  __device__ __forceinline__ static void stage_station_guarded(const A &a,
                                                                const_int *tlo, int s, int t0,
                                                                int tid, unsigned char *buf) {
     const int Lw = a.Lw, nw = Lw / SPA, SB = window_slot_bytes(Lw);
@@ -695,7 +698,8 @@ struct StackLds {
 
   // One work-group's tile: time tile tt (samples [512 tt, 512 tt + 512)) of
   // source tile `tile`, all stations.
-  __device__ __forceinline__ static void tile_job(const StackArgs &a, int tt, int tile,
+  template <class A>
+  __device__ __forceinline__ static void tile_job(const A &a, int tt, int tile,
                                                   unsigned char *smem) {
     const int Lw = a.Lw;
     const int SB = window_slot_bytes(Lw);
@@ -862,11 +866,13 @@ void stack_lds_kernel(StackArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Batched form: one persistent launch over the LDS-class tiles of many
-// windows (each its own network, length and feature buffer), reduce='max'.
-// Window w contributes n_time_tiles(w) * n_list(w) work items, numbered from
-// item_begin(w).  The grid (a multiple of 8 work-groups) stays resident:
-// work-group b runs on XCD b % 8, and XCD x walks the contiguous item range
+// Batched form: one launch over the LDS-class tiles of many windows (each
+// its own network, length and feature buffer), reduce='max'.  Window w
+// contributes n_time_tiles(w) * n_list(w) work items, numbered from
+// item_begin(w).  The grid is a multiple of 8 work-groups, either persistent
+// (a few per CU, each looping over items) or one per item (8 x the largest
+// XCD range; surplus work-groups exit at once).  Work-group b runs on XCD
+// b % 8, and XCD x walks the contiguous item range
 // [split[x], split[x+1]), so that within an XCD consecutive items
 // are consecutive time tiles of one window (the same L2 locality as one
 // launch per window).  Items cost different amounts (S varies per window),
@@ -900,12 +906,10 @@ void stack_batch_kernel(const BatchWindow *wins, int n_wins, const long long *sp
       else
         hi = mid - 1;
     }
-    // the window's arguments, copied word by word through scalar loads
-    StackArgs a;
-    static_assert(sizeof(StackArgs) % 4 == 0, "StackArgs is copied as words");
-    const_int *src = (const_int *)&cw[lo].a;
-#pragma unroll
-    for (int w = 0; w < (int)(sizeof(StackArgs) / 4); ++w) reinterpret_cast<int *>(&a)[w] = src[w];
+    // the window's arguments stay in constant memory: fields are scalar
+    // loads where used, like a kernel argument's
+    typedef const __attribute__((address_space(4))) StackArgs const_args;
+    const_args &a = cw[lo].a;
     int tt, li;
     block_tile((int)(L - cw[lo].item_begin), a.n_time_tiles, a.n_list, a.time_group, tt, li);
     StackLds<FMT, P, KW, W, NB, false>::tile_job(a, tt, as_const(a.tiles)[li], smem);

@@ -311,7 +311,7 @@ class Batch:
         return dict(time_group=c.time_group, stage_buffers=c.stage_buffers,
                     blocks_per_cu=c.blocks_per_cu, ms=c.ms)
 
-    def set_config(self, time_group=1, stage_buffers=2, blocks_per_cu=2):
+    def set_config(self, time_group=1, stage_buffers=2, blocks_per_cu=0):
         c = BatchConfig(time_group, stage_buffers, blocks_per_cu, 0.0)
         check(self.lib.bp_batch_set_config(self._h, ct.byref(c)), "bp_batch_set_config",
               self.lib)

@@ -190,14 +190,15 @@ int bp_prestack_planar(const float *u_sct, const float *weights_phases,
  * (no allocation, no host synchronisation); bp_batch_graph_launch captures
  * that sequence into a hipGraph on first use and replays the graph after.
  * bp_batch_tune times every configuration of the search space (time tiles
- * per block-order group 1/2/4/8, staging buffers 2/3, resident work-groups
- * per CU 1/2) on this batch, keeps the fastest and remembers it for
+ * per block-order group 1/2/4/8, staging buffers 2/3, persistent grid of 1
+ * or 2 work-groups per CU or one work-group per item) on this batch, keeps the fastest and remembers it for
  * batches of the same shape in this process. */
 typedef struct bp_batch bp_batch;
 typedef struct {
   int time_group;     /* time tiles per block-order group (1 = time-major) */
   int stage_buffers;  /* 2 or 3 (3 only where two work-groups still fit a CU) */
-  int blocks_per_cu;  /* resident work-groups per CU of the persistent grid */
+  int blocks_per_cu;  /* persistent grid of that many work-groups per CU (1, 2), or 0:
+                         one work-group per work item */
   float ms;           /* measured time of the whole batch (tuning output) */
 } bp_batch_config;
 int bp_batch_create(int n_windows, bp_network *const *networks,
