@@ -57,7 +57,32 @@ GPU_SYMBOLS = (
     "bp_network_beamform_max_view", "bp_network_padding", "bp_pack_features",
     "bp_batch_create", "bp_batch_launch", "bp_batch_graph_launch", "bp_batch_tune",
     "bp_batch_get_config", "bp_batch_set_config", "bp_batch_destroy",
+    "bp_network_prepare",
+    "bp_server_create", "bp_server_add_network", "bp_server_submit", "bp_server_wait",
+    "bp_server_stats", "bp_server_destroy", "bp_server_last_error",
 )
+
+
+class ServerConfig(ct.Structure):
+    """bp_server_config of include/beamform_gpu.h."""
+    _fields_ = [("hbm_budget", ct.c_size_t), ("max_batch", ct.c_int), ("fmt", ct.c_int),
+                ("in_flight", ct.c_int)]
+
+
+class RequestInfo(ct.Structure):
+    """bp_request_info of include/beamform_gpu.h."""
+    _fields_ = [("status", ct.c_int), ("batch_windows", ct.c_int), ("submit_s", ct.c_double),
+                ("admit_s", ct.c_double), ("ready_s", ct.c_double), ("launch_s", ct.c_double),
+                ("gpu_done_s", ct.c_double), ("done_s", ct.c_double)]
+
+
+class ServerCounters(ct.Structure):
+    """bp_server_counters of include/beamform_gpu.h."""
+    _fields_ = [("pending", ct.c_longlong), ("ready", ct.c_longlong),
+                ("in_flight_batches", ct.c_longlong), ("completed", ct.c_longlong),
+                ("batches", ct.c_longlong), ("batched_windows", ct.c_longlong),
+                ("arena_bytes", ct.c_size_t), ("arena_used", ct.c_size_t),
+                ("arena_peak", ct.c_size_t)]
 
 
 class FeatureViewC(ct.Structure):
@@ -105,8 +130,19 @@ def _declare_gpu_extension(lib):
     lib.bp_batch_get_config.argtypes = [vp, ct.POINTER(BatchConfig)]
     lib.bp_batch_set_config.argtypes = [vp, ct.POINTER(BatchConfig)]
     lib.bp_batch_destroy.argtypes = [vp]
+    lib.bp_network_prepare.argtypes = [vp, ct.c_int, vp]
+    lib.bp_server_create.argtypes = [ct.POINTER(ServerConfig), ct.POINTER(vp)]
+    lib.bp_server_add_network.argtypes = [vp, vp, vp, vp, _SZ, _SZ, _SZ, _SZ, _P_I]
+    lib.bp_server_submit.argtypes = [vp, ct.c_int, vp, _SZ, ct.c_int, vp, vp,
+                                     ct.POINTER(ct.c_longlong)]
+    lib.bp_server_wait.argtypes = [vp, ct.c_longlong, ct.c_double, ct.POINTER(RequestInfo)]
+    lib.bp_server_stats.argtypes = [vp, ct.POINTER(ServerCounters)]
+    lib.bp_server_destroy.argtypes = [vp]
+    lib.bp_server_last_error.restype = ct.c_char_p
+    lib.bp_server_last_error.argtypes = []
     for name in GPU_SYMBOLS:
-        if name not in ("bp_last_error", "bp_version", "bp_parity_stride"):
+        if name not in ("bp_last_error", "bp_version", "bp_parity_stride",
+                        "bp_server_last_error"):
             getattr(lib, name).restype = ct.c_int
     lib.bp_parity_stride.restype = _SZ
 

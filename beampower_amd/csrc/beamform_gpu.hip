@@ -804,6 +804,18 @@ int bp_network_beamform_max_view(bp_network *net, const bp_feature_view *feature
                      static_cast<hipStream_t>(stream));
 }
 
+int bp_network_prepare(bp_network *net, int fmt, const float *station_scale) {
+  if (!net) return fail(BP_EINVAL, "NULL network");
+  if (fmt != BP_F32 && fmt != BP_F16 && fmt != BP_BF16)
+    return fail(BP_EINVAL, "unknown feature format %d", fmt);
+  if (fmt == BP_F32 && station_scale)
+    return fail(BP_EINVAL, "station_scale applies to the 16-bit formats only");
+  BP_TRY(check_device());
+  Stream st;
+  BP_TRY(st.create());
+  return ensure_schedule(net, fmt, station_scale, st.s);
+}
+
 int bp_network_padding(const bp_network *net, size_t *pad_before, size_t *pad_after) {
   if (!net) return fail(BP_EINVAL, "NULL network");
   if (pad_before) *pad_before = (size_t)net->pad_lo;
@@ -831,7 +843,11 @@ int bp_pack_features(const float *F_spt, size_t n_stations, size_t n_phases, siz
   if (S == 0 || P == 0 || row_stride == 0) return BP_OK;
   hipStream_t st = static_cast<hipStream_t>(stream);
   DevBuf dscale;
-  if (fmt != BP_F32) {
+  if (fmt == BP_BF16) {
+    // bfloat16 keeps the float32 exponent range: scale 1 (a NULL scale
+    // table), no host round trip, asynchronous
+    for (size_t s = 0; s < S; ++s) station_scale[s] = 1.0f;
+  } else if (fmt == BP_F16) {
     DevBuf maxbits;
     BP_TRY(maxbits.alloc(S * sizeof(unsigned)));
     BP_TRY(dscale.alloc(S * sizeof(float)));
@@ -866,7 +882,7 @@ int bp_pack_features(const float *F_spt, size_t n_stations, size_t n_phases, siz
                      st, F_spt, (int)P, (long long)T, dscale.as<float>(), fmt,
                      (long long)row_stride, (long long)pad_before, out);
   HIP_TRY(hipGetLastError());
-  if (fmt != BP_F32) HIP_TRY(hipStreamSynchronize(st));  // dscale is freed on return
+  if (fmt == BP_F16) HIP_TRY(hipStreamSynchronize(st));  // dscale is freed on return
   return BP_OK;
 }
 

@@ -228,7 +228,7 @@ __global__ void pack_features_kernel(const float *__restrict__ F, int P, long lo
     static_cast<float *>(out)[(size_t)sp * ldout + i] = (t >= 0 && t < T) ? row[t] : 0.0f;
     return;
   }
-  const float sc = scale[sp / P];
+  const float sc = scale ? scale[sp / P] : 1.0f;
   unsigned short *o = static_cast<unsigned short *>(out);
   for (int par = 0; par < 2; ++par) {
     const long long tt = t + par;

@@ -141,9 +141,10 @@ int bp_network_padding(const bp_network *net, size_t *pad_before, size_t *pad_af
  * a feature view's buffer `out` (DEVICE, rows of row_stride elements: S*P
  * rows for BP_F32, S*P*2 for the 16-bit formats) with sample 0 at element
  * pad_before and zeros elsewhere.  16-bit formats choose per-station
- * power-of-two scales as bp_quantize_planar does, write their weight
- * multipliers to station_scale (HOST, n_stations) and return after the
- * packing; BP_F32 ignores station_scale and is asynchronous. */
+ * power-of-two scales as bp_quantize_planar does and write their weight
+ * multipliers to station_scale (HOST, n_stations): BP_F16 measures the
+ * stations (a host round trip) and returns after the packing; BP_BF16
+ * (scale 1) and BP_F32 (station_scale ignored) are asynchronous. */
 int bp_pack_features(const float *F_spt, size_t n_stations, size_t n_phases, size_t n_samples,
                      int fmt, size_t pad_before, size_t row_stride, float *station_scale,
                      void *out, void *stream);
@@ -212,6 +213,57 @@ int bp_batch_tune(bp_batch *batch, int reps, bp_batch_config *tried, int max_tri
 int bp_batch_get_config(const bp_batch *batch, bp_batch_config *config);
 int bp_batch_set_config(bp_batch *batch, const bp_batch_config *config);
 int bp_batch_destroy(bp_batch *batch);
+
+/* Build the network's schedule for a feature format now (otherwise built
+ * on first use); station_scale as for bp_network_beamform_max.
+ * Synchronous. */
+int bp_network_prepare(bp_network *net, int fmt, const float *station_scale);
+
+/* Serving form (no counterpart in the reference): resident networks,
+ * admission by a device-memory budget and continuous batching of window
+ * requests (reduce='max').  A request is one window of one network:
+ * waveform features (n_stations, n_channels, n_samples) float32 in HOST
+ * memory, prestacked on the device with the network's phase weights, stored
+ * in `fmt` rows padded for the network, beamformed in a batched launch with
+ * whatever other requests are ready, and its beam_max / beam_argmax (HOST,
+ * n_samples; beam_max 0 where no source is valid) written back.  Requests
+ * are admitted in arrival order while their device memory fits
+ * `hbm_budget`; requests arriving while batches run join the next launch;
+ * up to `in_flight` batches run at once on separate streams.  A result does
+ * not depend on the requests it is batched with.  The caller keeps the
+ * feature and output buffers alive until bp_server_wait returns for the
+ * request.  Thread-safe; bp_server_last_error() has this thread's message. */
+typedef struct bp_server bp_server;
+typedef struct {
+  size_t hbm_budget;  /* bytes of device memory for admitted windows */
+  int max_batch;      /* most windows per batched launch */
+  int fmt;            /* BP_BF16 or BP_F32 storage of admitted windows */
+  int in_flight;      /* batches on the device at once (1..4) */
+} bp_server_config;
+typedef struct {
+  int status;         /* BP_OK or the error of this request */
+  int batch_windows;  /* windows in the launch that served it */
+  double submit_s, admit_s, ready_s, launch_s, gpu_done_s, done_s;  /* steady clock */
+} bp_request_info;
+typedef struct {
+  long long pending, ready, in_flight_batches, completed, batches, batched_windows;
+  size_t arena_bytes, arena_used, arena_peak;
+} bp_server_counters;
+int bp_server_create(const bp_server_config *config, bp_server **out);
+int bp_server_add_network(bp_server *server, const int *time_delays,
+                          const float *weights_sources, const float *weights_phases,
+                          size_t n_sources, size_t n_stations, size_t n_channels,
+                          size_t n_phases, int *network_id);
+int bp_server_submit(bp_server *server, int network_id, const float *waveform_features,
+                     size_t n_samples, int out_of_bounds, float *beam_max, int *beam_argmax,
+                     long long *request_id);
+/* Waits for the request (timeout_s < 0: no limit); returns its status, or 1
+ * if the timeout passed first (the request stays pending). */
+int bp_server_wait(bp_server *server, long long request_id, double timeout_s,
+                   bp_request_info *info);
+int bp_server_stats(bp_server *server, bp_server_counters *stats);
+int bp_server_destroy(bp_server *server);
+const char *bp_server_last_error(void);
 
 /* Kernel timing: when enabled, the dominant (stack) kernel of every
  * bp_network_* call is bracketed with HIP events on its own stream and the
