@@ -133,6 +133,42 @@ def batched_c4_window(seed, minutes=(1.0, 30.0), stations=(10, 80), sources=(1_0
                          max(1, n_samples // 3000))
 
 
+SERVING_NETWORKS = (  # (stations, box km, source grid): 10k, 50k, 192k, 300k sources
+    (20, 50.0, (25, 25, 16)),
+    (50, 100.0, (50, 50, 20)),
+    (80, 150.0, (80, 80, 30)),
+    (120, 200.0, (100, 100, 30)),
+)
+
+
+def serving_networks(seed=5, shapes=SERVING_NETWORKS):
+    """The serving form's resident networks: (moveouts, source weights,
+    phase weights) per network, plain-method geometry in boxes growing with
+    the station count."""
+    out = []
+    for i, (n_stations, box, grid) in enumerate(shapes):
+        _, _, tau = make_network(seed + i, n_stations, box, grid)
+        out.append((tau, np.ones(tau.shape[:2], np.float32), tutorial_phase_weights(n_stations)))
+    return out
+
+
+def serving_requests(networks, n, seed, seconds=(30.0, 600.0), pool_per_network=6):
+    """n window requests (network index, features (S,3,T)) with networks
+    uniform and lengths uniform in `seconds` at 25 Hz; features come from a
+    pool of pool_per_network envelope windows per network (reused)."""
+    rng = np.random.default_rng(seed)
+    pool = []
+    for tau, _, _ in networks:
+        S = tau.shape[1]
+        pool.append([boxcar_envelopes(rng, S, 3, int(rng.uniform(*seconds) * SAMPLING_RATE_HZ))
+                     for _ in range(pool_per_network)])
+    reqs = []
+    for _ in range(n):
+        k = int(rng.integers(len(networks)))
+        reqs.append((k, pool[k][int(rng.integers(pool_per_network))]))
+    return reqs
+
+
 def subset_sources(n_sources, n_subset, seed):
     """Seeded sorted subset of source indices (the reduce='none' case)."""
     rng = np.random.default_rng(seed)

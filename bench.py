@@ -188,6 +188,56 @@ def bench_batched(args, log):
     return out
 
 
+def bench_serving(args, log):
+    """Serving form: four resident networks (20/50/80/120 stations,
+    10k/50k/192k/300k sources), window requests of 30 s - 10 min at 25 Hz on
+    random networks arriving as a Poisson process at each offered rate,
+    admitted by an HBM budget and continuously batched; BF16 window storage.
+    Latency is submit -> results in host memory."""
+    import numpy as np
+    import torch
+    from beampower_amd import device as bpd
+    from beampower_amd import synthetic
+    from beampower_amd.serve import Server, poisson_run
+
+    log("serving form: building networks")
+    nets = synthetic.serving_networks()
+    srv = Server(hbm_budget=args.serve_budget_gb << 30, max_batch=64, fmt=bpd.BP_BF16,
+                 in_flight=2)
+    for n in nets:
+        srv.add_network(*n)
+    out = {"networks": [{"stations": t.shape[1], "sources": t.shape[0]} for t, _, _ in nets],
+           "hbm_budget_bytes": args.serve_budget_gb << 30, "storage": "bf16", "max_batch": 64,
+           "in_flight": 2, "rates": []}
+    warm = synthetic.serving_requests(nets, 8, seed=99)
+    poisson_run(srv, warm, rate=1000.0, seed=1)
+    for rate in args.serve_rates:
+        n = int(max(16, min(600, rate * args.serve_seconds)))
+        reqs = synthetic.serving_requests(nets, n, seed=int(rate))
+        tickets, summary = poisson_run(srv, reqs, rate=float(rate), seed=int(rate))
+        out["rates"].append(summary)
+        log(f"serving {rate} req/s: p50 {summary['p50_ms']:.1f} ms, p99 {summary['p99_ms']:.1f} ms, "
+            f"{summary['throughput_req_s']:.1f} req/s, batch {summary['mean_batch_windows']:.1f}")
+    # argmax agreement with the FP32 drop-in on served windows
+    agree, total = 0, 0
+    dnets = [bpd.Network(t, b) for t, b, _ in nets]
+    for t in tickets[:24]:
+        k = t.network
+        F = bpd.prestack_planar(torch.from_numpy(t.features).cuda(),
+                                torch.from_numpy(nets[k][2]).cuda())
+        m, a = dnets[k].beamform_max(bpd.pack(F, bpd.BP_F32, *dnets[k].padding()), 0)
+        a = a.cpu().numpy()
+        agree += int((a == t.beam_argmax).sum())
+        total += a.size
+    out["argmax_agreement_vs_f32"] = agree / max(1, total)
+    out["stats"] = srv.stats()
+    srv.close()
+    del dnets
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -199,6 +249,11 @@ def main():
     ap.add_argument("--batch-windows", type=int, default=32,
                     help="windows of the batched form (256 = the full batched workload; 0 = skip)")
     ap.add_argument("--tune", action="store_true", help="autotune the batched form first")
+    ap.add_argument("--serve-rates", type=float, nargs="*", default=[5.0, 50.0, 200.0],
+                    help="offered request rates of the serving form (none = skip)")
+    ap.add_argument("--serve-seconds", type=float, default=3.0,
+                    help="arrival span per offered rate")
+    ap.add_argument("--serve-budget-gb", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs on a single MI355X "
@@ -306,6 +361,7 @@ def main():
     torch.cuda.synchronize()
     torch.cuda.empty_cache()
     batched = bench_batched(args, log) if args.batch_windows > 0 else None
+    serving = bench_serving(args, log) if args.serve_rates else None
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -336,6 +392,7 @@ def main():
         "roofline": roofline,
         "precision_forms": precision,
         "batched_form": batched,
+        "serving_form": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
