@@ -2,6 +2,9 @@
 // kernel uses on gfx950: LDS ds_read_b128 (conflict-free, 16 KiB per block,
 // 8 blocks of 4 waves per CU) and VALU v_fma_f32 (16 independent chains).
 // Every LDS index is masked into s[]; the program prints one line per probe.
+// `probe_peaks copy` instead runs three copies of 256 MiB (read and written
+// once each), the known-size kernel that calibrates the FETCH_SIZE and
+// WRITE_SIZE counters.
 // Build: python -m beampower_amd.build --probe  (-> beampower_amd/lib/probe_peaks)
 #include <hip/hip_runtime.h>
 #include <cstdio>
@@ -34,7 +37,26 @@ __global__ __launch_bounds__(256) void valu_fma(float *out, int iters, float b) 
   for (int i = 0; i < 16; ++i) r += a[i];
   out[blockIdx.x * 256 + threadIdx.x] = r;
 }
-int main() {
+__global__ void copy4(const float4 *__restrict__ a, float4 *__restrict__ b, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+int copy_calibration() {
+  const long bytes = 256l << 20, n = bytes / 16;
+  float4 *a, *b;
+  if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) return 1;
+  (void)hipMemset(a, 0, bytes);
+  for (int i = 0; i < 3; ++i)
+    hipLaunchKernelGGL(copy4, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, a, b, n);
+  (void)hipDeviceSynchronize();
+  printf("copy4: %ld bytes read, %ld bytes written per launch (%s)\n", bytes, bytes,
+         hipGetErrorString(hipGetLastError()));
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return 0;
+}
+int main(int argc, char **argv) {
+  if (argc > 1 && argv[1][0] == 'c') return copy_calibration();
   float *out; const int blocks = 256 * 8; (void)hipMalloc(&out, (size_t)blocks * 256 * 4);
   hipEvent_t a, b; (void)hipEventCreate(&a); (void)hipEventCreate(&b); float ms;
   for (int rep = 0; rep < 2; ++rep) {
