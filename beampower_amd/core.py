@@ -54,7 +54,8 @@ GPU_SYMBOLS = (
     "bp_interleaved_to_planar", "bp_prestack_planar",
     "bp_parity_stride", "bp_quantize_planar",
     "bp_profiling", "bp_profiling_read",
-    "bp_network_beamform_max_view", "bp_network_padding", "bp_pack_features",
+    "bp_network_beamform_max_view", "bp_network_beamform_view", "bp_network_padding",
+    "bp_pack_features",
     "bp_batch_create", "bp_batch_launch", "bp_batch_graph_launch", "bp_batch_tune",
     "bp_batch_get_config", "bp_batch_set_config", "bp_batch_destroy",
     "bp_network_prepare",
@@ -120,6 +121,8 @@ def _declare_gpu_extension(lib):
     lib.bp_network_beamform_max_view.argtypes = [vp, ct.POINTER(FeatureViewC), ct.c_int, vp,
                                                  vp, vp, vp]
     lib.bp_network_padding.argtypes = [vp, ct.POINTER(_SZ), ct.POINTER(_SZ)]
+    lib.bp_network_beamform_view.argtypes = [vp, ct.POINTER(FeatureViewC), ct.c_int, _SZ, _SZ,
+                                             vp, vp]
     lib.bp_pack_features.argtypes = [vp, _SZ, _SZ, _SZ, ct.c_int, _SZ, _SZ, vp, vp, vp]
     lib.bp_batch_create.argtypes = [ct.c_int, vp, ct.POINTER(FeatureViewC), vp, vp, vp, vp,
                                     ct.POINTER(vp)]

@@ -173,7 +173,8 @@ hipError_t set_lds_attrs() {
 int init_kernel_attrs() {
   std::call_once(g_attr_once, [] {
     hipError_t r[] = {set_lds_attrs<BP_F32, false>(), set_lds_attrs<BP_F32, true>(),
-                      set_lds_attrs<BP_F16, false>(), set_lds_attrs<BP_BF16, false>()};
+                      set_lds_attrs<BP_F16, false>(), set_lds_attrs<BP_BF16, false>(),
+                      set_lds_attrs<BP_F16, true>(),  set_lds_attrs<BP_BF16, true>()};
     for (hipError_t e : r)
       if (e != hipSuccess) {
         g_attr_rc = fail(BP_EHIP, "hipFuncSetAttribute: %s", hipGetErrorString(e));
@@ -913,6 +914,30 @@ int bp_network_beamform(bp_network *net, const float *features, size_t n_samples
   a.beam = beam;
   return launch_stack<BP_F32, true>(net, a, (int)source_begin, (int)source_end,
                                     static_cast<hipStream_t>(stream));
+}
+
+int bp_network_beamform_view(bp_network *net, const bp_feature_view *features,
+                             int out_of_bounds, size_t source_begin, size_t source_end,
+                             float *beam, void *stream) {
+  if (!net || !features || !beam) return fail(BP_EINVAL, "NULL argument");
+  if (out_of_bounds != 0 && out_of_bounds != 1)
+    return fail(BP_EINVAL, "out_of_bounds must be 0 (strict) or 1 (flexible), not %d",
+                out_of_bounds);
+  if (source_begin > source_end || source_end > (size_t)net->K)
+    return fail(BP_EINVAL, "bad source range [%zu, %zu) for %d sources", source_begin,
+                source_end, net->K);
+  BP_TRY(check_view(net, *features));
+  if (features->n_samples == 0) return BP_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int fmt = features->fmt;
+  BP_TRY(ensure_schedule(net, fmt, fmt == BP_F32 ? nullptr : features->station_scale, st));
+  bp::StackArgs a = make_args_view(net, *features, out_of_bounds);
+  a.beam = beam;
+  if (fmt == BP_F32)
+    return launch_stack<BP_F32, true>(net, a, (int)source_begin, (int)source_end, st);
+  if (fmt == BP_F16)
+    return launch_stack<BP_F16, true>(net, a, (int)source_begin, (int)source_end, st);
+  return launch_stack<BP_BF16, true>(net, a, (int)source_begin, (int)source_end, st);
 }
 
 int bp_interleaved_to_planar(const float *F_stp, size_t n_stations, size_t n_samples,

@@ -124,8 +124,20 @@ class Network:
     def beamform(self, features, out_of_bounds=0, source_begin=0, source_end=None,
                  beam=None, stream=None):
         """Full beams of sources [source_begin, source_end) as a (n, T)
-        float32 CUDA tensor (cells invalid in strict mode stay 0)."""
+        float32 CUDA tensor (cells invalid in strict mode stay 0).  features:
+        planar (S,P,T) float32 tensor, QuantizedFeatures or FeatureView."""
         torch = _torch()
+        if isinstance(features, (FeatureView, QuantizedFeatures)):
+            v = FeatureView.of(features)
+            source_end = self.n_sources if source_end is None else int(source_end)
+            n = source_end - int(source_begin)
+            if beam is None:
+                beam = torch.zeros((n, v.n_samples), dtype=torch.float32, device=v.data.device)
+            cv = v.c_view()
+            check(self.lib.bp_network_beamform_view(
+                self._h, ct.byref(cv), int(out_of_bounds), int(source_begin), source_end,
+                beam.data_ptr(), _stream_ptr(stream)), "bp_network_beamform_view", self.lib)
+            return beam
         S, P, T = features.shape
         source_end = self.n_sources if source_end is None else int(source_end)
         n = source_end - int(source_begin)

@@ -133,6 +133,13 @@ int bp_network_beamform_max_view(bp_network *net, const bp_feature_view *feature
                                  int out_of_bounds, float *beam_max, int *beam_argmax,
                                  unsigned long long *keys, void *stream);
 
+/* reduce='none' on a feature view (any format) for the sources
+ * [source_begin, source_end): beam DEVICE (n, n_samples) float32; cells
+ * invalid in strict mode are left untouched. */
+int bp_network_beamform_view(bp_network *net, const bp_feature_view *features,
+                             int out_of_bounds, size_t source_begin, size_t source_end,
+                             float *beam, void *stream);
+
 /* Zero padding (samples) a feature view of this network needs before and
  * after every row so that no LDS tile takes the guarded edge path. */
 int bp_network_padding(const bp_network *net, size_t *pad_before, size_t *pad_after);
