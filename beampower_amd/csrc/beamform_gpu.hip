@@ -940,6 +940,108 @@ int bp_network_beamform_view(bp_network *net, const bp_feature_view *features,
   return launch_stack<BP_BF16, true>(net, a, (int)source_begin, (int)source_end, st);
 }
 
+// ---- detection consumer (notebooks/3_beampower.ipynb:497-626) ------------
+
+int bp_detect_threshold(const float *beam_max, size_t n_samples, size_t window, double overlap,
+                        double num_dev, float *threshold, void *stream) {
+  if (!beam_max || !threshold) return fail(BP_EINVAL, "NULL argument");
+  const long long n = (long long)n_samples, W = (long long)window;
+  const long long shift = (long long)((1.0 - overlap) * (double)window);
+  if (!size_ok(n_samples) || W < 1 || n <= W || shift < 1)
+    return fail(BP_EINVAL, "need n_samples (%zu) > window (%zu) and a shift of >= 1 sample",
+                n_samples, window);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const long long nw = (n - W) / shift + 1;
+  DevBuf stats, dthr;
+  BP_TRY(stats.alloc((size_t)nw * 2 * sizeof(double)));
+  BP_TRY(dthr.alloc((size_t)nw * sizeof(float)));
+  hipLaunchKernelGGL(bp::window_stats_kernel, dim3((unsigned)nw), dim3(256), 0, st, beam_max,
+                     W, shift, stats.as<double>());
+  HIP_TRY(hipGetLastError());
+  std::vector<double> h((size_t)nw * 2);
+  HIP_TRY(hipMemcpyAsync(h.data(), stats.p, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                         st));
+  HIP_TRY(hipStreamSynchronize(st));
+  // center + num_dev * deviation in float32, then the one-step maximum with
+  // the previous and (after that) the next window (:556-560)
+  std::vector<float> thr((size_t)nw);
+  const float nd = (float)num_dev;
+  for (long long j = 0; j < nw; ++j) {
+    const float c = (float)h[2 * j], d = (float)h[2 * j + 1];
+    const float dev = nd * d;
+    thr[j] = c + dev;
+  }
+  std::vector<float> a = thr;
+  for (long long j = 1; j < nw; ++j) thr[j] = std::max(a[j - 1], a[j]);
+  for (long long j = 0; j + 1 < nw; ++j) thr[j] = std::max(thr[j], thr[j + 1]);
+  HIP_TRY(hipMemcpyAsync(dthr.p, thr.data(), thr.size() * sizeof(float), hipMemcpyHostToDevice,
+                         st));
+  hipLaunchKernelGGL(bp::expand_threshold_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256),
+                     0, st, dthr.as<float>(), (int)nw, n, W, shift, threshold);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(st));  // thr and dthr are released on return
+  return BP_OK;
+}
+
+int bp_detect_peaks(const float *beam_max, const float *threshold, size_t n_samples,
+                    size_t distance, int *peaks, size_t capacity, size_t *n_peaks,
+                    void *stream) {
+  if (!beam_max || !n_peaks || (capacity && !peaks)) return fail(BP_EINVAL, "NULL argument");
+  *n_peaks = 0;
+  if (!size_ok(n_samples) || distance < 1)
+    return fail(BP_EINVAL, "bad n_samples (%zu) or distance (%zu)", n_samples, distance);
+  const long long n = (long long)n_samples;
+  if (n < 3) return BP_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  DevBuf flags;
+  BP_TRY(flags.alloc((size_t)n));
+  HIP_TRY(hipMemsetAsync(flags.p, 0, (size_t)n, st));
+  hipLaunchKernelGGL(bp::local_max_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                     beam_max, n, flags.as<unsigned char>());
+  HIP_TRY(hipGetLastError());
+  std::vector<unsigned char> f((size_t)n);
+  std::vector<float> x((size_t)n), thr;
+  HIP_TRY(hipMemcpyAsync(f.data(), flags.p, (size_t)n, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(x.data(), beam_max, (size_t)n * sizeof(float), hipMemcpyDeviceToHost,
+                         st));
+  if (threshold) {
+    thr.resize((size_t)n);
+    HIP_TRY(hipMemcpyAsync(thr.data(), threshold, (size_t)n * sizeof(float),
+                           hipMemcpyDeviceToHost, st));
+  }
+  HIP_TRY(hipStreamSynchronize(st));
+  std::vector<long long> cand;
+  for (long long i = 0; i < n; ++i)
+    if (f[(size_t)i]) cand.push_back(i);
+  // scipy.signal.find_peaks(distance=...): visit peaks by decreasing height
+  // (equal heights: later index first); a visited, kept peak removes every
+  // peak closer than `distance`
+  std::vector<size_t> order(cand.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+    const float ha = x[(size_t)cand[a]], hb = x[(size_t)cand[b]];
+    return ha != hb ? ha > hb : cand[a] > cand[b];
+  });
+  std::vector<unsigned char> keep(cand.size(), 1);
+  const long long dist = (long long)distance;
+  for (size_t o : order) {
+    if (!keep[o]) continue;
+    for (long long k = (long long)o - 1; k >= 0 && cand[o] - cand[(size_t)k] < dist; --k)
+      keep[(size_t)k] = 0;
+    for (size_t k = o + 1; k < cand.size() && cand[k] - cand[o] < dist; ++k) keep[k] = 0;
+  }
+  size_t m = 0;
+  for (size_t i = 0; i < cand.size(); ++i) {
+    if (!keep[i]) continue;
+    const size_t t = (size_t)cand[i];
+    if (threshold && !(x[t] > thr[t])) continue;
+    if (m < capacity) peaks[m] = (int)t;
+    ++m;
+  }
+  *n_peaks = m;
+  return BP_OK;
+}
+
 int bp_interleaved_to_planar(const float *F_stp, size_t n_stations, size_t n_samples,
                              size_t n_phases, float *F_spt, void *stream) {
   if (!F_stp || !F_spt) return fail(BP_EINVAL, "NULL argument");

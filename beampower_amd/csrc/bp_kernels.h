@@ -238,6 +238,89 @@ __global__ void pack_features_kernel(const float *__restrict__ F, int P, long lo
 }
 
 // ---------------------------------------------------------------------------
+// Detection consumer (notebooks/3_beampower.ipynb:497-626): time-dependent
+// threshold and local maxima of beam_max.
+// ---------------------------------------------------------------------------
+
+// Mean and population standard deviation of x over the window
+// [j*shift, j*shift + window) of block j, exact zeros excluded, two passes
+// accumulated in double with a fixed reduction tree (deterministic).
+__global__ __launch_bounds__(256) void window_stats_kernel(const float *__restrict__ x,
+                                                           long long window, long long shift,
+                                                           double *__restrict__ stats) {
+  __shared__ double s_sum[256], s_cnt[256];
+  const float *w = x + (long long)blockIdx.x * shift;
+  const int tid = threadIdx.x;
+  double s = 0.0, c = 0.0;
+  for (long long i = tid; i < window; i += 256) {
+    const float v = w[i];
+    if (v != 0.0f) {
+      s += v;
+      c += 1.0;
+    }
+  }
+  s_sum[tid] = s;
+  s_cnt[tid] = c;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (tid < o) {
+      s_sum[tid] += s_sum[tid + o];
+      s_cnt[tid] += s_cnt[tid + o];
+    }
+    __syncthreads();
+  }
+  const double n = s_cnt[0];
+  const double mean = n > 0.0 ? s_sum[0] / n : 0.0;
+  __syncthreads();
+  double q = 0.0;
+  for (long long i = tid; i < window; i += 256) {
+    const float v = w[i];
+    if (v != 0.0f) q += ((double)v - mean) * ((double)v - mean);
+  }
+  s_sum[tid] = q;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (tid < o) s_sum[tid] += s_sum[tid + o];
+    __syncthreads();
+  }
+  if (tid == 0) {
+    stats[2 * blockIdx.x] = mean;
+    stats[2 * blockIdx.x + 1] = n > 0.0 ? sqrt(s_sum[0] / n) : 0.0;
+  }
+}
+
+// Per-window thresholds -> every sample: sample t takes window t / shift
+// (clamped to the last), samples before window/2 take the value of sample
+// window/2 and samples from n - (window - window/2) on the value of the
+// sample before.
+__global__ void expand_threshold_kernel(const float *__restrict__ thr, int nw, long long n,
+                                        long long window, long long shift,
+                                        float *__restrict__ out) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const long long half = window / 2, end = n - (window - half);
+  const long long tt = t < half ? half : (t >= end ? end - 1 : t);
+  long long j = tt / shift;
+  if (j >= nw) j = nw - 1;
+  out[t] = thr[j];
+}
+
+// Local maxima of x, plateaus at their midpoint (the left one of two):
+// the thread at the left edge of a rise scans the plateau and flags its
+// midpoint when the series falls after it.  The first and last samples are
+// never maxima.
+__global__ void local_max_kernel(const float *__restrict__ x, long long n,
+                                 unsigned char *__restrict__ flag) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 1 || i >= n - 1) return;
+  const float xi = x[i];
+  if (!(x[i - 1] < xi)) return;
+  long long j = i + 1;
+  while (j < n - 1 && x[j] == xi) ++j;
+  if (x[j] < xi) flag[(i + j - 1) / 2] = 1;
+}
+
+// ---------------------------------------------------------------------------
 // Layout and prestack.
 // ---------------------------------------------------------------------------
 

@@ -272,6 +272,28 @@ int bp_server_stats(bp_server *server, bp_server_counters *stats);
 int bp_server_destroy(bp_server *server);
 const char *bp_server_last_error(void);
 
+/* Detection consumer (no counterpart in the library; the tutorial's step
+ * after the beamforming, notebooks/3_beampower.ipynb:497-626).
+ * bp_detect_threshold: time-dependent threshold of beam_max (DEVICE,
+ *   n_samples) into threshold (DEVICE, n_samples): per window of `window`
+ *   samples starting every int((1 - overlap) * window) samples, the mean
+ *   plus num_dev population standard deviations (the tutorial's "rms"
+ *   flavour; exact zeros are left out of the statistics where the tutorial
+ *   fills them with unseeded noise), the maximum with each neighbouring
+ *   window, each sample taking its window's value.  Needs n_samples > window.
+ *   Synchronous.
+ * bp_detect_peaks: local maxima of beam_max (plateaus at their midpoint) at
+ *   least `distance` samples apart, higher peaks first (equal heights: the
+ *   later one), as scipy.signal.find_peaks(distance=...); with threshold
+ *   (DEVICE, or NULL) only those strictly above it.  Writes up to `capacity`
+ *   ascending sample indices to peaks (HOST) and the total count to
+ *   n_peaks.  Synchronous. */
+int bp_detect_threshold(const float *beam_max, size_t n_samples, size_t window, double overlap,
+                        double num_dev, float *threshold, void *stream);
+int bp_detect_peaks(const float *beam_max, const float *threshold, size_t n_samples,
+                    size_t distance, int *peaks, size_t capacity, size_t *n_peaks,
+                    void *stream);
+
 /* Kernel timing: when enabled, the dominant (stack) kernel of every
  * bp_network_* call is bracketed with HIP events on its own stream and the
  * elapsed milliseconds are accumulated per thread. */

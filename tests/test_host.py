@@ -93,6 +93,9 @@ def test_c_abi_rejects_bad_arguments_without_a_device():
     assert lib.bp_batch_launch(None, None) == -22
     assert lib.bp_batch_tune(None, 1, None, 0, None, None, None) == -22
     assert lib.bp_batch_destroy(None) == 0
+    assert lib.bp_detect_threshold(None, 10, 5, 0.66, 8.0, None, None) == -22
+    n = ct.c_size_t(7)
+    assert lib.bp_detect_peaks(None, None, 10, 1, None, 0, ct.byref(n), None) == -22
 
 
 def test_degenerate_shapes_follow_reference_semantics():
