@@ -9,6 +9,9 @@ behaviour -- with device="gpu" served by the gfx950 library
 * prestack runs on the GPU for device="gpu" (the reference always prestacks
   on the CPU, beampower.py:103-105); the FMA sequence is the same, so the
   prestacked traces are bit-identical to the pip build's;
+* mode="differential" runs on the GPU (the reference prints "not yet
+  implemented" there, beampower.py:220-223) with the CPU branch's outputs
+  (beampower.py:194-216);
 * prestack_traces(device="gpu") returns the prestacked traces (the
   reference returns zeros there: its only prestack call is under
   `if device == "cpu"`, beampower.py:271-282);
@@ -54,7 +57,8 @@ def beamform(
     device: 'cpu' or 'gpu'
     reduce: 'max' returns (max over sources, argmax) per sample; 'none',
         'None' or None returns the full (n_sources, n_samples) beams.
-    mode: 'direct' (or 'normal') or 'differential' (CPU library only).
+    mode: 'direct' (or 'normal') or 'differential' (time delays relative to
+        the centre sample (n_samples - 1) // 2; one beam per source).
     out_of_bounds: 'strict' -- a beam is computed only where every used
         moveout points inside the data; 'flexible' -- out-of-range terms are
         dropped.
@@ -63,7 +67,9 @@ def beamform(
     Returns
     -------
     beam: (n_sources, n_samples) float32 for reduce='none', or
-    (beam_max (n_samples,) float32, beam_argmax (n_samples,) int32).
+    (beam_max (n_samples,) float32, beam_argmax (n_samples,) int32);
+    mode='differential': (n_sources,) float32 for reduce='none', or
+    (max over sources, its first index) for reduce='max'.
     """
     if out_of_bounds not in ["strict", "flexible"]:
         print("out_of_bounds should be either of 'strict' or 'flexible',"
@@ -80,7 +86,7 @@ def beamform(
     n_stations, _, n_samples = waveform_features.shape
     n_sources, _, n_phases = time_delays.shape
 
-    if mode in ["normal", "direct"] or (mode == "differential" and not on_gpu):
+    if mode in ["normal", "direct", "differential"]:
         F = prestack_traces(waveform_features, weights_phases, num_threads=num_threads,
                             device=device)
         F = F.flatten().astype(np.float32)
@@ -109,11 +115,10 @@ def beamform(
         return None
 
     if mode == "differential":
-        if on_gpu:
-            print("differential mode not yet implemented on GPU")
-            return
         beam = np.zeros(n_sources, dtype=np.float32)
-        lib.beamform_differential(*args, int(num_threads), beam.ctypes.data_as(_P_F))
+        status = lib.beamform_differential(*args, int(num_threads), beam.ctypes.data_as(_P_F))
+        if on_gpu:
+            check(status, "beamform_differential", lib)
         if reduce in ["none", "None", None]:
             return beam
         elif reduce == "max":

@@ -40,14 +40,17 @@ LIBRARIES = {
         # core.py:65-87 of the reference
         "beamform_argtypes": [_P_F, _P_I, _P_F, _SZ, _SZ, _SZ, _SZ, ct.c_int, _P_F],
         "beamform_max_argtypes": [_P_F, _P_I, _P_F, _SZ, _SZ, _SZ, _SZ, ct.c_int, _P_F, _P_I],
-        # the CPU library's prestack signature (core.py:40-49), on the GPU
+        # the CPU library's prestack and differential signatures
+        # (core.py:26-49), on the GPU
         "prestack_waveform_features_argtypes": [_P_F, _P_F, _SZ, _SZ, _SZ, _SZ, ct.c_int, _P_F],
+        "beamform_differential_argtypes": [_P_F, _P_I, _P_F, _SZ, _SZ, _SZ, _SZ, ct.c_int,
+                                           _P_F],
     },
 }
 
 # Symbols the gpu library declares in include/beamform_gpu.h.
 GPU_SYMBOLS = (
-    "beamform", "beamform_max", "prestack_waveform_features",
+    "beamform", "beamform_max", "prestack_waveform_features", "beamform_differential",
     "bp_last_error", "bp_version",
     "bp_network_create", "bp_network_destroy", "bp_network_info", "bp_network_classes",
     "bp_network_beamform_max", "bp_network_beamform",
@@ -55,6 +58,7 @@ GPU_SYMBOLS = (
     "bp_parity_stride", "bp_quantize_planar",
     "bp_profiling", "bp_profiling_read",
     "bp_network_beamform_max_view", "bp_network_beamform_view", "bp_network_padding",
+    "bp_network_beamform_differential",
     "bp_pack_features",
     "bp_batch_create", "bp_batch_launch", "bp_batch_graph_launch", "bp_batch_tune",
     "bp_batch_get_config", "bp_batch_set_config", "bp_batch_destroy",
@@ -123,6 +127,7 @@ def _declare_gpu_extension(lib):
     lib.bp_network_padding.argtypes = [vp, ct.POINTER(_SZ), ct.POINTER(_SZ)]
     lib.bp_network_beamform_view.argtypes = [vp, ct.POINTER(FeatureViewC), ct.c_int, _SZ, _SZ,
                                              vp, vp]
+    lib.bp_network_beamform_differential.argtypes = [vp, ct.POINTER(FeatureViewC), vp, vp]
     lib.bp_pack_features.argtypes = [vp, _SZ, _SZ, _SZ, ct.c_int, _SZ, _SZ, vp, vp, vp]
     lib.bp_batch_create.argtypes = [ct.c_int, vp, ct.POINTER(FeatureViewC), vp, vp, vp, vp,
                                     ct.POINTER(vp)]
@@ -170,10 +175,10 @@ def load_library(device="cpu"):
         lib.beamform.argtypes = info["beamform_argtypes"]
         lib.beamform_max.argtypes = info["beamform_max_argtypes"]
         lib.prestack_waveform_features.argtypes = info["prestack_waveform_features_argtypes"]
-        if device_name == "cpu":
-            lib.beamform_differential.argtypes = info["beamform_differential_argtypes"]
-        else:
-            for name in ("beamform", "beamform_max", "prestack_waveform_features"):
+        lib.beamform_differential.argtypes = info["beamform_differential_argtypes"]
+        if device_name == "gpu":
+            for name in ("beamform", "beamform_max", "prestack_waveform_features",
+                         "beamform_differential"):
                 getattr(lib, name).restype = ct.c_int
             _declare_gpu_extension(lib)
         info["is_loaded"] = True

@@ -205,6 +205,8 @@ struct bp_network {
   std::vector<int> tiles[3];  // ascending tile ids per class (host copy)
   DevBuf tau, beta, lohi, tile_lo, tile_all, tile_lw, dtiles[3];
   DevBuf sched[3], scale[3];
+  DevBuf diff_tau, diff_beta;  // differential mode: wave-transposed tables, built on first use
+  bool diff_ok = false;
   bool sched_ok[3] = {false, false, false};
   std::vector<float> sched_scale[3];
   std::mutex mu;  // schedule (re)builds
@@ -700,6 +702,78 @@ int batch_configure(bp_batch *b, const bp_batch_config &c) {
   return BP_OK;
 }
 
+// Differential mode (beamform.c:338-394), one thread per source.
+// (K, S, P) delays and (K, S) weights on the device -> the wave-transposed
+// tables of the differential kernel (64-source blocks, entry-major).
+int build_diff_tables(const int *tau, const float *beta, long long K, int S, int P,
+                      DevBuf &tauT, DevBuf &betaT, hipStream_t st) {
+  const long long nblk = (K + bp::kDiffBlock - 1) / bp::kDiffBlock;
+  const long long sp = (long long)S * P;
+  BP_TRY(tauT.alloc((size_t)(nblk * sp * bp::kDiffBlock) * sizeof(int)));
+  BP_TRY(betaT.alloc((size_t)(nblk * S * bp::kDiffBlock) * sizeof(float)));
+  auto tiles = [](long long e) {
+    return (unsigned)std::min<long long>((e + bp::kDiffBlock - 1) / bp::kDiffBlock, 1024);
+  };
+  hipLaunchKernelGGL(bp::diff_tables_kernel, dim3((unsigned)nblk, tiles(sp)), dim3(256), 0, st,
+                     reinterpret_cast<const unsigned *>(tau), K, (int)sp, tauT.as<unsigned>());
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(bp::diff_tables_kernel, dim3((unsigned)nblk, tiles(S)), dim3(256), 0, st,
+                     reinterpret_cast<const unsigned *>(beta), K, S, betaT.as<unsigned>());
+  HIP_TRY(hipGetLastError());
+  return BP_OK;
+}
+
+template <int FMT>
+void launch_differential_p(const bp::StackArgs &a, const int *tauT, const float *betaT,
+                           float *beam, hipStream_t st) {
+  const dim3 grid((unsigned)((a.K + bp::kDiffSources - 1) / bp::kDiffSources));
+  const dim3 block(bp::kDiffSources);
+  switch (a.P) {
+    case 1:
+      hipLaunchKernelGGL((bp::differential_kernel<FMT, 1>), grid, block, 0, st, a, tauT, betaT,
+                         beam);
+      break;
+    case 2:
+      hipLaunchKernelGGL((bp::differential_kernel<FMT, 2>), grid, block, 0, st, a, tauT, betaT,
+                         beam);
+      break;
+    case 3:
+      hipLaunchKernelGGL((bp::differential_kernel<FMT, 3>), grid, block, 0, st, a, tauT, betaT,
+                         beam);
+      break;
+    case 4:
+      hipLaunchKernelGGL((bp::differential_kernel<FMT, 4>), grid, block, 0, st, a, tauT, betaT,
+                         beam);
+      break;
+    default:
+      hipLaunchKernelGGL((bp::differential_kernel<FMT, 0>), grid, block, 0, st, a, tauT, betaT,
+                         beam);
+  }
+}
+
+// Size limits of the kernel's 32-bit offsets.
+int check_differential(int fmt, long long S, long long P, long long ldf) {
+  const long long sp = S * P, rows = sp * (fmt == BP_F32 ? 1 : 2);
+  if (sp * bp::kDiffBlock >= (1ll << 30))
+    return fail(BP_EINVAL, "n_stations * n_phases = %lld is too large for differential mode", sp);
+  if (rows * ldf >= (1ll << 30))
+    return fail(BP_EINVAL, "differential mode takes features under 2^30 elements (%lld rows of "
+                "%lld)", rows, ldf);
+  return BP_OK;
+}
+
+int launch_differential(int fmt, const bp::StackArgs &a, const int *tauT, const float *betaT,
+                        float *beam, hipStream_t st) {
+  if (fmt == BP_F32)
+    launch_differential_p<BP_F32>(a, tauT, betaT, beam, st);
+  else if (fmt == BP_F16)
+    launch_differential_p<BP_F16>(a, tauT, betaT, beam, st);
+  else
+    launch_differential_p<BP_BF16>(a, tauT, betaT, beam, st);
+  HIP_TRY(hipGetLastError());
+  return BP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -938,6 +1012,85 @@ int bp_network_beamform_view(bp_network *net, const bp_feature_view *features,
   if (fmt == BP_F16)
     return launch_stack<BP_F16, true>(net, a, (int)source_begin, (int)source_end, st);
   return launch_stack<BP_BF16, true>(net, a, (int)source_begin, (int)source_end, st);
+}
+
+// ---- differential mode (beamform.c:338-394) -------------------------------
+
+int bp_network_beamform_differential(bp_network *net, const bp_feature_view *features,
+                                     float *beam, void *stream) {
+  if (!net || !features || !beam) return fail(BP_EINVAL, "NULL argument");
+  BP_TRY(check_view(net, *features));
+  if (features->n_samples == 0 || net->K == 0) return BP_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int fmt = features->fmt;
+  BP_TRY(check_differential(fmt, net->S, net->P, (long long)features->row_stride));
+  if (fmt != BP_F32) BP_TRY(ensure_schedule(net, fmt, features->station_scale, st));
+  {
+    // the transposed tables, built on first use and kept with the network
+    std::lock_guard<std::mutex> lock(net->mu);
+    if (!net->diff_ok) {
+      BP_TRY(build_diff_tables(net->tau.as<int>(), net->beta.as<float>(), net->K, net->S,
+                               net->P, net->diff_tau, net->diff_beta, st));
+      HIP_TRY(hipStreamSynchronize(st));  // ready for calls on any stream
+      net->diff_ok = true;
+    }
+  }
+  const bp::StackArgs a = make_args_view(net, *features, 0);
+  return launch_differential(fmt, a, net->diff_tau.as<int>(), net->diff_beta.as<float>(), beam,
+                             st);
+}
+
+// The CPU library's entry point (beamform_cpu.h:6, beamform.c:338-394) on
+// the GPU, host pointers, synchronous; num_threads is accepted and unused.
+// Sources left out keep the caller's beam values (the reference wrapper
+// passes zeros, beampower.py:198).
+int beamform_differential(float *waveform_features, int *time_delays, float *weights,
+                          size_t n_samples, size_t n_sources, size_t n_stations,
+                          size_t n_phases, int num_threads, float *beam) {
+  (void)num_threads;
+  BP_TRY(check_shapes(n_samples, n_sources, n_stations, n_phases));
+  // no sample, no source, or no station/phase (no source uses a station)
+  if (n_samples == 0 || n_sources == 0 || n_stations == 0 || n_phases == 0) return BP_OK;
+  if (!beam) return fail(BP_EINVAL, "NULL output");
+  if (!waveform_features || !time_delays || !weights) return fail(BP_EINVAL, "NULL input");
+  BP_TRY(check_device());
+  Stream st;
+  BP_TRY(st.create());
+  const size_t T = n_samples, K = n_sources, S = n_stations, P = n_phases;
+  BP_TRY(check_differential(BP_F32, (long long)S, (long long)P, (long long)T));
+  DevBuf tau, beta, tauT, betaT, f_stp, f_pl, dbeam;
+  BP_TRY(tau.alloc(K * S * P * sizeof(int)));
+  BP_TRY(beta.alloc(K * S * sizeof(float)));
+  BP_TRY(f_stp.alloc(S * T * P * sizeof(float)));
+  BP_TRY(f_pl.alloc(S * T * P * sizeof(float)));
+  BP_TRY(dbeam.alloc(K * sizeof(float)));
+  HIP_TRY(hipMemcpyAsync(tau.p, time_delays, K * S * P * sizeof(int), hipMemcpyHostToDevice,
+                         st.s));
+  HIP_TRY(hipMemcpyAsync(beta.p, weights, K * S * sizeof(float), hipMemcpyHostToDevice, st.s));
+  HIP_TRY(hipMemcpyAsync(f_stp.p, waveform_features, S * T * P * sizeof(float),
+                         hipMemcpyHostToDevice, st.s));
+  HIP_TRY(hipMemcpyAsync(dbeam.p, beam, K * sizeof(float), hipMemcpyHostToDevice, st.s));
+  hipLaunchKernelGGL(bp::interleaved_to_planar_kernel,
+                     dim3((unsigned)((T + 255) / 256), (unsigned)S), dim3(256), 0, st.s,
+                     f_stp.as<float>(), (int)S, (int)T, (int)P, (long long)T, 0ll,
+                     f_pl.as<float>());
+  HIP_TRY(hipGetLastError());
+  bp::StackArgs a{};
+  a.F = f_pl.p;
+  a.ldf = (long long)T;
+  a.tau = tau.as<int>();
+  a.beta = beta.as<float>();
+  a.T = (int)T;
+  a.K = (int)K;
+  a.S = (int)S;
+  a.P = (int)P;
+  BP_TRY(build_diff_tables(tau.as<int>(), beta.as<float>(), (long long)K, (int)S, (int)P, tauT,
+                           betaT, st.s));
+  BP_TRY(launch_differential(BP_F32, a, tauT.as<int>(), betaT.as<float>(), dbeam.as<float>(),
+                             st.s));
+  HIP_TRY(hipMemcpyAsync(beam, dbeam.p, K * sizeof(float), hipMemcpyDeviceToHost, st.s));
+  HIP_TRY(hipStreamSynchronize(st.s));
+  return BP_OK;
 }
 
 // ---- detection consumer (notebooks/3_beampower.ipynb:497-626) ------------

@@ -1047,6 +1047,20 @@ __device__ __forceinline__ float feature(const StackArgs &a, int s, int p, long 
   return __uint_as_float((unsigned)h << 16);
 }
 
+// Weight of a used station as the kernels apply it: times the station's
+// 16-bit storage scale, and for BF16 rounded to bfloat16 like the features.
+template <int FMT>
+__device__ __forceinline__ float applied_weight(float b0, float scale) {
+  float b = b0 * scale;
+  if (FMT == BP_BF16) b = __uint_as_float(bf16_bits(b) << 16);
+  return b;
+}
+
+template <int FMT>
+__device__ __forceinline__ float station_weight(const StackArgs &a, float b0, int s) {
+  return applied_weight<FMT>(b0, a.scale ? a.scale[s] : 1.0f);
+}
+
 template <int FMT, bool NONE>
 __global__ __launch_bounds__(256) void stack_direct_kernel(StackArgs a, int chunk) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1064,8 +1078,7 @@ __global__ __launch_bounds__(256) void stack_direct_kernel(StackArgs a, int chun
     for (int s = 0; s < S; ++s) {
       const float b0 = a.beta[(size_t)k * S + s];
       if (b0 == 0.0f) continue;
-      float b = b0 * (a.scale ? a.scale[s] : 1.0f);
-      if (FMT == BP_BF16) b = __uint_as_float(bf16_bits(b) << 16);
+      const float b = station_weight<FMT>(a, b0, s);
       for (int p = 0; p < P; ++p) {
         const long long tg = t + a.tau[((size_t)k * S + s) * P + p];
         if (tg < 0 || tg >= T) continue;  // only reachable in flexible mode
@@ -1080,6 +1093,149 @@ __global__ __launch_bounds__(256) void stack_direct_kernel(StackArgs a, int chun
     }
   }
   if (!NONE && t < T && bv > -FLT_MAX) atomicMax(a.keys + t, pack_key(bv, bk));
+}
+
+// ---------------------------------------------------------------------------
+// Differential mode (beamform.c:338-394): the time delays are relative to a
+// zero lag at the centre sample t = (T - 1) / 2 of a short window (the
+// features are inter-station cross-correlations), and each source gets one
+// beam, at t, if it uses a station and every used delay points inside
+// [0, T); other entries of `beam` are left as they are.  The FMA chain is
+// the one of every other beam.
+//
+// One thread per source, 64 sources per wave.  The tables are read in a
+// wave-transposed layout built once per network (diff_tables_kernel): for
+// the 64 sources of wave block b, delay entry j = s * P + p of all 64 is
+// one contiguous 256-byte line, tauT[(b * S * P + j) * 64 + lane], and so
+// is weight s, betaT[(b * S + s) * 64 + lane].  Every table load of a wave
+// is then one full line (row-major (K, S, P) rows would give each lane its
+// own line, and the load instructions, not the bytes, would set the pace).
+// ---------------------------------------------------------------------------
+constexpr int kDiffSources = 256;  // per work-group: 4 independent waves
+constexpr int kDiffBlock = 64;     // sources per wave block of the transposed tables
+
+// (K, E) row-major 4-byte table -> (ceil(K / 64), E, 64) wave-transposed,
+// zero past the last source.  Grid (ceil(K / 64), any), 256 threads, the
+// y blocks striding over 64-entry tiles; a 64 x 64 tile goes through LDS so
+// both sides are coalesced.
+__global__ __launch_bounds__(256) void diff_tables_kernel(const unsigned *__restrict__ in,
+                                                          long long K, int E,
+                                                          unsigned *__restrict__ out) {
+  __shared__ unsigned tile[kDiffBlock][kDiffBlock + 1];
+  const long long b = blockIdx.x;
+  for (int e0 = blockIdx.y * kDiffBlock; e0 < E; e0 += gridDim.y * kDiffBlock) {
+    __syncthreads();  // the previous tile is written out
+    for (int i = threadIdx.x; i < kDiffBlock * kDiffBlock; i += blockDim.x) {
+      const int r = i / kDiffBlock, c = i % kDiffBlock;
+      const long long k = b * kDiffBlock + r;
+      tile[r][c] = (k < K && e0 + c < E) ? in[k * E + e0 + c] : 0u;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kDiffBlock * kDiffBlock; i += blockDim.x) {
+      const int c = i / kDiffBlock, r = i % kDiffBlock;
+      if (e0 + c < E) out[(b * E + e0 + c) * kDiffBlock + r] = tile[r][c];
+    }
+  }
+}
+
+// Feature element t of planar row `row` (= s * P + p): float32 rows, or
+// plane 0 of the 16-bit parity rows.  The element offset is 32-bit (the
+// host keeps the view under 2^30 elements).
+template <int FMT>
+__device__ __forceinline__ float feature_row(const StackArgs &a, unsigned row, unsigned t) {
+  if (FMT == BP_F32) return static_cast<const float *>(a.F)[row * (unsigned)a.ldf + t];
+  const unsigned short h =
+      static_cast<const unsigned short *>(a.F)[row * 2u * (unsigned)a.ldf + t];
+  if (FMT == BP_F16) return (float)__builtin_bit_cast(_Float16, h);
+  return __uint_as_float((unsigned)h << 16);
+}
+
+// PT: the phase count when it is 1..4 (a chunk of CS = 16 / PT stations is
+// loaded at once), 0 for any other count (station by station).  The
+// per-source bounds of the reference (beamform.c:10-47) are taken in the
+// same pass: a used delay outside [0, T) adds no term and marks the source
+// out of range, and the beam is written only for a source that uses a
+// station and never went out of range -- for it no term was left out, so
+// the chain is the reference's.  The bounds are private to each source
+// (the reference shares them between OpenMP threads,
+// beamform.c:353-354,370-380; this is its one-thread behaviour).
+template <int FMT, int PT>
+__global__ __launch_bounds__(kDiffSources) void differential_kernel(
+    StackArgs a, const int *__restrict__ tauT, const float *__restrict__ betaT,
+    float *__restrict__ beam) {
+  constexpr int CS = PT > 0 ? 16 / PT : 1;  // stations per chunk
+  constexpr int NE = PT > 0 ? CS * PT : 1;
+  const int lane = threadIdx.x % kDiffBlock;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kDiffBlock);
+  const long long blk = (long long)blockIdx.x * (kDiffSources / kDiffBlock) + wv;
+  const long long k0 = blk * kDiffBlock;
+  if (k0 >= a.K) return;  // whole wave past the last source
+  const int S = a.S, P = PT > 0 ? PT : a.P;
+  const long long T = a.T, t = (T - 1) / 2;
+  // 32-bit offsets inside the wave's block: 64 * S * P < 2^30 (checked by the host)
+  const int *tb = tauT + blk * S * P * kDiffBlock + lane;
+  const float *wb = betaT + blk * S * kDiffBlock + lane;
+  float acc = 0.0f;
+  bool used = false, inside = true;
+  for (int s0 = 0; s0 < S; s0 += CS) {
+    int d[NE];
+    float b[CS], x[NE];
+    // the chunk's weights and delays (stations past S read station S - 1 and count as unused)
+#pragma unroll
+    for (int c = 0; c < CS; ++c) {
+      const int s = min(s0 + c, S - 1);
+      b[c] = wb[(unsigned)s * kDiffBlock];
+      if (PT > 0) {
+#pragma unroll
+        for (int p = 0; p < PT; ++p) d[c * PT + p] = tb[(unsigned)(s * PT + p) * kDiffBlock];
+      }
+    }
+    if (PT > 0) {
+      // then the gathers; entries not used or out of range read element 0 of row 0
+#pragma unroll
+      for (int c = 0; c < CS; ++c) {
+#pragma unroll
+        for (int p = 0; p < PT; ++p) {
+          const long long tg = t + d[c * PT + p];
+          const bool live = s0 + c < S && b[c] != 0.0f && tg >= 0 && tg < T;
+          x[c * PT + p] = feature_row<FMT>(a, live ? (unsigned)((s0 + c) * PT + p) : 0u,
+                                           live ? (unsigned)tg : 0u);
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < CS; ++c) {
+        const bool u = s0 + c < S && b[c] != 0.0f;
+        const float sc = (FMT == BP_F32 || !a.scale) ? 1.0f : a.scale[min(s0 + c, S - 1)];
+        const float wt = applied_weight<FMT>(b[c], sc);
+        used |= u;
+#pragma unroll
+        for (int p = 0; p < PT; ++p) {
+          const long long tg = t + d[c * PT + p];
+          const bool in = tg >= 0 && tg < T;
+          inside &= !u || in;
+          const float y = fmaf(wt, x[c * PT + p], acc);
+          acc = (u && in) ? y : acc;
+        }
+      }
+    } else {
+      // any phase count: one station at a time (CS == 1)
+      const bool u = b[0] != 0.0f;
+      const float sc = (FMT == BP_F32 || !a.scale) ? 1.0f : a.scale[s0];
+      const float wt = applied_weight<FMT>(b[0], sc);
+      used |= u;
+      for (int p = 0; p < P; ++p) {
+        const long long tg = t + tb[(unsigned)(s0 * P + p) * kDiffBlock];
+        const bool in = tg >= 0 && tg < T;
+        const bool live = u && in;
+        const float xv = feature_row<FMT>(a, live ? (unsigned)(s0 * P + p) : 0u,
+                                          live ? (unsigned)tg : 0u);
+        inside &= !u || in;
+        const float y = fmaf(wt, xv, acc);
+        acc = live ? y : acc;
+      }
+    }
+  }
+  if (k0 + lane < a.K && used && inside) beam[k0 + lane] = acc;
 }
 
 }  // namespace bp

@@ -149,6 +149,22 @@ class Network:
             "bp_network_beamform", self.lib)
         return beam
 
+    def beamform_differential(self, features, beam=None, stream=None):
+        """mode='differential' (beamform.c:338-394): one beam per source at
+        the centre sample (T - 1) // 2, as a (n_sources,) float32 CUDA
+        tensor; sources with a used delay outside [0, T) keep `beam`'s value
+        (zeros by default).  features: planar (S,P,T) float32 tensor,
+        QuantizedFeatures or FeatureView."""
+        torch = _torch()
+        v = FeatureView.of(features)
+        if beam is None:
+            beam = torch.zeros(self.n_sources, dtype=torch.float32, device=v.data.device)
+        cv = v.c_view()
+        check(self.lib.bp_network_beamform_differential(self._h, ct.byref(cv), beam.data_ptr(),
+                                                        _stream_ptr(stream)),
+              "bp_network_beamform_differential", self.lib)
+        return beam
+
 
 class QuantizedFeatures:
     """16-bit feature storage: parity planes (S, P, 2, ldq) as uint16 on the

@@ -169,6 +169,54 @@ def serving_requests(networks, n, seed, seconds=(30.0, 600.0), pool_per_network=
     return reqs
 
 
+def differential_workload(name, seed, n_stations, box_km, grid_shape, margin=10, n_events=1,
+                          depth_km=(0.0, 20.0)):
+    """mode='differential': the "stations" are the n_stations*(n_stations-1)/2
+    station pairs (i < j) and the features their cross-correlation
+    envelopes, one channel, zero lag at the centre sample (T - 1) / 2.
+    time_delays[k, (i, j), p] = tau[k, j, p] - tau[k, i, p] for absolute
+    P and S moveouts tau; T = 2 * max|delay| + 1 + 2 * margin.  Features are
+    smoothed |N(0,1)| with, for each of n_events true sources, a peak of 5
+    at the P and S differential times of every pair; phase weights 1; source
+    weights 1."""
+    stations_km, sources_km, _ = make_network(seed, n_stations, box_km, grid_shape, depth_km)
+    tt = build_time_delays(sources_km, stations_km, (VP_KM_S, VS_KM_S), SAMPLING_RATE_HZ,
+                           relative=False)
+    i, j = np.triu_indices(n_stations, k=1)
+    diff = (tt[:, j, :] - tt[:, i, :]).astype(np.int32)
+    half = int(np.abs(diff).max()) + margin
+    T = 2 * half + 1
+    rng = np.random.default_rng(seed + 1)
+    n_pairs = len(i)
+    feats = boxcar_envelopes(rng, n_pairs, 1, T, width=3)
+    events = []
+    for _ in range(n_events):
+        k = int(rng.integers(len(sources_km)))
+        for q in range(n_pairs):
+            for ph in range(2):
+                feats[q, 0, half + diff[k, q, ph]] += 5.0
+        events.append((k, half))
+    return Workload(
+        name=name, waveform_features=feats, time_delays=diff,
+        weights_phases=np.ones((n_pairs, 1, 2), np.float32),
+        weights_sources=np.ones((len(sources_km), n_pairs), np.float32), events=events,
+        meta=dict(seed=seed, box_km=box_km, grid_shape=tuple(grid_shape), n_stations=n_stations,
+                  sampling_rate=SAMPLING_RATE_HZ, margin=margin))
+
+
+def differential_c5(seed=7):
+    """Differential mode, CPU-runnable: 20 stations (190 pairs) in 50x50 km,
+    20x20x10 grid (4,000 sources)."""
+    return differential_workload("differential-20sta-190pairs-4k", seed, 20, 50.0, (20, 20, 10))
+
+
+def differential_c6(seed=8):
+    """Differential mode at one-GPU shape: 50 stations (1,225 pairs) in
+    100x100 km, 80x80x30 grid (192,000 sources)."""
+    return differential_workload("differential-50sta-1225pairs-192k", seed, 50, 100.0,
+                                 (80, 80, 30))
+
+
 def subset_sources(n_sources, n_subset, seed):
     """Seeded sorted subset of source indices (the reduce='none' case)."""
     rng = np.random.default_rng(seed)

@@ -60,6 +60,21 @@ int prestack_waveform_features(float *waveform_features, float *weights_phases,
                                size_t n_channels, size_t n_phases,
                                int num_threads, float *prestacked);
 
+/* Replaces the CPU library's beamform_differential (beamform_cpu.h:6,
+ * beamform.c:338-394; argtypes core.py:26-35), which the reference's GPU
+ * path lacks (beampower.py:220-223).  The time delays are relative to a
+ * zero lag at the centre sample t = (n_samples - 1) / 2.
+ * waveform_features: (n_stations, n_samples, n_phases) float32 prestacked
+ * time_delays: (n_sources, n_stations, n_phases) int32
+ * weights: (n_sources, n_stations) float32
+ * beam: (n_sources) float32; beam[k] = the beam of source k at t when all
+ *       of its used delays point inside [0, n_samples), else left as passed
+ *       (the wrapper passes zeros, beampower.py:198).
+ * num_threads is accepted for signature compatibility and ignored. */
+int beamform_differential(float *waveform_features, int *time_delays, float *weights,
+                          size_t n_samples, size_t n_sources, size_t n_stations,
+                          size_t n_phases, int num_threads, float *beam);
+
 /* ---- extension API (no counterpart in the reference) -------------------- */
 
 /* Thread-local message for the last non-zero status of this thread. */
@@ -139,6 +154,13 @@ int bp_network_beamform_max_view(bp_network *net, const bp_feature_view *feature
 int bp_network_beamform_view(bp_network *net, const bp_feature_view *features,
                              int out_of_bounds, size_t source_begin, size_t source_end,
                              float *beam, void *stream);
+
+/* Differential mode on a resident network (same rule as
+ * beamform_differential): beam DEVICE (n_sources) float32, entries of
+ * sources out of range at the centre sample of the view are left
+ * untouched.  Any storage format.  Asynchronous on `stream`. */
+int bp_network_beamform_differential(bp_network *net, const bp_feature_view *features,
+                                     float *beam, void *stream);
 
 /* Zero padding (samples) a feature view of this network needs before and
  * after every row so that no LDS tile takes the guarded edge path. */

@@ -16,6 +16,7 @@
  *   oracle_prestack       <- prestack_waveform_features beamform.c:108-152
  *   oracle_beamform       <- beamform (reduce='none')   beamform.c:154-228
  *   oracle_beamform_max   <- beamform_max (reduce='max') beamform.c:230-336
+ *   oracle_beamform_differential <- beamform_differential beamform.c:338-394
  *
  * Numerics.  The per-(source, time) beam is accumulated in float from
  * 0.0f, stations ascending, phases ascending, exactly as beamform.c:59-70.
@@ -222,6 +223,32 @@ void oracle_beamform_max(const float *F, const int *tau, const float *beta,
         if (best > -FLT_MAX)
             beam_max[t] = best;
         beam_argmax[t] = best_k;
+    }
+}
+
+/* beamform.c:338-394 (mode='differential'): one beam per source at the
+ * centre sample t = (T - 1) / 2, for sources whose used delays all point
+ * inside [0, T); other entries of `beam` are left untouched.  The
+ * reference shares its per-source bounds between OpenMP threads
+ * (time_delay_min/max are declared outside the parallel loop and not
+ * private), a data race with more than one thread; here they are private,
+ * which is the reference's single-thread behaviour. */
+void oracle_beamform_differential(const float *F, const int *tau, const float *beta,
+                                  size_t n_samples, size_t n_sources, size_t n_stations,
+                                  size_t n_phases, int num_threads, float *beam,
+                                  int *lohi_scratch)
+{
+    int nt = nthreads_or_default(num_threads);
+    if (n_samples == 0)
+        return;
+    long long t = ((long long)n_samples - 1) / 2;
+    oracle_delay_bounds(tau, beta, n_sources, n_stations, n_phases, nt, lohi_scratch);
+#pragma omp parallel for num_threads(nt) schedule(static)
+    for (size_t k = 0; k < n_sources; k++) {
+        if (!strict_valid(t, lohi_scratch[2 * k], lohi_scratch[2 * k + 1], n_samples))
+            continue;
+        beam[k] = stack_strict(F, tau + k * n_stations * n_phases, beta + k * n_stations,
+                               t, n_samples, n_stations, n_phases);
     }
 }
 

@@ -60,6 +60,8 @@ def _oracle(flavour):
                                         ct.c_int, ct.c_int, _P_F, _P_I]
         lib.oracle_beamform_max.argtypes = [_P_F, _P_I, _P_F, _SZ, _SZ, _SZ, _SZ,
                                             ct.c_int, ct.c_int, _P_F, _P_I, _P_I]
+        lib.oracle_beamform_differential.argtypes = [_P_F, _P_I, _P_F, _SZ, _SZ, _SZ, _SZ,
+                                                     ct.c_int, _P_F, _P_I]
         lib.oracle_prestack_f64.argtypes = [_P_D, _P_D, _SZ, _SZ, _SZ, _SZ, ct.c_int, _P_D]
         lib.oracle_beamform_f64.argtypes = [_P_D, _P_I, _P_D, _SZ, _SZ, _SZ, _SZ,
                                             ct.c_int, ct.c_int, ct.c_longlong,
@@ -68,7 +70,8 @@ def _oracle(flavour):
                                                 ct.c_int, ct.c_int, ct.c_longlong,
                                                 ct.c_longlong, _P_D, _P_D, _P_I, _P_U8, _P_I]
         for name in ("oracle_delay_bounds", "oracle_prestack", "oracle_beamform",
-                     "oracle_beamform_max", "oracle_prestack_f64", "oracle_beamform_f64",
+                     "oracle_beamform_max", "oracle_beamform_differential", "oracle_prestack_f64",
+                     "oracle_beamform_f64",
                      "oracle_beamform_max_f64"):
             getattr(lib, name).restype = None
         lib._typed = True
@@ -146,6 +149,24 @@ def beamform_max(F, tau, beta, out_of_bounds=0, num_threads=0, flavour="fma", in
                                          _ptr(bmax, _P_F), _ptr(barg, _P_I),
                                          _ptr(scratch, _P_I))
     return bmax, barg
+
+
+def beamform_differential(F, tau, beta, num_threads=0, flavour="fma", init=None):
+    """(K,) beams at the centre sample (T - 1) // 2 from prestacked (S,T,P)
+    features (beamform.c:338-394).  Sources with a used delay pointing
+    outside [0, T) keep ``init`` (default zeros, beampower.py:198)."""
+    F = _c(F, np.float32)
+    tau = _c(tau, np.int32)
+    beta = _c(beta, np.float32)
+    S, T, P = F.shape
+    K = tau.shape[0]
+    beam = np.zeros(K, np.float32) if init is None else _c(init, np.float32).copy()
+    scratch = np.empty(2 * K, np.int32)
+    _oracle(flavour).oracle_beamform_differential(_ptr(F, _P_F), _ptr(tau, _P_I),
+                                                  _ptr(beta, _P_F), T, K, S, P,
+                                                  _nt(num_threads), _ptr(beam, _P_F),
+                                                  _ptr(scratch, _P_I))
+    return beam
 
 
 def prestack_f64(u, alpha, num_threads=0):
@@ -235,7 +256,10 @@ def _ref(flavour):
                                      ct.c_int, _P_F, _P_I]
         lib.prestack_waveform_features.argtypes = [_P_F, _P_F, _SZ, _SZ, _SZ, _SZ,
                                                    ct.c_int, _P_F]
-        for name in ("beamform", "beamform_max", "prestack_waveform_features"):
+        lib.beamform_differential.argtypes = [_P_F, _P_I, _P_F, _SZ, _SZ, _SZ, _SZ,
+                                              ct.c_int, _P_F]
+        for name in ("beamform", "beamform_max", "prestack_waveform_features",
+                     "beamform_differential"):
             getattr(lib, name).restype = None
         lib._typed = True
     return lib
@@ -276,3 +300,17 @@ def ref_beamform_max(F, tau, beta, out_of_bounds=0, num_threads=1, flavour="cmak
                                int(out_of_bounds), _nt(num_threads), _ptr(bmax, _P_F),
                                _ptr(barg, _P_I))
     return bmax, barg
+
+
+def ref_beamform_differential(F, tau, beta, num_threads=1, flavour="cmake"):
+    """The reference's beamform_differential (beamform.c:338-394).  Keep
+    num_threads=1: its per-source bounds are shared between threads."""
+    F = _c(F, np.float32)
+    tau = _c(tau, np.int32)
+    beta = _c(beta, np.float32)
+    S, T, P = F.shape
+    K = tau.shape[0]
+    beam = np.zeros(K, np.float32)
+    _ref(flavour).beamform_differential(_ptr(F, _P_F), _ptr(tau, _P_I), _ptr(beta, _P_F), T, K,
+                                        S, P, _nt(num_threads), _ptr(beam, _P_F))
+    return beam

@@ -53,6 +53,36 @@ def run_case(u, alpha, tau, beta, reduces=("max", "none"), oobs=(0, 1)):
     return out
 
 
+def differential_case(seed, T, K=80, nonfinite=False):
+    """mode='differential': moveouts on both sides of zero lag, so that some
+    sources fall outside [0, T) at the centre sample on either side.  The
+    reference runs single-threaded: its per-source bounds are shared between
+    OpenMP threads (beamform.c:353-354,370-380)."""
+    half = T // 2 + 4
+    u, alpha, tau, beta = synthetic.random_case(seed, S=6, C=3, P=2, T=T, K=K,
+                                                tau_range=(-half, half))
+    if nonfinite:
+        # plant NaN / inf where valid sources read them
+        t = (T - 1) // 2
+        used = beta != 0
+        lo = np.where(used[:, :, None], tau, T).min(axis=(1, 2))
+        hi = np.where(used[:, :, None], tau, -T).max(axis=(1, 2))
+        valid = np.flatnonzero(used.any(1) & (t + lo >= 0) & (t + hi < T))
+        for j, v in zip(valid[:3], (np.nan, np.inf, -np.inf)):
+            s = int(np.flatnonzero(used[j])[0])
+            if j == valid[2]:
+                beta[j, s] = v
+            else:
+                u[s, 0, t + tau[j, s, 0]] = v
+    out = {}
+    for fl in FLAVOURS:
+        F = oracle.ref_prestack(u, alpha, num_threads=1, flavour=fl)
+        out[f"F_{fl}"] = F
+        out[f"diff_{fl}"] = oracle.ref_beamform_differential(F, tau, beta, num_threads=1,
+                                                             flavour=fl)
+    return dict(u=u, alpha=alpha, tau=tau, beta=beta), out
+
+
 def save(name, inputs, outputs, **meta):
     path = os.path.join(HERE, f"{name}.npz")
     np.savez_compressed(path, **inputs, **outputs,
@@ -60,9 +90,18 @@ def save(name, inputs, outputs, **meta):
     print(f"{path}: {os.path.getsize(path) / 1024:.1f} KiB")
 
 
-def main():
+def main(only=None):
     if not oracle.ref_available("cmake") or not oracle.ref_available("make"):
         sys.exit("oracle/_ref is not built: run `make -C oracle ref`")
+
+    # 8. mode='differential', odd and even T, non-finite values
+    for name, seed, T, nf in (("differential_odd", 17, 41, False),
+                              ("differential_even", 18, 64, False),
+                              ("differential_nonfinite", 19, 33, True)):
+        inputs, out = differential_case(seed, T, nonfinite=nf)
+        save(name, inputs, out, seed=seed)
+    if only == "differential":
+        return
 
     # 1. small random, strict + flexible, mixed weights (zeros, negatives,
     #    -0.0, a source with no used station)
@@ -134,4 +173,5 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    # `make_golden.py differential` writes only the mode='differential' cases
+    main(sys.argv[1] if len(sys.argv) > 1 else None)

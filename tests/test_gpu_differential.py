@@ -1,0 +1,150 @@
+"""mode='differential' on the GPU (beamform.c:338-394) against the reference's
+golden vectors, the CPU oracle and the direct beams.
+
+Bar: bit-identical beams to the reference's pip/CMake build (FMA-contracted)
+and to the oracle's "fma" flavour; sources out of range at the centre sample
+keep the caller's value.  At the one-GPU shape (1,225 station pairs,
+192,000 sources) the check is against the oracle in full and against the
+size-independent property that a differential beam is the direct strict
+beam at the centre sample (T - 1) // 2.
+"""
+import ctypes as ct
+
+import numpy as np
+import pytest
+
+from conftest import bits_equal, load_golden
+from oracle import oracle
+import beampower_amd as bp
+from beampower_amd import synthetic
+
+pytestmark = pytest.mark.gpu
+
+P_F, P_I = ct.POINTER(ct.c_float), ct.POINTER(ct.c_int)
+DIFF_CASES = ("differential_odd", "differential_even", "differential_nonfinite")
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def abi_differential(lib, F, tau, beta, init):
+    F = np.ascontiguousarray(F, np.float32)
+    tau = np.ascontiguousarray(tau, np.int32)
+    beta = np.ascontiguousarray(beta, np.float32)
+    S, T, P = F.shape
+    K = tau.shape[0]
+    b = np.ascontiguousarray(init, np.float32).copy()
+    rc = lib.beamform_differential(F.ctypes.data_as(P_F), tau.ctypes.data_as(P_I),
+                                   beta.ctypes.data_as(P_F), T, K, S, P, 4,
+                                   b.ctypes.data_as(P_F))
+    assert rc == 0, bp.core.last_error(lib)
+    return b
+
+
+@pytest.mark.parametrize("case", DIFF_CASES)
+def test_golden_through_c_abi(gpu_lib, case):
+    g = load_golden(case)
+    F, tau, beta = g["F_cmake"], g["tau"], g["beta"]
+    K = tau.shape[0]
+    assert bits_equal(abi_differential(gpu_lib, F, tau, beta, np.zeros(K)), g["diff_cmake"])
+    # sources left out keep the caller's values
+    init = np.arange(K, dtype=np.float32) + 0.5
+    got = abi_differential(gpu_lib, F, tau, beta, init)
+    assert bits_equal(got, oracle.beamform_differential(F, tau, beta, init=init))
+
+
+@pytest.mark.parametrize("case", DIFF_CASES)
+def test_golden_through_python_api(gpu_lib, case):
+    g = load_golden(case)
+    args = (g["u"], g["tau"], g["alpha"], g["beta"])
+    b = bp.beamform(*args, device="gpu", mode="differential", reduce="none")
+    assert bits_equal(b, g["diff_cmake"])
+    m, a = bp.beamform(*args, device="gpu", mode="differential", reduce="max")
+    assert bits_equal(np.asarray(m), np.max(g["diff_cmake"], axis=0))
+    assert a == np.argmax(g["diff_cmake"], axis=0)
+
+
+@pytest.mark.parametrize("seed,S,P,T,K", [(31, 7, 1, 1, 50), (32, 5, 2, 2, 77), (33, 33, 1, 64, 700),
+                                          (34, 11, 3, 101, 1000), (35, 4, 7, 40, 513),
+                                          (36, 17, 2, 257, 4096), (37, 9, 4, 50, 300),
+                                          (38, 3, 5, 31, 129)])
+def test_random_shapes_vs_oracle(gpu_lib, seed, S, P, T, K):
+    """One to seven phases (the 1-4 phase kernels load 16 / P stations at
+    a time, others one station at a time), station counts off those chunks,
+    source counts off the 64-source wave blocks."""
+    half = T // 2 + 3
+    u, alpha, tau, beta = synthetic.random_case(seed, S=S, C=2, P=P, T=T, K=K,
+                                                tau_range=(-half, half))
+    F = oracle.prestack(u, alpha)
+    init = np.full(K, -9.0, np.float32)
+    got = abi_differential(gpu_lib, F, tau, beta, init)
+    assert bits_equal(got, oracle.beamform_differential(F, tau, beta, init=init))
+
+
+def test_c5_device_api_vs_oracle_and_event(gpu_lib):
+    """20 stations (190 pairs), 4,000 sources: the device-resident network,
+    float32 planar features, bit-exact; the injected source is the argmax."""
+    torch = _torch()
+    from beampower_amd import device as bpd
+    w = synthetic.differential_c5()
+    F = oracle.prestack(w.waveform_features, w.weights_phases)
+    ref = oracle.beamform_differential(F, w.time_delays, w.weights_sources)
+    net = bpd.Network(w.time_delays, w.weights_sources)
+    Fd = torch.from_numpy(np.ascontiguousarray(F.transpose(0, 2, 1))).cuda()
+    b = net.beamform_differential(Fd).cpu().numpy()
+    assert bits_equal(b, ref)
+    assert int(np.argmax(b)) == w.events[0][0]
+
+
+@pytest.mark.parametrize("fmt", ["f32", "bf16", "f16"])
+def test_views_equal_direct_beam_at_centre(gpu_lib, fmt):
+    """Every storage format and padded rows: the differential beam is the
+    direct strict beam of the same view at the centre sample, and sources
+    out of range are left untouched in both."""
+    torch = _torch()
+    from beampower_amd import device as bpd
+    w = synthetic.differential_c5()
+    net = bpd.Network(w.time_delays, w.weights_sources)
+    Fd = bpd.prestack_planar(torch.from_numpy(w.waveform_features).cuda(),
+                             torch.from_numpy(w.weights_phases).cuda())
+    f = {"f32": bpd.BP_F32, "bf16": bpd.BP_BF16, "f16": bpd.BP_F16}[fmt]
+    V = bpd.pack(Fd, f, 5, 8)
+    K, T = w.time_delays.shape[0], w.waveform_features.shape[2]
+    init = torch.full((K,), -3.0, device="cuda")
+    b = net.beamform_differential(V, beam=init.clone()).cpu().numpy()
+    direct = torch.full((K, T), -3.0, device="cuda")
+    net.beamform(V, 0, beam=direct)
+    col = direct[:, (T - 1) // 2].contiguous().cpu().numpy()
+    assert bits_equal(b, col)
+    if fmt == "f32":
+        F = oracle.prestack(w.waveform_features, w.weights_phases)
+        assert bits_equal(b, oracle.beamform_differential(F, w.time_delays, w.weights_sources,
+                                                          init=np.full(K, -3.0, np.float32)))
+
+
+def test_c6_full_size_vs_oracle_and_direct(gpu_lib):
+    """50 stations (1,225 pairs), 192,000 sources, T = 1,581: bit-exact
+    against the oracle, and equal to the direct strict beams at the centre
+    sample, computed 16,000 sources at a time."""
+    torch = _torch()
+    from beampower_amd import device as bpd
+    w = synthetic.differential_c6()
+    F = oracle.prestack(w.waveform_features, w.weights_phases, num_threads=8)
+    ref = oracle.beamform_differential(F, w.time_delays, w.weights_sources, num_threads=8)
+    net = bpd.Network(w.time_delays, w.weights_sources)
+    Fd = torch.from_numpy(np.ascontiguousarray(F.transpose(0, 2, 1))).cuda()
+    b = net.beamform_differential(Fd)
+    torch.cuda.synchronize()
+    bh = b.cpu().numpy()
+    assert bits_equal(bh, ref)
+    assert int(np.argmax(bh)) == w.events[0][0]
+    K, T = w.time_delays.shape[0], F.shape[1]
+    t = (T - 1) // 2
+    for k0 in range(0, K, 16000):
+        k1 = min(K, k0 + 16000)
+        d = net.beamform(Fd, 0, source_begin=k0, source_end=k1)[:, t].contiguous()
+        assert bits_equal(d.cpu().numpy(), bh[k0:k1]), k0

@@ -63,10 +63,23 @@ def test_bad_mode_returns_one(capsys):
     assert bp.beamform(w[0], w[2], w[1], w[3], device="gpu", mode="sideways") == 1
 
 
-def test_differential_on_gpu_is_not_implemented(capsys):
-    w = synthetic.random_case(0, S=2, C=3, P=2, T=16, K=3)
-    assert bp.beamform(w[0], w[2], w[1], w[3], device="gpu", mode="differential") is None
-    assert "not yet implemented" in capsys.readouterr().out
+def test_differential_c_abi_checks_arguments_without_a_device():
+    """beamform_differential (the CPU library's signature) validates before
+    touching a device; shapes with nothing to compute leave `beam` as is."""
+    lib = bp.load_library("gpu")
+    P_F = ct.POINTER(ct.c_float)
+    assert lib.beamform_differential(None, None, None, 10, 4, 2, 2, 1, None) == -22
+    assert "NULL output" in core.last_error(lib)
+    b = np.full(4, 3.0, np.float32)
+    assert lib.beamform_differential(None, None, None, 10, 4, 2, 2, 1,
+                                     b.ctypes.data_as(P_F)) == -22
+    assert "NULL input" in core.last_error(lib)
+    for T, K, S, P in ((0, 4, 2, 2), (10, 0, 2, 2), (10, 4, 0, 2), (10, 4, 2, 0)):
+        assert lib.beamform_differential(None, None, None, T, K, S, P, 1,
+                                         b.ctypes.data_as(P_F)) == 0
+    assert np.all(b == 3.0)
+    assert lib.beamform_differential(None, None, None, 1 << 40, 4, 2, 2, 1, None) == -22
+    assert lib.bp_network_beamform_differential(None, None, None, None) == -22
 
 
 def test_c_abi_rejects_bad_arguments_without_a_device():

@@ -214,3 +214,59 @@ def test_fp64_max_vs_bruteforce(oob):
     srt = np.sort(masked, axis=0)
     ok = val & np.isfinite(srt[-2])
     np.testing.assert_allclose(sec[ok], srt[-2][ok], rtol=1e-12)
+
+
+# ---- mode='differential' (beamform.c:338-394) ------------------------------
+
+DIFF_CASES = ("differential_odd", "differential_even", "differential_nonfinite")
+
+
+@pytest.mark.parametrize("case", DIFF_CASES)
+@pytest.mark.parametrize("ref_flavour", ["cmake", "make"])
+def test_differential_oracle_matches_golden(case, ref_flavour):
+    g = load_golden(case)
+    fl = FLAV[ref_flavour]
+    F = oracle.prestack(g["u"], g["alpha"], flavour=fl)
+    assert bits_equal(F, g[f"F_{ref_flavour}"])
+    b = oracle.beamform_differential(F, g["tau"], g["beta"], flavour=fl)
+    assert bits_equal(b, g[f"diff_{ref_flavour}"])
+
+
+@pytest.mark.parametrize("case", DIFF_CASES)
+def test_differential_is_the_centre_column_of_the_direct_beams(case):
+    """The differential beam of a source is its direct strict beam at the
+    centre sample (T - 1) // 2, and the same sources are left untouched."""
+    g = load_golden(case)
+    F = oracle.prestack(g["u"], g["alpha"])
+    T = F.shape[1]
+    K = g["tau"].shape[0]
+    direct = oracle.beamform(F, g["tau"], g["beta"], 0, init=np.full((K, T), -7.0, np.float32))
+    diff = oracle.beamform_differential(F, g["tau"], g["beta"], init=np.full(K, -7.0, np.float32))
+    assert bits_equal(diff, np.ascontiguousarray(direct[:, (T - 1) // 2]))
+
+
+def test_differential_centre_planted_mistake_is_caught():
+    """An FP64 brute force at the right centre sample matches the reference's
+    golden beams; the same at T // 2 (a plausible off-by-one for even T)
+    does not."""
+    g = load_golden("differential_even")
+    F64 = bf.prestack(g["u"], g["alpha"])
+    T = F64.shape[1]
+    gold = g["diff_cmake"].astype(np.float64)
+    beams, valid = bf.beams(F64, g["tau"], g["beta"], flexible=False)
+    good = np.where(valid[:, (T - 1) // 2], beams[:, (T - 1) // 2], 0.0)
+    bad = np.where(valid[:, T // 2], beams[:, T // 2], 0.0)
+    assert np.allclose(good, gold, rtol=1e-5, atol=1e-5)
+    assert not np.allclose(bad, gold, rtol=1e-5, atol=1e-5)
+
+
+@pytest.mark.skipif(not oracle.ref_available("cmake"), reason="oracle/_ref not built")
+@pytest.mark.parametrize("seed,T", [(201, 1), (202, 2), (203, 257), (204, 1000)])
+def test_differential_oracle_matches_reference_build_live(seed, T):
+    half = T // 2 + 3
+    u, alpha, tau, beta = synthetic.random_case(seed, S=9, C=3, P=2, T=T, K=300,
+                                                tau_range=(-half, half))
+    for ref_flavour, fl in FLAV.items():
+        F = oracle.prestack(u, alpha, flavour=fl)
+        br = oracle.ref_beamform_differential(F, tau, beta, num_threads=1, flavour=ref_flavour)
+        assert bits_equal(oracle.beamform_differential(F, tau, beta, flavour=fl), br)
