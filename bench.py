@@ -238,6 +238,87 @@ def bench_serving(args, log):
     return out
 
 
+def bench_differential(args, log):
+    """mode='differential' at one-GPU shape: 50 stations (1,225 station
+    pairs), 192,000 sources, a 1,581-sample cross-correlation window.  One
+    step is one beam per source from resident tables (the wave-transposed
+    tables are built on the first call, outside the timed region).  The
+    kernel streams the delay and weight tables once, so it is priced by
+    those HBM bytes."""
+    import numpy as np
+    import torch
+    from beampower_amd import device as bpd
+    from beampower_amd import synthetic
+
+    log("differential form: generating workload")
+    w = synthetic.differential_c6()
+    S, C, T = w.waveform_features.shape
+    K, _, P = w.time_delays.shape
+    net = bpd.Network(w.time_delays, w.weights_sources)
+    F = bpd.prestack_planar(torch.from_numpy(w.waveform_features).cuda(),
+                            torch.from_numpy(w.weights_phases).cuda())
+    beam = torch.zeros(K, dtype=torch.float32, device="cuda")
+    for _ in range(max(1, args.warmup)):
+        net.beamform_differential(F, beam=beam)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    n = max(20, args.steps)
+    e0.record()
+    for _ in range(n):
+        net.beamform_differential(F, beam=beam)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / n
+    recovered = int(int(beam.argmax().item()) == w.events[0][0])
+    table_bytes = K * S * P * 4 + K * S * 4 + K * 4
+    feature_bytes = S * P * T * 4
+    achieved = (table_bytes + feature_bytes) / (ms * 1e-3)
+    out = {
+        "workload": w.name, "stations": w.meta["n_stations"], "station_pairs": S, "phases": P,
+        "sources": K, "samples": T, "value": K / (ms * 1e-3), "unit": "sources/s",
+        "ms_per_step": ms, "event_recovered": recovered,
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                     "kernel": "differential_kernel",
+                     "algorithmic_bytes_per_launch": table_bytes + feature_bytes},
+    }
+    log(f"differential: {ms:.3f} ms per step, {achieved / 1e9:.0f} GB/s")
+    del net, F, beam
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    if not args.no_cpu_baseline:
+        # the reference engine single-threaded (its per-source bounds are
+        # shared between OpenMP threads), whole calls repeated for about
+        # cpu_seconds
+        from oracle import oracle
+        kind = "reference" if oracle.ref_available("cmake") else "port"
+        Fh = oracle.prestack(w.waveform_features, w.weights_phases, num_threads=cpu_threads())
+        tau, beta = w.time_delays, w.weights_sources
+
+        def call():
+            if kind == "reference":
+                oracle.ref_beamform_differential(Fh, tau, beta, num_threads=1, flavour="cmake")
+            else:
+                oracle.beamform_differential(Fh, tau, beta, num_threads=1, flavour="fma")
+
+        t0 = time.perf_counter()
+        call()
+        first = time.perf_counter() - t0
+        reps = max(1, int(args.cpu_seconds / max(first, 1e-3)))
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            call()
+        dt = (time.perf_counter() - t0) / reps
+        out["cpu_baseline"] = {
+            "value": K / dt, "unit": "sources/s", "cores": 1, "kind": kind,
+            "sample": (f"beamform_differential, the whole workload ({K} sources, {S} pairs), "
+                       f"{reps} calls of {dt:.2f} s, one thread; "
+                       + ("reference beamform.c built by oracle/Makefile (CMake flags)"
+                          if kind == "reference" else "oracle port (fma flavour)"))}
+        out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -254,6 +335,10 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=3.0,
                     help="arrival span per offered rate")
     ap.add_argument("--serve-budget-gb", type=int, default=16)
+    ap.add_argument("--no-differential", action="store_true",
+                    help="skip the differential-mode form")
+    ap.add_argument("--differential-only", action="store_true",
+                    help="run only the differential-mode form and print its object")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs on a single MI355X "
@@ -268,6 +353,9 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs the GPU"
     torch.cuda.set_device(0)
     oob = 0 if args.oob == "strict" else 1
+    if args.differential_only:
+        print(json.dumps(bench_differential(args, log)), flush=True)
+        return
 
     log("generating workload")
     w = synthetic.dropin_c2()
@@ -362,6 +450,7 @@ def main():
     torch.cuda.empty_cache()
     batched = bench_batched(args, log) if args.batch_windows > 0 else None
     serving = bench_serving(args, log) if args.serve_rates else None
+    differential = None if args.no_differential else bench_differential(args, log)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -393,6 +482,7 @@ def main():
         "precision_forms": precision,
         "batched_form": batched,
         "serving_form": serving,
+        "differential_form": differential,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
