@@ -462,9 +462,10 @@ int network_create(const int *tau, const float *beta, size_t K, size_t S, size_t
   BP_TRY(net->tile_all.alloc(8 * (size_t)net->n_tiles * sizeof(int)));
   HIP_TRY(hipMemcpyAsync(net->tau.p, tau, ksp * sizeof(int), hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemcpyAsync(net->beta.p, beta, ks * sizeof(float), hipMemcpyHostToDevice, st));
-  hipLaunchKernelGGL(bp::source_bounds_kernel, dim3((unsigned)((K + 255) / 256)), dim3(256),
-                     0, st, net->tau.as<int>(), net->beta.as<float>(), net->K, net->S,
-                     net->P, net->lohi.as<int>());
+  hipLaunchKernelGGL(bp::source_bounds_kernel,
+                     dim3((unsigned)((K + bp::kBoundsSources - 1) / bp::kBoundsSources)),
+                     dim3(64 * bp::kBoundsSources), 0, st, net->tau.as<int>(),
+                     net->beta.as<float>(), net->K, net->S, net->P, net->lohi.as<int>());
   HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(bp::tile_bounds_kernel, dim3((unsigned)net->n_tiles), dim3(256), 0, st,
                      net->tau.as<int>(), net->beta.as<float>(), net->K, net->S, net->P,
@@ -497,6 +498,8 @@ int network_create(const int *tau, const float *beta, size_t K, size_t S, size_t
 int check_shapes(size_t T, size_t K, size_t S, size_t P) {
   if (!size_ok(T) || !size_ok(K) || !size_ok(S) || !size_ok(P))
     return fail(BP_EINVAL, "dimension too large (T=%zu K=%zu S=%zu P=%zu)", T, K, S, P);
+  if (S * P > (size_t)INT32_MAX - 4096)
+    return fail(BP_EINVAL, "n_stations * n_phases = %zu is too large", S * P);
   if (K > 0 && S > 0 && P > 0 && (K * S * P) / (S * P) != K)
     return fail(BP_EINVAL, "K*S*P overflows");
   return BP_OK;

@@ -41,24 +41,35 @@ struct StackArgs {
 // ---------------------------------------------------------------------------
 
 // beamform.c:10-47 -- per-source min/max over used stations (beta != 0).
-__global__ void source_bounds_kernel(const int *__restrict__ tau,
-                                     const float *__restrict__ beta, int K, int S,
-                                     int P, int *__restrict__ lohi) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= K) return;
+// One wave per source (kBoundsSources per work-group): the lanes stride
+// over the source's S*P delay entries, so every load instruction reads
+// consecutive words, then a wave reduction.
+constexpr int kBoundsSources = 4;
+__global__ __launch_bounds__(64 * kBoundsSources) void source_bounds_kernel(
+    const int *__restrict__ tau, const float *__restrict__ beta, int K, int S, int P,
+    int *__restrict__ lohi) {
+  const long long k = (long long)blockIdx.x * kBoundsSources + threadIdx.x / 64;
+  const int lane = threadIdx.x % 64;
+  if (k >= K) return;  // whole wave
+  const int SP = S * P;
+  const int *tk = tau + k * SP;
+  const float *bk = beta + k * S;
   int lo = INT_MAX, hi = INT_MIN;
-  const int *tk = tau + (size_t)k * S * P;
-  const float *bk = beta + (size_t)k * S;
-  for (int s = 0; s < S; ++s) {
-    if (bk[s] == 0.0f) continue;
-    for (int p = 0; p < P; ++p) {
-      int d = tk[s * P + p];
-      lo = d < lo ? d : lo;
-      hi = d > hi ? d : hi;
-    }
+  for (int j = lane; j < SP; j += 64) {
+    if (bk[j / P] == 0.0f) continue;  // the station is not used
+    const int d = tk[j];
+    lo = d < lo ? d : lo;
+    hi = d > hi ? d : hi;
   }
-  lohi[2 * k] = lo;
-  lohi[2 * k + 1] = hi;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = min(lo, __shfl_xor(lo, o));
+    hi = max(hi, __shfl_xor(hi, o));
+  }
+  if (lane == 0) {
+    lohi[2 * k] = lo;
+    lohi[2 * k + 1] = hi;
+  }
 }
 
 // Per source tile [tile*KT, tile*KT+KT): min delay of every (s,p) over the
