@@ -244,6 +244,70 @@ def bench_serving(args, log):
     return out
 
 
+def bench_none_subset(args, log, w, F_plain):
+    """reduce='none' on a seeded 4,096-source subset of the drop-in workload:
+    full (4096, 90000) beams from a resident network of the subset's rows,
+    features resident and padded for it.  The reference C engine's
+    reduce='none' (16 threads) is timed beside it on a prefix of the
+    subset."""
+    import numpy as np
+    import torch
+    from beampower_amd import device as bpd
+    from beampower_amd import synthetic
+
+    K_all, S, P = w.time_delays.shape
+    T = w.waveform_features.shape[2]
+    sub = synthetic.subset_sources(K_all, 4096, seed=2)
+    tau = np.ascontiguousarray(w.time_delays[sub])
+    beta = np.ascontiguousarray(w.weights_sources[sub])
+    K = len(sub)
+    net = bpd.Network(tau, beta)
+    V = bpd.pack(F_plain, bpd.BP_F32, *net.padding())
+    beam = torch.zeros((K, T), dtype=torch.float32, device="cuda")
+    for _ in range(max(1, args.warmup)):
+        net.beamform(V, 0, beam=beam)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    n = max(5, args.steps)
+    e0.record()
+    for _ in range(n):
+        net.beamform(V, 0, beam=beam)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / n
+    out = {"sources": K, "samples": T, "value": K * T / (ms * 1e-3),
+           "unit": "source·samples/s", "ms_per_step": ms,
+           "output_bytes": K * T * 4}
+    log(f"reduce='none' subset: {ms:.2f} ms per step")
+    del net, V, beam
+    torch.cuda.empty_cache()
+    if not args.no_cpu_baseline:
+        from oracle import oracle
+        kind = "reference" if oracle.ref_available("cmake") else "port"
+        threads = cpu_threads()
+        F = oracle.prestack(w.waveform_features, w.weights_phases, num_threads=threads)
+
+        def run(kk):
+            t0 = time.perf_counter()
+            if kind == "reference":
+                oracle.ref_beamform(F, tau[:kk], beta[:kk], 0, num_threads=threads,
+                                    flavour="cmake")
+            else:
+                oracle.beamform(F, tau[:kk], beta[:kk], 0, num_threads=threads, flavour="fma")
+            return time.perf_counter() - t0
+
+        kk = 64
+        dt = run(kk)
+        if dt < 0.5 * args.cpu_seconds:
+            kk = max(kk + 1, min(K, int(kk * 0.5 * args.cpu_seconds / dt)))
+            dt = run(kk)
+        out["cpu_baseline"] = {
+            "value": kk * T / dt, "unit": "source·samples/s", "cores": threads, "kind": kind,
+            "sample": (f"beamform reduce='none' strict, the first {kk} sources of the subset, "
+                       f"all {T} samples, {dt:.2f} s")}
+    return out
+
+
 def bench_differential(args, log):
     """mode='differential' at one-GPU shape: 50 stations (1,225 station
     pairs), 192,000 sources, a 1,581-sample cross-correlation window.  One
@@ -477,8 +541,11 @@ def main():
                                 / (kernel_ms * 1e-3)) / HBM_PEAK,
     }
 
-    del net, F, F_plain, bmax, barg, keys
+    del net, F, bmax, barg, keys
     torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    none_subset = bench_none_subset(args, log, w, F_plain)
+    del F_plain
     torch.cuda.empty_cache()
     batched = bench_batched(args, log) if args.batch_windows > 0 else None
     serving = bench_serving(args, log) if args.serve_rates else None
@@ -512,6 +579,7 @@ def main():
         "events_recovered": f"{recovered}/{len(w.events)}",
         "roofline": roofline,
         "dropin_host_pointers": dropin_host,
+        "reduce_none_subset": none_subset,
         "precision_forms": precision,
         "batched_form": batched,
         "serving_form": serving,
