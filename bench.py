@@ -411,7 +411,7 @@ def main():
                     help="run only the differential-mode form and print its object")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build runs on a single MI355X "
+        print(f"bench.py measures one GPU: the library runs on a single MI355X "
               f"(--gpus {args.gpus} is not supported)", file=sys.stderr)
         sys.exit(2)
 

@@ -29,7 +29,7 @@
  * against the reference compiled from its own sources with its own flags
  * (oracle/Makefile target `ref`).
  *
- * Bounds semantics (normative for this build):
+ * Bounds semantics (normative for the library and its tests):
  *   used(k,s)   <=> beta[k,s] != 0           (beamform.c:31,61,91)
  *   lo_k, hi_k  =  min/max of tau over used stations and all phases,
  *                  INT_MAX/INT_MIN when no station is used (beamform.c:22-23)

@@ -6,7 +6,7 @@ bench.py's cpu_baseline leg as the checker.  The product package
 
 Two kinds of CPU engines are exposed:
 
-* the oracle proper (oracle/beamform_oracle.c, this build's restatement of
+* the oracle proper (oracle/beamform_oracle.c, a restatement of
   the reference's beampower/src/beamform.c:10-336), in the two contraction
   flavours of the reference's builds: ``"fma"`` (CMakeLists.txt:28-33, the
   pip wheels) and ``"nofma"`` (Makefile:69, -std=c99);
