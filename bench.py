@@ -34,6 +34,7 @@ LDS_PEAK = N_CU * LDS_BYTES_PER_CLK_CU * CLOCK_HZ  # 157.3 TB/s
 HBM_PEAK = 8.0e12
 VALU_FP32_PEAK_FLOPS = 157.3e12          # published, counts packed FMA
 VALU_FP32_UNPACKED_FLOPS = 78.6e12       # v_fma_f32: 64 FMA/clk/CU
+VALU_FMA_MEASURED = 34.3e12              # v_fma_f32 probe (profiles/peaks_probe.md)
 
 
 def log(msg):
@@ -537,6 +538,9 @@ def main():
         "algorithmic_bytes_per_launch": lds_bytes,
         "valu_frac": (2.0 * fmas / (kernel_ms * 1e-3)) / VALU_FP32_PEAK_FLOPS,
         "valu_frac_unpacked": (2.0 * fmas / (kernel_ms * 1e-3)) / VALU_FP32_UNPACKED_FLOPS,
+        # against the v_fma_f32 rate the pipe probe measures under load
+        # (34.3 TFMA/s, profiles/peaks_probe.md)
+        "valu_frac_of_measured_peak": (fmas / (kernel_ms * 1e-3)) / VALU_FMA_MEASURED,
         "hbm_compulsory_frac": ((S * P * T * 4 + K * S * P * 4 + K * S * 4 + 8 * T)
                                 / (kernel_ms * 1e-3)) / HBM_PEAK,
     }
