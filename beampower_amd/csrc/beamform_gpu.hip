@@ -575,16 +575,30 @@ int init_batch_attrs() {
 
 }  // namespace
 
+// Windows of one LDS class run by one fused launch: descriptors (host copy
+// and device), their work items numbered consecutively, the XCD ranges.
+struct FusedSet {
+  std::vector<bp::BatchWindow> wins;
+  DevBuf dwins;
+  DevBuf dsplit;            // 9 item indices: XCD x walks [split[x], split[x+1])
+  long long n_items = 0;
+  long long max_range = 0;  // largest XCD range (items)
+  int max_lw = 0;
+  void add(const bp::StackArgs &a) {
+    wins.push_back(bp::BatchWindow{a, n_items});
+    n_items += (long long)a.n_time_tiles * a.n_list;
+    max_lw = std::max(max_lw, a.Lw);
+  }
+};
+
 struct bp_batch {
   int fmt = BP_F32, P = 0, n = 0, max_T = 0;
-  bp_batch_config cfg{kTimeGroup, 2, 0, 0.0f};
-  // per LDS class: window descriptors (host copy and device), item count, largest window
-  std::vector<bp::BatchWindow> wins[2];
-  DevBuf dwins[2];
-  DevBuf dsplit[2];  // per class: 9 item indices, the XCD ranges of the grid
-  long long max_range[2] = {0, 0};  // largest XCD range (items)
-  long long n_items[2] = {0, 0};
-  int max_lw[2] = {0, 0};
+  bp_batch_config cfg{kTimeGroup, 2, -1, 0.0f};
+  // per LDS class: every window fused (blocks_per_cu >= 0), or the windows
+  // that fill the GPU alone in launches of their own and the others fused
+  // (blocks_per_cu < 0)
+  FusedSet all[2], small[2];
+  std::vector<bp::StackArgs> large[2];
   std::vector<bp::StackArgs> direct;  // per window with direct-kernel tiles
   std::vector<int> direct_n;
   DevBuf dkeys;
@@ -602,50 +616,118 @@ struct bp_batch {
 
 namespace {
 
+// A window of at least this many work items (source tile x time tile)
+// fills the GPU's 512 work-group slots 8 times over.
+constexpr long long kLargeWindowItems = 4096;
+
 // Tuning results of this process, keyed by the batch's shape.
 std::mutex g_tune_mu;
 std::map<std::tuple<int, int, long long, long long, int, int>, bp_batch_config> g_tune_cache;
 
 std::tuple<int, int, long long, long long, int, int> tune_key(const bp_batch *b) {
-  return std::make_tuple(b->fmt, b->P, b->n_items[0], b->n_items[1], b->max_lw[0], b->max_lw[1]);
+  return std::make_tuple(b->fmt, b->P, b->all[0].n_items, b->all[1].n_items, b->all[0].max_lw,
+                         b->all[1].max_lw);
 }
 
 // Stage buffers usable for class c: 3 only when two work-groups still fit a CU.
 template <int FMT, int P>
 int class_nb(const bp_batch *b, int c) {
   return (b->cfg.stage_buffers == 3 &&
-          Lds<FMT, P, 3, false>::lds_bytes(b->max_lw[c]) <= kLdsPerCU / 2) ? 3 : 2;
+          Lds<FMT, P, 3, false>::lds_bytes(b->all[c].max_lw) <= kLdsPerCU / 2) ? 3 : 2;
 }
 
+// Upload a fused set's descriptors and cut its items into 8 XCD ranges at
+// equal shares of the cost (items x stations): windows differ 8x in
+// stations, and equal item counts made the fused launch 23 % slower than
+// the windows one by one.
+int finish_fused(FusedSet &f) {
+  if (f.wins.empty()) return BP_OK;
+  BP_TRY(f.dwins.alloc(f.wins.size() * sizeof(bp::BatchWindow)));
+  const std::vector<bp::BatchWindow> &ws = f.wins;
+  auto items_of = [&](size_t w) {
+    return (w + 1 < ws.size() ? ws[w + 1].item_begin : f.n_items) - ws[w].item_begin;
+  };
+  double total = 0.0;
+  for (size_t w = 0; w < ws.size(); ++w) total += (double)items_of(w) * ws[w].a.S;
+  long long split[9];
+  split[0] = 0;
+  split[8] = f.n_items;
+  size_t w = 0;
+  double before = 0.0;  // cost of the windows before w
+  for (int x = 1; x < 8; ++x) {
+    const double target = total * x / 8.0;
+    for (;;) {
+      const long long items = items_of(w);
+      const double cost = (double)items * ws[w].a.S;
+      if (before + cost >= target || w + 1 == ws.size()) {
+        const long long k = (long long)std::ceil((target - before) / ws[w].a.S);
+        split[x] = std::min(f.n_items, ws[w].item_begin + std::max(0LL, std::min(items, k)));
+        break;
+      }
+      before += cost;
+      ++w;
+    }
+    split[x] = std::max(split[x], split[x - 1]);
+  }
+  for (int x = 0; x < 8; ++x) f.max_range = std::max(f.max_range, split[x + 1] - split[x]);
+  BP_TRY(f.dsplit.alloc(sizeof(split)));
+  HIP_TRY(hipMemcpy(f.dsplit.p, split, sizeof(split), hipMemcpyHostToDevice));
+  return BP_OK;
+}
+
+// One window's tiles of class c in a launch of its own: the kernel a window
+// alone runs.
 template <int FMT, int P>
-int launch_batch_class(bp_batch *b, int c, hipStream_t st) {
-  if (b->n_items[c] == 0) return BP_OK;
-  const int nb = class_nb<FMT, P>(b, c);
-  const int lds = nb == 3 ? Lds<FMT, P, 3, false>::lds_bytes(b->max_lw[c])
-                          : Lds<FMT, P, 2, false>::lds_bytes(b->max_lw[c]);
-  // blocks_per_cu > 0: a persistent grid of that many work-groups per CU;
-  // 0: one work-group per item (8 x the largest XCD range; the hardware
-  // then refills CUs as work-groups finish, which keeps co-resident
-  // work-groups out of phase)
+int launch_window_alone(const bp::StackArgs &a, int nb, hipStream_t st) {
+  const long long blocks = (long long)a.n_time_tiles * a.n_list;
+  if (blocks > INT32_MAX) return fail(BP_EINVAL, "grid too large (%lld blocks)", blocks);
+  if (nb == 3)
+    hipLaunchKernelGGL((bp::stack_lds_kernel<FMT, P, kKW, kW, 3, false>), dim3((unsigned)blocks),
+                       dim3(64 * kW), (Lds<FMT, P, 3, false>::lds_bytes(a.Lw)), st, a);
+  else
+    hipLaunchKernelGGL((bp::stack_lds_kernel<FMT, P, kKW, kW, 2, false>), dim3((unsigned)blocks),
+                       dim3(64 * kW), (Lds<FMT, P, 2, false>::lds_bytes(a.Lw)), st, a);
+  HIP_TRY(hipGetLastError());
+  return BP_OK;
+}
+
+// A fused launch over the windows of `f`.  blocks_per_cu > 0: a persistent
+// grid of that many work-groups per CU; otherwise one work-group per item
+// (8 x the largest XCD range; the hardware then refills CUs as work-groups
+// finish, which keeps co-resident work-groups out of phase).
+template <int FMT, int P>
+int launch_fused(const FusedSet &f, int nb, int blocks_per_cu, hipStream_t st) {
+  if (f.n_items == 0) return BP_OK;
+  const int lds = nb == 3 ? Lds<FMT, P, 3, false>::lds_bytes(f.max_lw)
+                          : Lds<FMT, P, 2, false>::lds_bytes(f.max_lw);
   long long want;
-  if (b->cfg.blocks_per_cu == 0) {
-    want = 8 * b->max_range[c];
+  if (blocks_per_cu <= 0) {
+    want = 8 * f.max_range;
   } else {
-    const int per_cu = std::max(1, std::min(b->cfg.blocks_per_cu, kLdsPerCU / lds));
-    want = std::min((long long)kNumCU * per_cu, (b->n_items[c] + 7) / 8 * 8);
+    const int per_cu = std::max(1, std::min(blocks_per_cu, kLdsPerCU / lds));
+    want = std::min((long long)kNumCU * per_cu, (f.n_items + 7) / 8 * 8);
   }
   if (want > INT32_MAX) return fail(BP_EINVAL, "batch grid too large (%lld)", want);
   const unsigned grid = (unsigned)std::max<long long>(8, want);
   if (nb == 3)
     hipLaunchKernelGGL((bp::stack_batch_kernel<FMT, P, kKW, kW, 3>), dim3(grid), dim3(64 * kW),
-                       lds, st, b->dwins[c].as<bp::BatchWindow>(), (int)b->wins[c].size(),
-                       b->dsplit[c].as<long long>());
+                       lds, st, f.dwins.as<bp::BatchWindow>(), (int)f.wins.size(),
+                       f.dsplit.as<long long>());
   else
     hipLaunchKernelGGL((bp::stack_batch_kernel<FMT, P, kKW, kW, 2>), dim3(grid), dim3(64 * kW),
-                       lds, st, b->dwins[c].as<bp::BatchWindow>(), (int)b->wins[c].size(),
-                       b->dsplit[c].as<long long>());
+                       lds, st, f.dwins.as<bp::BatchWindow>(), (int)f.wins.size(),
+                       f.dsplit.as<long long>());
   HIP_TRY(hipGetLastError());
   return BP_OK;
+}
+
+template <int FMT, int P>
+int launch_batch_class(bp_batch *b, int c, hipStream_t st) {
+  if (b->all[c].n_items == 0) return BP_OK;
+  const int nb = class_nb<FMT, P>(b, c);
+  if (b->cfg.blocks_per_cu >= 0) return launch_fused<FMT, P>(b->all[c], nb, b->cfg.blocks_per_cu, st);
+  for (const bp::StackArgs &a : b->large[c]) BP_TRY((launch_window_alone<FMT, P>(a, nb, st)));
+  return launch_fused<FMT, P>(b->small[c], nb, 0, st);
 }
 
 template <int FMT>
@@ -691,15 +773,18 @@ int batch_enqueue(bp_batch *b, hipStream_t st, bool profile) {
 // Apply a configuration: the block order lives in the window descriptors.
 int batch_configure(bp_batch *b, const bp_batch_config &c) {
   if (c.time_group < 1 || c.time_group > 64 || (c.stage_buffers != 2 && c.stage_buffers != 3) ||
-      c.blocks_per_cu < 0 || c.blocks_per_cu > 4)
+      c.blocks_per_cu < -1 || c.blocks_per_cu > 4)
     return fail(BP_EINVAL, "invalid batch configuration (%d, %d, %d)", c.time_group,
                 c.stage_buffers, c.blocks_per_cu);
   b->cfg = c;
   for (int k = 0; k < 2; ++k) {
-    if (b->wins[k].empty()) continue;
-    for (auto &w : b->wins[k]) w.a.time_group = c.time_group;
-    HIP_TRY(hipMemcpy(b->dwins[k].p, b->wins[k].data(),
-                      b->wins[k].size() * sizeof(bp::BatchWindow), hipMemcpyHostToDevice));
+    for (FusedSet *f : {&b->all[k], &b->small[k]}) {
+      if (f->wins.empty()) continue;
+      for (auto &w : f->wins) w.a.time_group = c.time_group;
+      HIP_TRY(hipMemcpy(f->dwins.p, f->wins.data(), f->wins.size() * sizeof(bp::BatchWindow),
+                        hipMemcpyHostToDevice));
+    }
+    for (auto &a : b->large[k]) a.time_group = c.time_group;
   }
   b->drop_graph();
   return BP_OK;
@@ -1402,9 +1487,11 @@ int bp_batch_create(int n_windows, bp_network *const *networks,
       a.tiles = net->dtiles[c].as<int>();
       a.n_list = (int)net->tiles[c].size();
       a.Lw = net->Lw[c];
-      b->wins[c].push_back(bp::BatchWindow{a, b->n_items[c]});
-      b->n_items[c] += (long long)a.n_time_tiles * a.n_list;
-      b->max_lw[c] = std::max(b->max_lw[c], a.Lw);
+      b->all[c].add(a);
+      if ((long long)a.n_time_tiles * a.n_list >= kLargeWindowItems)
+        b->large[c].push_back(a);
+      else
+        b->small[c].add(a);
     }
     if (!net->tiles[2].empty()) {
       const int nd = (int)net->tiles[2].size();
@@ -1416,45 +1503,14 @@ int bp_batch_create(int n_windows, bp_network *const *networks,
     }
   }
   for (int c = 0; c < 2; ++c) {
-    if (b->wins[c].empty()) continue;
-    BP_TRY(b->dwins[c].alloc(b->wins[c].size() * sizeof(bp::BatchWindow)));
-    // XCD ranges at equal shares of items x stations
-    const std::vector<bp::BatchWindow> &ws = b->wins[c];
-    double total = 0.0;
-    for (size_t w = 0; w < ws.size(); ++w) {
-      const long long items = (w + 1 < ws.size() ? ws[w + 1].item_begin : b->n_items[c]) -
-                              ws[w].item_begin;
-      total += (double)items * ws[w].a.S;
-    }
-    long long split[9];
-    split[0] = 0;
-    split[8] = b->n_items[c];
-    size_t w = 0;
-    double before = 0.0;  // cost of the windows before w
-    for (int x = 1; x < 8; ++x) {
-      const double target = total * x / 8.0;
-      for (;;) {
-        const long long items = (w + 1 < ws.size() ? ws[w + 1].item_begin : b->n_items[c]) -
-                                ws[w].item_begin;
-        const double cost = (double)items * ws[w].a.S;
-        if (before + cost >= target || w + 1 == ws.size()) {
-          const long long k = (long long)std::ceil((target - before) / ws[w].a.S);
-          split[x] = std::min(b->n_items[c], ws[w].item_begin + std::max(0LL, std::min(items, k)));
-          break;
-        }
-        before += cost;
-        ++w;
-      }
-      split[x] = std::max(split[x], split[x - 1]);
-    }
-    for (int x = 0; x < 8; ++x) b->max_range[c] = std::max(b->max_range[c], split[x + 1] - split[x]);
-    BP_TRY(b->dsplit[c].alloc(sizeof(split)));
-    HIP_TRY(hipMemcpy(b->dsplit[c].p, split, sizeof(split), hipMemcpyHostToDevice));
+    BP_TRY(finish_fused(b->all[c]));
+    BP_TRY(finish_fused(b->small[c]));
   }
   BP_TRY(b->dkeys.alloc(kd.size() * sizeof(bp::BatchKeys)));
   HIP_TRY(hipMemcpy(b->dkeys.p, kd.data(), kd.size() * sizeof(bp::BatchKeys),
                     hipMemcpyHostToDevice));
   BP_TRY(init_batch_attrs());
+  BP_TRY(init_kernel_attrs());
   bp_batch_config cfg = b->cfg;
   {
     std::lock_guard<std::mutex> lock(g_tune_mu);
@@ -1511,7 +1567,7 @@ int bp_batch_tune(bp_batch *b, int reps, bp_batch_config *tried, int max_tried, 
   std::vector<bp_batch_config> cands;
   for (int tg : {1, 2, 4, 8})
     for (int nb : {2, 3})
-      for (int bpc : {0, 1, 2}) cands.push_back(bp_batch_config{tg, nb, bpc, 0.0f});
+      for (int bpc : {-1, 0, 1, 2}) cands.push_back(bp_batch_config{tg, nb, bpc, 0.0f});
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));

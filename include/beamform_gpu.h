@@ -227,8 +227,10 @@ typedef struct bp_batch bp_batch;
 typedef struct {
   int time_group;     /* time tiles per block-order group (1 = time-major) */
   int stage_buffers;  /* 2 or 3 (3 only where two work-groups still fit a CU) */
-  int blocks_per_cu;  /* persistent grid of that many work-groups per CU (1, 2), or 0:
-                         one work-group per work item */
+  int blocks_per_cu;  /* -1 (default): windows of >= 4096 work items in launches of their
+                         own, the others fused with one work-group per work item; 0: all
+                         fused, one work-group per work item; 1, 2: all fused, a persistent
+                         grid of that many work-groups per CU */
   float ms;           /* measured time of the whole batch (tuning output) */
 } bp_batch_config;
 int bp_batch_create(int n_windows, bp_network *const *networks,

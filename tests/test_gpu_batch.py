@@ -86,8 +86,15 @@ def test_batch_matches_windows_alone(fmt_name):
         batch.graph_launch()
         torch.cuda.synchronize()
         assert_batch_equals(batch, ref, f"graph replay {rep}")
+    for bpc in (-1, 1, 0):  # each window alone, persistent, one work-group per item
+        batch.set_config(blocks_per_cu=bpc)
+        for m in batch.beam_max:
+            m.zero_()
+        batch.graph_launch()
+        torch.cuda.synchronize()
+        assert_batch_equals(batch, ref, f"blocks_per_cu {bpc}")
     best, tried = batch.tune(reps=1)
-    assert len(tried) == 24 and all(t["ms"] > 0 for t in tried)
+    assert len(tried) == 32 and all(t["ms"] > 0 for t in tried)
     assert best["ms"] == min(t["ms"] for t in tried)
     assert batch.config()["time_group"] == best["time_group"]
     for m in batch.beam_max:
