@@ -69,6 +69,31 @@ def test_c2_max_matches_oracle_on_slices(c2, c2_max):
     assert found >= len(w.events) - 1, (found, len(w.events))
 
 
+def test_c2_flexible_max_matches_oracle_on_slices(gpu_lib, c2):
+    """Flexible bounds: with delays >= 0 a halo-padded slice drops exactly
+    the terms the full call drops (none before the last halo of samples)."""
+    w, F = c2
+    S, T, P = F.shape
+    tau, beta = w.time_delays, w.weights_sources
+    m = np.zeros(T, np.float32)
+    a = np.zeros(T, np.int32)
+    rc = gpu_lib.beamform_max(F.ctypes.data_as(P_F), tau.ctypes.data_as(P_I),
+                              beta.ctypes.data_as(P_F), T, tau.shape[0], S, P, 1,
+                              m.ctypes.data_as(P_F), a.ctypes.data_as(P_I))
+    assert rc == 0
+    halo = int(tau.max()) + 1
+    W = 160
+    for t0 in (0, T - W - halo):
+        Fs = np.ascontiguousarray(F[:, t0:t0 + W + halo, :])
+        mo, ao = oracle.beamform_max(Fs, tau, beta, 1, num_threads=16)
+        assert bits_equal(m[t0:t0 + W], mo[:W]), t0
+        assert np.array_equal(a[t0:t0 + W], ao[:W]), t0
+    # the last samples: every source is a candidate, with its in-range terms
+    Fs = np.ascontiguousarray(F[:, T - W:, :])
+    mo, ao = oracle.beamform_max(Fs, tau, beta, 1, num_threads=16)
+    assert bits_equal(m[T - W:], mo) and np.array_equal(a[T - W:], ao)
+
+
 def test_c2_none_subset_matches_oracle(gpu_lib, c2, c2_max):
     w, F = c2
     m, a = c2_max

@@ -1,0 +1,71 @@
+"""Re-entrancy of the drop-in C ABI: ctypes releases the GIL during a call,
+so Python threads can be inside the library together.  Every call uses its
+own stream, buffers and network plan; concurrent calls must give the same
+bits as the same calls made one at a time."""
+import ctypes as ct
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+import pytest
+
+from conftest import bits_equal
+from oracle import oracle
+from beampower_amd import synthetic
+
+pytestmark = pytest.mark.gpu
+
+P_F, P_I = ct.POINTER(ct.c_float), ct.POINTER(ct.c_int)
+
+
+def call(lib, kind, F, tau, beta, oob):
+    S, T, P = F.shape
+    K = tau.shape[0]
+    args = (F.ctypes.data_as(P_F), tau.ctypes.data_as(P_I), beta.ctypes.data_as(P_F), T, K, S, P)
+    if kind == "max":
+        m = np.zeros(T, np.float32)
+        a = np.zeros(T, np.int32)
+        rc = lib.beamform_max(*args, oob, m.ctypes.data_as(P_F), a.ctypes.data_as(P_I))
+        out = (m, a)
+    elif kind == "none":
+        b = np.zeros((K, T), np.float32)
+        rc = lib.beamform(*args, oob, b.ctypes.data_as(P_F))
+        out = (b,)
+    else:
+        b = np.zeros(K, np.float32)
+        rc = lib.beamform_differential(*args, 1, b.ctypes.data_as(P_F))
+        out = (b,)
+    assert rc == 0
+    return out
+
+
+def test_concurrent_calls_match_sequential(gpu_lib):
+    jobs = []
+    for i in range(12):
+        kind = ("max", "none", "diff")[i % 3]
+        T = 2000 + 137 * i
+        if kind == "diff":
+            half = T // 2
+            u, alpha, tau, beta = synthetic.random_case(70 + i, S=9, C=3, P=2, T=T, K=900,
+                                                        tau_range=(-half, half))
+        else:
+            u, alpha, tau, beta = synthetic.random_case(70 + i, S=9, C=3, P=2, T=T, K=900,
+                                                        tau_range=(0, 250))
+        F = oracle.prestack(u, alpha)
+        jobs.append((kind, F, tau, beta, i % 2))
+    seq = [call(gpu_lib, *j) for j in jobs]
+    with ThreadPoolExecutor(max_workers=6) as ex:
+        for rep in range(2):
+            par = list(ex.map(lambda j: call(gpu_lib, *j), jobs))
+            for i, (x, y) in enumerate(zip(seq, par)):
+                for u, v in zip(x, y):
+                    assert (bits_equal(u, v) if u.dtype == np.float32
+                            else np.array_equal(u, v)), (rep, i, jobs[i][0])
+    # and the sequential results are the oracle's
+    for (kind, F, tau, beta, oob), out in zip(jobs[:3], seq[:3]):
+        if kind == "max":
+            mo, ao = oracle.beamform_max(F, tau, beta, oob)
+            assert bits_equal(out[0], mo) and np.array_equal(out[1], ao)
+        elif kind == "none":
+            assert bits_equal(out[0], oracle.beamform(F, tau, beta, oob))
+        else:
+            assert bits_equal(out[0], oracle.beamform_differential(F, tau, beta))
