@@ -916,13 +916,25 @@ struct StackLds {
         if (!a.oob)
           strict_range(as_const(a.src_lohi)[2 * k], as_const(a.src_lohi)[2 * k + 1], a.T, vlo,
                        vhi);
+        if (vlo <= t0 && t0 + TT <= vhi) {
+          // every sample of the tile is valid for source k (wave-uniform)
 #pragma unroll
-        for (int j = 0; j < NACC; ++j) {
-          const int t = sample_of(t0, lane, j);
-          const float v = acc[kk][j];
-          if (t >= vlo && t < vhi && v > bv[j]) {
-            bv[j] = v;
-            bk[j] = k;
+          for (int j = 0; j < NACC; ++j) {
+            const float v = acc[kk][j];
+            if (v > bv[j]) {
+              bv[j] = v;
+              bk[j] = k;
+            }
+          }
+        } else {
+#pragma unroll
+          for (int j = 0; j < NACC; ++j) {
+            const int t = sample_of(t0, lane, j);
+            const float v = acc[kk][j];
+            if (t >= vlo && t < vhi && v > bv[j]) {
+              bv[j] = v;
+              bk[j] = k;
+            }
           }
         }
       }
