@@ -144,3 +144,13 @@ def test_relative_moveouts_and_plain_geometry():
     assert tau.max() <= int(np.sqrt(50 ** 2 * 2 + 20 ** 2) / 3.5 * 25) + 1
     assert w.waveform_features.shape == (20, 3, 6000)
     assert w.weights_phases[:, 0, 0].min() == 1 and w.weights_phases[:, 1:, 1].max() == 0.5
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; asking for more exits with a message."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(REPO, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
