@@ -792,21 +792,25 @@ int batch_configure(bp_batch *b, const bp_batch_config &c) {
 
 // Differential mode (beamform.c:338-394), one thread per source.
 // (K, S, P) delays and (K, S) weights on the device -> the wave-transposed
-// tables of the differential kernel (64-source blocks, entry-major).
+// tables of the differential kernel (64-source blocks, entry-major), padded
+// to bp::diff_padded_stations(S, P) stations.
 int build_diff_tables(const int *tau, const float *beta, long long K, int S, int P,
                       DevBuf &tauT, DevBuf &betaT, hipStream_t st) {
   const long long nblk = (K + bp::kDiffBlock - 1) / bp::kDiffBlock;
-  const long long sp = (long long)S * P;
-  BP_TRY(tauT.alloc((size_t)(nblk * sp * bp::kDiffBlock) * sizeof(int)));
-  BP_TRY(betaT.alloc((size_t)(nblk * S * bp::kDiffBlock) * sizeof(float)));
+  const int S_pad = bp::diff_padded_stations(S, P);
+  const long long sp = (long long)S * P, sp_pad = (long long)S_pad * P;
+  BP_TRY(tauT.alloc((size_t)(nblk * sp_pad * bp::kDiffBlock) * sizeof(int)));
+  BP_TRY(betaT.alloc((size_t)(nblk * S_pad * bp::kDiffBlock) * sizeof(float)));
   auto tiles = [](long long e) {
     return (unsigned)std::min<long long>((e + bp::kDiffBlock - 1) / bp::kDiffBlock, 1024);
   };
-  hipLaunchKernelGGL(bp::diff_tables_kernel, dim3((unsigned)nblk, tiles(sp)), dim3(256), 0, st,
-                     reinterpret_cast<const unsigned *>(tau), K, (int)sp, tauT.as<unsigned>());
+  hipLaunchKernelGGL(bp::diff_tables_kernel, dim3((unsigned)nblk, tiles(sp_pad)), dim3(256), 0,
+                     st, reinterpret_cast<const unsigned *>(tau), K, (int)sp, (int)sp_pad,
+                     tauT.as<unsigned>());
   HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(bp::diff_tables_kernel, dim3((unsigned)nblk, tiles(S)), dim3(256), 0, st,
-                     reinterpret_cast<const unsigned *>(beta), K, S, betaT.as<unsigned>());
+  hipLaunchKernelGGL(bp::diff_tables_kernel, dim3((unsigned)nblk, tiles(S_pad)), dim3(256), 0,
+                     st, reinterpret_cast<const unsigned *>(beta), K, S, S_pad,
+                     betaT.as<unsigned>());
   HIP_TRY(hipGetLastError());
   return BP_OK;
 }
@@ -816,33 +820,34 @@ void launch_differential_p(const bp::StackArgs &a, const int *tauT, const float 
                            float *beam, hipStream_t st) {
   const dim3 grid((unsigned)((a.K + bp::kDiffSources - 1) / bp::kDiffSources));
   const dim3 block(bp::kDiffSources);
+  const int S_pad = bp::diff_padded_stations(a.S, a.P);
   switch (a.P) {
     case 1:
       hipLaunchKernelGGL((bp::differential_kernel<FMT, 1>), grid, block, 0, st, a, tauT, betaT,
-                         beam);
+                         S_pad, beam);
       break;
     case 2:
       hipLaunchKernelGGL((bp::differential_kernel<FMT, 2>), grid, block, 0, st, a, tauT, betaT,
-                         beam);
+                         S_pad, beam);
       break;
     case 3:
       hipLaunchKernelGGL((bp::differential_kernel<FMT, 3>), grid, block, 0, st, a, tauT, betaT,
-                         beam);
+                         S_pad, beam);
       break;
     case 4:
       hipLaunchKernelGGL((bp::differential_kernel<FMT, 4>), grid, block, 0, st, a, tauT, betaT,
-                         beam);
+                         S_pad, beam);
       break;
     default:
       hipLaunchKernelGGL((bp::differential_kernel<FMT, 0>), grid, block, 0, st, a, tauT, betaT,
-                         beam);
+                         S_pad, beam);
   }
 }
 
 // Size limits of the kernel's 32-bit offsets.
 int check_differential(int fmt, long long S, long long P, long long ldf) {
   const long long sp = S * P, rows = sp * (fmt == BP_F32 ? 1 : 2);
-  if (sp * bp::kDiffBlock >= (1ll << 30))
+  if ((S + 32) * P * bp::kDiffBlock >= (1ll << 30))
     return fail(BP_EINVAL, "n_stations * n_phases = %lld is too large for differential mode", sp);
   if (rows * ldf >= (1ll << 30))
     return fail(BP_EINVAL, "differential mode takes features under 2^30 elements (%lld rows of "

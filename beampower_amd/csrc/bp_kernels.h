@@ -1137,28 +1137,37 @@ __global__ __launch_bounds__(256) void stack_direct_kernel(StackArgs a, int chun
 constexpr int kDiffSources = 256;  // per work-group: 4 independent waves
 constexpr int kDiffBlock = 64;     // sources per wave block of the transposed tables
 
-// (K, E) row-major 4-byte table -> (ceil(K / 64), E, 64) wave-transposed,
-// zero past the last source.  Grid (ceil(K / 64), any), 256 threads, the
-// y blocks striding over 64-entry tiles; a 64 x 64 tile goes through LDS so
-// both sides are coalesced.
+// (K, E_in) row-major 4-byte table -> (ceil(K / 64), E_out, 64) wave-transposed,
+// E_out >= E_in, zero past the last source and past entry E_in.  Grid
+// (ceil(K / 64), any), 256 threads, the y blocks striding over 64-entry
+// tiles; a 64 x 64 tile goes through LDS so both sides are coalesced.
 __global__ __launch_bounds__(256) void diff_tables_kernel(const unsigned *__restrict__ in,
-                                                          long long K, int E,
+                                                          long long K, int E_in, int E_out,
                                                           unsigned *__restrict__ out) {
   __shared__ unsigned tile[kDiffBlock][kDiffBlock + 1];
   const long long b = blockIdx.x;
-  for (int e0 = blockIdx.y * kDiffBlock; e0 < E; e0 += gridDim.y * kDiffBlock) {
+  for (int e0 = blockIdx.y * kDiffBlock; e0 < E_out; e0 += gridDim.y * kDiffBlock) {
     __syncthreads();  // the previous tile is written out
     for (int i = threadIdx.x; i < kDiffBlock * kDiffBlock; i += blockDim.x) {
       const int r = i / kDiffBlock, c = i % kDiffBlock;
       const long long k = b * kDiffBlock + r;
-      tile[r][c] = (k < K && e0 + c < E) ? in[k * E + e0 + c] : 0u;
+      tile[r][c] = (k < K && e0 + c < E_in) ? in[k * E_in + e0 + c] : 0u;
     }
     __syncthreads();
     for (int i = threadIdx.x; i < kDiffBlock * kDiffBlock; i += blockDim.x) {
       const int c = i / kDiffBlock, r = i % kDiffBlock;
-      if (e0 + c < E) out[(b * E + e0 + c) * kDiffBlock + r] = tile[r][c];
+      if (e0 + c < E_out) out[(b * E_out + e0 + c) * kDiffBlock + r] = tile[r][c];
     }
   }
+}
+
+// Stations per chunk of the differential kernel for P phases (1-4), and the
+// station count of its tables: S rounded up to two chunks, plus one chunk
+// that the last prefetch reads.  Padding stations have zero weights.
+__host__ __device__ constexpr int diff_chunk_stations(int P) { return P >= 1 && P <= 4 ? 16 / P : 1; }
+__host__ __device__ inline int diff_padded_stations(int S, int P) {
+  const int cs = diff_chunk_stations(P);
+  return (S + 2 * cs - 1) / (2 * cs) * (2 * cs) + cs;
 }
 
 // Feature element t of planar row `row` (= s * P + p): float32 rows, or
@@ -1184,9 +1193,9 @@ __device__ __forceinline__ float feature_row(const StackArgs &a, unsigned row, u
 // beamform.c:353-354,370-380; this is its one-thread behaviour).
 template <int FMT, int PT>
 __global__ __launch_bounds__(kDiffSources) void differential_kernel(
-    StackArgs a, const int *__restrict__ tauT, const float *__restrict__ betaT,
+    StackArgs a, const int *__restrict__ tauT, const float *__restrict__ betaT, int S_pad,
     float *__restrict__ beam) {
-  constexpr int CS = PT > 0 ? 16 / PT : 1;  // stations per chunk
+  constexpr int CS = diff_chunk_stations(PT);  // stations per chunk (PT > 0)
   constexpr int NE = PT > 0 ? CS * PT : 1;
   const int lane = threadIdx.x % kDiffBlock;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kDiffBlock);
@@ -1195,56 +1204,19 @@ __global__ __launch_bounds__(kDiffSources) void differential_kernel(
   if (k0 >= a.K) return;  // whole wave past the last source
   const int S = a.S, P = PT > 0 ? PT : a.P;
   const long long T = a.T, t = (T - 1) / 2;
-  // 32-bit offsets inside the wave's block: 64 * S * P < 2^30 (checked by the host)
-  const int *tb = tauT + blk * S * P * kDiffBlock + lane;
-  const float *wb = betaT + blk * S * kDiffBlock + lane;
+  // 32-bit offsets inside the wave's block: 64 * S_pad * P < 2^30 (checked by the host)
+  const int *tb = tauT + blk * S_pad * P * kDiffBlock + lane;
+  const float *wb = betaT + blk * S_pad * kDiffBlock + lane;
   float acc = 0.0f;
   bool used = false, inside = true;
-  for (int s0 = 0; s0 < S; s0 += CS) {
-    int d[NE];
-    float b[CS], x[NE];
-    // the chunk's weights and delays (stations past S read station S - 1 and count as unused)
-#pragma unroll
-    for (int c = 0; c < CS; ++c) {
-      const int s = min(s0 + c, S - 1);
-      b[c] = wb[(unsigned)s * kDiffBlock];
-      if (PT > 0) {
-#pragma unroll
-        for (int p = 0; p < PT; ++p) d[c * PT + p] = tb[(unsigned)(s * PT + p) * kDiffBlock];
-      }
-    }
-    if (PT > 0) {
-      // then the gathers; entries not used or out of range read element 0 of row 0
-#pragma unroll
-      for (int c = 0; c < CS; ++c) {
-#pragma unroll
-        for (int p = 0; p < PT; ++p) {
-          const long long tg = t + d[c * PT + p];
-          const bool live = s0 + c < S && b[c] != 0.0f && tg >= 0 && tg < T;
-          x[c * PT + p] = feature_row<FMT>(a, live ? (unsigned)((s0 + c) * PT + p) : 0u,
-                                           live ? (unsigned)tg : 0u);
-        }
-      }
-#pragma unroll
-      for (int c = 0; c < CS; ++c) {
-        const bool u = s0 + c < S && b[c] != 0.0f;
-        const float sc = (FMT == BP_F32 || !a.scale) ? 1.0f : a.scale[min(s0 + c, S - 1)];
-        const float wt = applied_weight<FMT>(b[c], sc);
-        used |= u;
-#pragma unroll
-        for (int p = 0; p < PT; ++p) {
-          const long long tg = t + d[c * PT + p];
-          const bool in = tg >= 0 && tg < T;
-          inside &= !u || in;
-          const float y = fmaf(wt, x[c * PT + p], acc);
-          acc = (u && in) ? y : acc;
-        }
-      }
-    } else {
-      // any phase count: one station at a time (CS == 1)
-      const bool u = b[0] != 0.0f;
+
+  if (PT == 0) {
+    // any phase count: one station at a time
+    for (int s0 = 0; s0 < S; ++s0) {
+      const float b0 = wb[(unsigned)s0 * kDiffBlock];
+      const bool u = b0 != 0.0f;
       const float sc = (FMT == BP_F32 || !a.scale) ? 1.0f : a.scale[s0];
-      const float wt = applied_weight<FMT>(b[0], sc);
+      const float wt = applied_weight<FMT>(b0, sc);
       used |= u;
       for (int p = 0; p < P; ++p) {
         const long long tg = t + tb[(unsigned)(s0 * P + p) * kDiffBlock];
@@ -1257,6 +1229,76 @@ __global__ __launch_bounds__(kDiffSources) void differential_kernel(
         acc = live ? y : acc;
       }
     }
+    if (k0 + lane < a.K && used && inside) beam[k0 + lane] = acc;
+    return;
+  }
+
+  // 1-4 phases: chunks of CS stations.  The tables are padded to S_pad
+  // stations (pairs of whole chunks and one more) with zero weights, so
+  // every chunk is whole and the padding stations are simply unused.  The next chunk's
+  // delays and weights are in flight while the current chunk's gathers and
+  // FMAs run (two register sets used in turn; the scheduling barriers keep
+  // the gathers ahead of those loads, so waiting for a gather never waits
+  // for them).
+  struct Chunk {
+    int d[NE];
+    float b[CS];
+  };
+  auto load = [&](int s0, Chunk &ch) {
+#pragma unroll
+    for (int c = 0; c < CS; ++c) {
+      ch.b[c] = wb[(unsigned)(s0 + c) * kDiffBlock];
+#pragma unroll
+      for (int p = 0; p < PT; ++p)
+        ch.d[c * PT + p] = tb[(unsigned)((s0 + c) * PT + p) * kDiffBlock];
+    }
+  };
+  auto gather = [&](int s0, const Chunk &ch, float (&x)[NE]) {
+    // entries not used or out of range read element 0 of row 0
+#pragma unroll
+    for (int c = 0; c < CS; ++c) {
+#pragma unroll
+      for (int p = 0; p < PT; ++p) {
+        const long long tg = t + ch.d[c * PT + p];
+        const bool live = ch.b[c] != 0.0f && tg >= 0 && tg < T;
+        x[c * PT + p] = feature_row<FMT>(a, live ? (unsigned)((s0 + c) * PT + p) : 0u,
+                                         live ? (unsigned)tg : 0u);
+      }
+    }
+  };
+  auto chain = [&](int s0, const Chunk &ch, const float (&x)[NE]) {
+#pragma unroll
+    for (int c = 0; c < CS; ++c) {
+      const bool u = ch.b[c] != 0.0f;
+      const float sc =
+          (FMT == BP_F32 || !a.scale) ? 1.0f : a.scale[s0 + c < S ? s0 + c : S - 1];
+      const float wt = applied_weight<FMT>(ch.b[c], sc);
+      used |= u;
+#pragma unroll
+      for (int p = 0; p < PT; ++p) {
+        const long long tg = t + ch.d[c * PT + p];
+        const bool in = tg >= 0 && tg < T;
+        inside &= !u || in;
+        const float y = fmaf(wt, x[c * PT + p], acc);
+        acc = (u && in) ? y : acc;
+      }
+    }
+  };
+  Chunk c0, c1;
+  float x[NE];
+  const int s_end = S_pad - CS;  // whole pairs of chunks; the last chunk is read ahead only
+  load(0, c0);
+  for (int s0 = 0; s0 < s_end; s0 += 2 * CS) {
+    gather(s0, c0, x);
+    __builtin_amdgcn_sched_barrier(0);
+    load(s0 + CS, c1);
+    __builtin_amdgcn_sched_barrier(0);
+    chain(s0, c0, x);
+    gather(s0 + CS, c1, x);
+    __builtin_amdgcn_sched_barrier(0);
+    load(s0 + 2 * CS, c0);
+    __builtin_amdgcn_sched_barrier(0);
+    chain(s0 + CS, c1, x);
   }
   if (k0 + lane < a.K && used && inside) beam[k0 + lane] = acc;
 }

@@ -4,7 +4,8 @@
 // Every LDS index is masked into s[]; the program prints one line per probe.
 // `probe_peaks copy` instead runs three copies of 256 MiB (read and written
 // once each), the known-size kernel that calibrates the FETCH_SIZE and
-// WRITE_SIZE counters.
+// WRITE_SIZE counters, then three streaming reads of 2 GiB: the HBM
+// bandwidth a kernel can reach, beside the 8 TB/s of the data sheet.
 // Build: python -m beampower_amd.build --probe  (-> beampower_amd/lib/probe_peaks)
 #include <hip/hip_runtime.h>
 #include <cstdio>
@@ -41,18 +42,57 @@ __global__ void copy4(const float4 *__restrict__ a, float4 *__restrict__ b, long
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) b[i] = a[i];
 }
+// Streaming read: each thread sums `per` float4 a grid-stride apart and
+// writes one float (reads dominate: 2 GiB in, 4 MiB out).
+__global__ __launch_bounds__(256) void read4(const float4 *__restrict__ a, float *__restrict__ o,
+                                             long n) {
+  const long i0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long stride = (long)gridDim.x * blockDim.x;
+  float acc = 0.f;
+  for (long i = i0; i < n; i += stride) {
+    const float4 v = a[i];
+    acc += v.x + v.y + v.z + v.w;
+  }
+  o[i0] = acc;
+}
 int copy_calibration() {
   const long bytes = 256l << 20, n = bytes / 16;
   float4 *a, *b;
   if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) return 1;
   (void)hipMemset(a, 0, bytes);
-  for (int i = 0; i < 3; ++i)
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  float ms = 0.f;
+  for (int i = 0; i < 3; ++i) {
+    (void)hipEventRecord(e0);
     hipLaunchKernelGGL(copy4, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, a, b, n);
-  (void)hipDeviceSynchronize();
-  printf("copy4: %ld bytes read, %ld bytes written per launch (%s)\n", bytes, bytes,
-         hipGetErrorString(hipGetLastError()));
+    (void)hipEventRecord(e1);
+    (void)hipEventSynchronize(e1);
+    (void)hipEventElapsedTime(&ms, e0, e1);
+  }
+  printf("copy4: %ld bytes read, %ld bytes written per launch, %.3f ms, %.2f TB/s read+write (%s)\n",
+         bytes, bytes, ms, 2.0 * bytes / ms / 1e9, hipGetErrorString(hipGetLastError()));
   (void)hipFree(a);
   (void)hipFree(b);
+  // read bandwidth: 2 GiB, 8 blocks of 256 threads per CU
+  const long rbytes = 2048l << 20, rn = rbytes / 16;
+  const unsigned rblocks = 256 * 8;
+  float *o;
+  if (hipMalloc(&a, rbytes) != hipSuccess || hipMalloc(&o, (size_t)rblocks * 256 * 4) != hipSuccess)
+    return 1;
+  (void)hipMemset(a, 0, rbytes);
+  for (int i = 0; i < 3; ++i) {
+    (void)hipEventRecord(e0);
+    hipLaunchKernelGGL(read4, dim3(rblocks), dim3(256), 0, 0, a, o, rn);
+    (void)hipEventRecord(e1);
+    (void)hipEventSynchronize(e1);
+    (void)hipEventElapsedTime(&ms, e0, e1);
+  }
+  printf("read4: %ld bytes read per launch, %.3f ms, %.2f TB/s (%s)\n", rbytes, ms,
+         rbytes / ms / 1e9, hipGetErrorString(hipGetLastError()));
+  (void)hipFree(a);
+  (void)hipFree(o);
   return 0;
 }
 int main(int argc, char **argv) {

@@ -35,6 +35,7 @@ HBM_PEAK = 8.0e12
 VALU_FP32_PEAK_FLOPS = 157.3e12          # published, counts packed FMA
 VALU_FP32_UNPACKED_FLOPS = 78.6e12       # v_fma_f32: 64 FMA/clk/CU
 VALU_FMA_MEASURED = 34.3e12              # v_fma_f32 probe (profiles/peaks_probe.md)
+HBM_READ_MEASURED = 6.1e12               # streaming-read probe (profiles/peaks_probe.md)
 
 
 def log(msg):
@@ -350,6 +351,7 @@ def bench_differential(args, log):
         "ms_per_step": ms, "event_recovered": recovered,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                     "frac_of_measured_stream": achieved / HBM_READ_MEASURED,
                      "kernel": "differential_kernel",
                      "algorithmic_bytes_per_launch": table_bytes + feature_bytes},
     }
