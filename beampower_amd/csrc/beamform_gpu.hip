@@ -70,7 +70,7 @@ constexpr int kSmallLw = 1024;   // windows of the common tiles
 constexpr int kLargeLw = bp::kMaxWindow;  // largest window
 constexpr int kLdsPerCU = 160 * 1024;
 constexpr int kNumCU = 256;        // MI355X
-constexpr int kTimeGroup = 1;      // block order: time-major (see bp_kernels.h, block_tile)
+constexpr int kTimeGroup = 8;      // block order: groups of 8 time tiles (bp_kernels.h, block_tile)
 constexpr double kSmallShare = 0.97;  // share of tiles the small window must cover
 constexpr int kDirectChunk = kTileSources;
 

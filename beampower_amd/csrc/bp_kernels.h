@@ -429,10 +429,12 @@ __global__ void finalize_max_kernel(const unsigned long long *__restrict__ keys,
 // are dispatched round-robin over the 8 XCDs, so block b runs on XCD b % 8;
 // XCD x is given the contiguous range [x*N/8, (x+1)*N/8) of an order that
 // walks groups of TG consecutive time tiles, all listed source tiles per
-// group and the group's time tiles innermost.  With TG = 1 the work-groups
-// running together on an XCD share one time tile, so every station's
-// feature windows are fetched from HBM once and then hit in that XCD's L2;
-// larger groups trade that for reuse of each source tile's schedule.
+// group and the group's time tiles innermost.  The work-groups running
+// together on an XCD then share TG time tiles, whose feature windows stay in
+// that XCD's L2, and read each source tile's schedule once per TG time
+// tiles instead of once per time tile.  TG = 8 at the 1 h, 192,000-source
+// drop-in shape: 1.9 MB of windows per XCD, 5.3 GB of HBM reads per launch
+// (41 GB with TG = 1, where the schedule stream evicts the windows).
 __device__ __forceinline__ int xcd_remap(int b, int n) {
   const int n8 = n & ~7;
   if (b >= n8) return b;

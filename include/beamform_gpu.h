@@ -225,7 +225,7 @@ int bp_prestack_planar(const float *u_sct, const float *weights_phases,
  * batches of the same shape in this process. */
 typedef struct bp_batch bp_batch;
 typedef struct {
-  int time_group;     /* time tiles per block-order group (1 = time-major) */
+  int time_group;     /* time tiles per block-order group (default 8; 1 = time-major) */
   int stage_buffers;  /* 2 or 3 (3 only where two work-groups still fit a CU) */
   int blocks_per_cu;  /* -1 (default): windows of >= 4096 work items in launches of their
                          own, the others fused with one work-group per work item; 0: all
