@@ -896,10 +896,19 @@ struct StackLds {
           strict_range(as_const(a.src_lohi)[2 * k], as_const(a.src_lohi)[2 * k + 1], a.T, vlo,
                        vhi);
         float *row = a.beam + (size_t)(k - a.k_begin) * a.T;
+        if (vlo <= t0 && t0 + TT <= vhi && ((size_t)row & 15) == 0) {
+          // the whole tile is valid for source k and the row 16-byte aligned
+          // (wave-uniform): each run of SPA samples as 16-byte stores
 #pragma unroll
-        for (int j = 0; j < NACC; ++j) {
-          const int t = sample_of(t0, lane, j);
-          if (t >= vlo && t < vhi) row[t] = acc[kk][j];
+          for (int j = 0; j < NACC; j += 4)
+            *reinterpret_cast<float4 *>(row + sample_of(t0, lane, j)) =
+                make_float4(acc[kk][j], acc[kk][j + 1], acc[kk][j + 2], acc[kk][j + 3]);
+        } else {
+#pragma unroll
+          for (int j = 0; j < NACC; ++j) {
+            const int t = sample_of(t0, lane, j);
+            if (t >= vlo && t < vhi) row[t] = acc[kk][j];
+          }
         }
       }
     } else {
