@@ -1,7 +1,7 @@
 /*
  * beamform_oracle.c -- CPU oracle for the beamforming hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker: it is loaded by
+ * TEST INFRASTRUCTURE ONLY.  This file is the oracle: it is loaded by
  * tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg,
  * and never by the product library (beampower_amd/lib/beamform_gpu.so),
  * which has no CPU fallback.

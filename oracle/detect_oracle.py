@@ -1,6 +1,6 @@
 """CPU restatement of the tutorial's detection step on beam_max.
 
-TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker of the library's
+TEST INFRASTRUCTURE ONLY: imported by tests/ as the oracle of the library's
 detection consumer (bp_detect_threshold / bp_detect_peaks).
 
 Reference: notebooks/3_beampower.ipynb

@@ -1,7 +1,7 @@
 """ctypes front-end of the CPU oracle.
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg as the checker.  The product package
+bench.py's cpu_baseline leg, to check results.  The product package
 (beampower_amd) never imports this module.
 
 Two kinds of CPU engines are exposed:
