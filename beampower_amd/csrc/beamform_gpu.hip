@@ -23,6 +23,7 @@
 
 #include "beamform_gpu.h"
 #include "bp_kernels.h"
+#include "bp_order.h"
 
 #define BP_VERSION "0.1.0"
 
@@ -207,6 +208,11 @@ struct bp_network {
   DevBuf sched[3], scale[3];
   DevBuf diff_tau, diff_beta;  // differential mode: wave-transposed tables, built on first use
   bool diff_ok = false;
+  // source order: tau, beta and the plan are in position order; perm maps a
+  // position to the caller's index (device and host), inv the reverse (host)
+  bool permuted = false;
+  DevBuf perm;
+  std::vector<int> order, inv;
   bool sched_ok[3] = {false, false, false};
   std::vector<float> sched_scale[3];
   std::mutex mu;  // schedule (re)builds
@@ -222,6 +228,7 @@ bp::StackArgs make_args(const bp_network *net, const void *F, int fmt, int T, in
   a.tau = net->tau.as<int>();
   a.beta = net->beta.as<float>();
   a.src_lohi = net->lohi.as<int>();
+  a.perm = net->permuted ? net->perm.as<int>() : nullptr;
   a.tile_lo = net->tile_lo.as<int>();
   a.tile_all = net->tile_all.as<int>();
   a.sched = net->sched[fmt].as<int>();
@@ -328,8 +335,19 @@ int launch_stack(const bp_network *net, bp::StackArgs a, int k_begin, int k_end,
   a.k_begin = k_begin;
   a.k_end = k_end;
   if (k_end <= k_begin || a.T <= 0) return BP_OK;
-  const int tile0 = k_begin / kTileSources;
-  const int tile1 = (k_end - 1) / kTileSources + 1;
+  // tiles holding the sources [k_begin, k_end): with a source order, the
+  // range of their positions (tiles in between write nothing)
+  int tile0 = k_begin / kTileSources;
+  int tile1 = (k_end - 1) / kTileSources + 1;
+  if (net->permuted && (k_begin > 0 || k_end < net->K)) {
+    int lo = INT_MAX, hi = -1;
+    for (int k = k_begin; k < k_end; ++k) {
+      lo = std::min(lo, net->inv[k]);
+      hi = std::max(hi, net->inv[k]);
+    }
+    tile0 = lo / kTileSources;
+    tile1 = hi / kTileSources + 1;
+  }
   hipEvent_t ev[2];
   BP_TRY(prof_begin(st, ev));
   for (int cls = 0; cls < 3; ++cls) {
@@ -447,21 +465,13 @@ void classify_tiles(bp_network *net, const std::vector<int> &tile_all) {
   net->pad_hi = round_up8(hi_max);
 }
 
-int network_create(const int *tau, const float *beta, size_t K, size_t S, size_t P,
-                   hipStream_t st, std::unique_ptr<bp_network> &out) {
-  auto net = std::make_unique<bp_network>();
-  net->K = (int)K;
-  net->S = (int)S;
-  net->P = (int)P;
-  net->n_tiles = (int)((K + kTileSources - 1) / kTileSources);
-  const size_t ksp = K * S * P, ks = K * S, tsp = (size_t)net->n_tiles * S * P;
-  BP_TRY(net->tau.alloc(ksp * sizeof(int)));
-  BP_TRY(net->beta.alloc(ks * sizeof(float)));
+// Bounds, tile classes and tile lists of a network whose delays and
+// weights (in position order) are on the device.
+int build_plan(bp_network *net, hipStream_t st) {
+  const size_t K = net->K, S = net->S, P = net->P, tsp = (size_t)net->n_tiles * S * P;
   BP_TRY(net->lohi.alloc(2 * K * sizeof(int)));
   BP_TRY(net->tile_lo.alloc(tsp * sizeof(int)));
   BP_TRY(net->tile_all.alloc(8 * (size_t)net->n_tiles * sizeof(int)));
-  HIP_TRY(hipMemcpyAsync(net->tau.p, tau, ksp * sizeof(int), hipMemcpyHostToDevice, st));
-  HIP_TRY(hipMemcpyAsync(net->beta.p, beta, ks * sizeof(float), hipMemcpyHostToDevice, st));
   hipLaunchKernelGGL(bp::source_bounds_kernel,
                      dim3((unsigned)((K + bp::kBoundsSources - 1) / bp::kBoundsSources)),
                      dim3(64 * bp::kBoundsSources), 0, st, net->tau.as<int>(),
@@ -475,7 +485,7 @@ int network_create(const int *tau, const float *beta, size_t K, size_t S, size_t
   HIP_TRY(hipMemcpyAsync(tile_all.data(), net->tile_all.p, tile_all.size() * sizeof(int),
                          hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
-  classify_tiles(net.get(), tile_all);
+  classify_tiles(net, tile_all);
 
   std::vector<int> tile_lw(net->n_tiles, 0);
   for (int c = 0; c < 2; ++c)
@@ -489,7 +499,72 @@ int network_create(const int *tau, const float *beta, size_t K, size_t S, size_t
       HIP_TRY(hipMemcpyAsync(net->dtiles[c].p, net->tiles[c].data(),
                              net->tiles[c].size() * sizeof(int), hipMemcpyHostToDevice, st));
   }
-  HIP_TRY(hipStreamSynchronize(st));
+  HIP_TRY(hipStreamSynchronize(st));  // the host vectors are read by the copies
+  return BP_OK;
+}
+
+// Relative cost of a plan's stack launches: a tile of an LDS class costs
+// 440 + its window length (a linear fit of the measured time per tile of
+// 592- and 1,200-sample windows on the 50-station, 192k-source, 1 h
+// workload: the staging grows with the window, the gathers do not), a
+// direct-kernel tile ten 1,024-sample tiles.
+double plan_cost(const bp_network *net) {
+  double c = 0.0;
+  for (int cls = 0; cls < 2; ++cls) c += (double)net->tiles[cls].size() * (440.0 + net->Lw[cls]);
+  return c + (double)net->tiles[2].size() * 10.0 * (440.0 + 1024.0);
+}
+
+// Order policy of the per-call entry points: the similarity order is tried
+// only when the natural plan's common window is wide (the per-call plan
+// then pays for itself); otherwise as BP_ORDER_NATURAL.
+constexpr int kOrderAutoWide = 3;
+
+int network_create(const int *tau, const float *beta, size_t K, size_t S, size_t P,
+                   int order, hipStream_t st, std::unique_ptr<bp_network> &out) {
+  auto make = [&]() {
+    auto n = std::make_unique<bp_network>();
+    n->K = (int)K;
+    n->S = (int)S;
+    n->P = (int)P;
+    n->n_tiles = (int)((K + kTileSources - 1) / kTileSources);
+    return n;
+  };
+  const size_t ksp = K * S * P, ks = K * S;
+  auto net = make();
+  BP_TRY(net->tau.alloc(ksp * sizeof(int)));
+  BP_TRY(net->beta.alloc(ks * sizeof(float)));
+  HIP_TRY(hipMemcpyAsync(net->tau.p, tau, ksp * sizeof(int), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(net->beta.p, beta, ks * sizeof(float), hipMemcpyHostToDevice, st));
+  BP_TRY(build_plan(net.get(), st));
+
+  bool try_order = order == BP_ORDER_SIMILARITY || (order == BP_ORDER_AUTO && K > kTileSources);
+  if (order == kOrderAutoWide)
+    try_order = K > kTileSources && has_lds_tiles(net.get()) && net->Lw[0] > kTT + 256;
+  if (try_order) {
+    std::vector<int> ord = bp::similarity_order(tau, K, S * P, kTileSources);
+    auto alt = make();
+    BP_TRY(alt->perm.alloc(K * sizeof(int)));
+    BP_TRY(alt->tau.alloc(ksp * sizeof(int)));
+    BP_TRY(alt->beta.alloc(ks * sizeof(float)));
+    HIP_TRY(hipMemcpyAsync(alt->perm.p, ord.data(), K * sizeof(int), hipMemcpyHostToDevice, st));
+    const unsigned gb = (unsigned)std::min<size_t>(4096, (ksp + 255) / 256);
+    hipLaunchKernelGGL(bp::gather_rows_kernel, dim3(gb), dim3(256), 0, st,
+                       net->tau.as<unsigned>(), alt->perm.as<int>(), (long long)K, (int)(S * P),
+                       alt->tau.as<unsigned>());
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(bp::gather_rows_kernel, dim3(gb), dim3(256), 0, st,
+                       net->beta.as<unsigned>(), alt->perm.as<int>(), (long long)K, (int)S,
+                       alt->beta.as<unsigned>());
+    HIP_TRY(hipGetLastError());
+    BP_TRY(build_plan(alt.get(), st));  // synchronizes: `ord` stays alive until then
+    if (order == BP_ORDER_SIMILARITY || plan_cost(alt.get()) < plan_cost(net.get())) {
+      alt->permuted = true;
+      alt->inv.assign(K, 0);
+      for (size_t i = 0; i < K; ++i) alt->inv[ord[i]] = (int)i;
+      alt->order = std::move(ord);
+      net = std::move(alt);
+    }
+  }
   BP_TRY(ensure_schedule(net.get(), BP_F32, nullptr, st));
   out = std::move(net);
   return BP_OK;
@@ -900,7 +975,16 @@ int bp_profiling_read(double *stack_ms_total, long long *stack_launches) {
 
 int bp_network_create(const int *time_delays, const float *weights_sources, size_t n_sources,
                       size_t n_stations, size_t n_phases, bp_network **out) {
+  return bp_network_create_ordered(time_delays, weights_sources, n_sources, n_stations,
+                                    n_phases, BP_ORDER_AUTO, out);
+}
+
+int bp_network_create_ordered(const int *time_delays, const float *weights_sources,
+                              size_t n_sources, size_t n_stations, size_t n_phases, int order,
+                              bp_network **out) {
   if (!out) return fail(BP_EINVAL, "out is NULL");
+  if (order != BP_ORDER_NATURAL && order != BP_ORDER_SIMILARITY && order != BP_ORDER_AUTO)
+    return fail(BP_EINVAL, "unknown source order policy %d", order);
   *out = nullptr;
   if (n_sources == 0 || n_stations == 0 || n_phases == 0)
     return fail(BP_EINVAL, "network needs n_sources, n_stations, n_phases > 0");
@@ -910,14 +994,22 @@ int bp_network_create(const int *time_delays, const float *weights_sources, size
   Stream st;
   BP_TRY(st.create());
   std::unique_ptr<bp_network> net;
-  BP_TRY(network_create(time_delays, weights_sources, n_sources, n_stations, n_phases, st.s,
-                        net));
+  BP_TRY(network_create(time_delays, weights_sources, n_sources, n_stations, n_phases, order,
+                        st.s, net));
   *out = net.release();
   return BP_OK;
 }
 
 int bp_network_destroy(bp_network *net) {
   delete net;
+  return BP_OK;
+}
+
+int bp_network_source_order(const bp_network *net, int *permuted, int *order) {
+  if (!net) return fail(BP_EINVAL, "NULL network");
+  if (permuted) *permuted = net->permuted ? 1 : 0;
+  if (order)
+    for (int i = 0; i < net->K; ++i) order[i] = net->permuted ? net->order[i] : i;
   return BP_OK;
 }
 
@@ -1120,15 +1212,42 @@ int bp_network_beamform_differential(bp_network *net, const bp_feature_view *fea
   if (fmt != BP_F32) BP_TRY(ensure_schedule(net, fmt, features->station_scale, st));
   {
     // the transposed tables, built on first use and kept with the network
+    // The tables are in the caller's source order even when the stack plan
+    // orders sources by similarity: a wave's 64 lanes are 64 sources, and on
+    // a grid 64 consecutive indices gather from fewer cache lines than a
+    // similarity tile (0.51 against 0.59 ms at 1,225 pairs x 192,000 sources).
     std::lock_guard<std::mutex> lock(net->mu);
     if (!net->diff_ok) {
-      BP_TRY(build_diff_tables(net->tau.as<int>(), net->beta.as<float>(), net->K, net->S,
-                               net->P, net->diff_tau, net->diff_beta, st));
+      if (net->permuted) {
+        const size_t K = net->K, S = net->S, P = net->P;
+        DevBuf inv, tau, beta;
+        BP_TRY(inv.alloc(K * sizeof(int)));
+        BP_TRY(tau.alloc(K * S * P * sizeof(int)));
+        BP_TRY(beta.alloc(K * S * sizeof(float)));
+        HIP_TRY(hipMemcpyAsync(inv.p, net->inv.data(), K * sizeof(int), hipMemcpyHostToDevice,
+                               st));
+        const unsigned gb = (unsigned)std::min<size_t>(4096, (K * S * P + 255) / 256);
+        hipLaunchKernelGGL(bp::gather_rows_kernel, dim3(gb), dim3(256), 0, st,
+                           net->tau.as<unsigned>(), inv.as<int>(), (long long)K, (int)(S * P),
+                           tau.as<unsigned>());
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(bp::gather_rows_kernel, dim3(gb), dim3(256), 0, st,
+                           net->beta.as<unsigned>(), inv.as<int>(), (long long)K, (int)S,
+                           beta.as<unsigned>());
+        HIP_TRY(hipGetLastError());
+        BP_TRY(build_diff_tables(tau.as<int>(), beta.as<float>(), net->K, net->S, net->P,
+                                 net->diff_tau, net->diff_beta, st));
+        HIP_TRY(hipStreamSynchronize(st));  // before the temporaries are freed
+      } else {
+        BP_TRY(build_diff_tables(net->tau.as<int>(), net->beta.as<float>(), net->K, net->S,
+                                 net->P, net->diff_tau, net->diff_beta, st));
+      }
       HIP_TRY(hipStreamSynchronize(st));  // ready for calls on any stream
       net->diff_ok = true;
     }
   }
-  const bp::StackArgs a = make_args_view(net, *features, 0);
+  bp::StackArgs a = make_args_view(net, *features, 0);
+  a.perm = nullptr;  // tables in the caller's order
   return launch_differential(fmt, a, net->diff_tau.as<int>(), net->diff_beta.as<float>(), beam,
                              st);
 }
@@ -1337,8 +1456,8 @@ int beamform_max(float *waveform_features, int *time_delays, float *weights_sour
   Stream st;
   BP_TRY(st.create());
   std::unique_ptr<bp_network> net;
-  BP_TRY(network_create(time_delays, weights_sources, n_sources, n_stations, n_phases, st.s,
-                        net));
+  BP_TRY(network_create(time_delays, weights_sources, n_sources, n_stations, n_phases,
+                        kOrderAutoWide, st.s, net));
   const size_t T = n_samples, nf = n_stations * n_samples * n_phases;
   bp_feature_view v{};
   DevBuf f_stp, f_pad, keys, bmax, barg;
@@ -1379,8 +1498,8 @@ int beamform(float *waveform_features, int *time_delays, float *weights_sources,
   Stream st;
   BP_TRY(st.create());
   std::unique_ptr<bp_network> net;
-  BP_TRY(network_create(time_delays, weights_sources, n_sources, n_stations, n_phases, st.s,
-                        net));
+  BP_TRY(network_create(time_delays, weights_sources, n_sources, n_stations, n_phases,
+                        kOrderAutoWide, st.s, net));
   bp_feature_view v{};
   DevBuf f_stp, f_pad, dbeam;
   BP_TRY(stage_dropin_features(net.get(), waveform_features, n_stations, T, n_phases, f_stp,

@@ -21,6 +21,8 @@ struct StackArgs {
   const int *tau;         // (K, S, P) moveouts
   const float *beta;      // (K, S) source weights
   const int *src_lohi;    // (K, 2) per-source min/max delay over used stations
+  const int *perm;        // (K) original index of the source at each position, or NULL
+                          // (identity); tau, beta, src_lohi and the tiles are in position order
   const int *tile_lo;     // (n_tiles, S, P) min delay over the tile's users of (s,p)
   const int *tile_all;    // (n_tiles, 8) min lo, max hi, max lo, max (hi - lo), non-finite weight flag
   const int *sched;       // (n_tiles, S, KT, P+1) LDS byte offsets and weight bits
@@ -31,9 +33,9 @@ struct StackArgs {
   int Lw;                 // LDS window length (samples), multiple of 8
   int n_time_tiles;
   int time_group;         // block order: time tiles per group (>= 1)
-  int k_begin, k_end;     // sources written: [k_begin, k_end)
-  unsigned long long *keys;  // reduce=max: (T) packed keys
-  float *beam;               // reduce=none: beam[(k - k_begin) * T + t]
+  int k_begin, k_end;     // sources written, by original index: [k_begin, k_end)
+  unsigned long long *keys;  // reduce=max: (T) packed keys of (value, original index)
+  float *beam;               // reduce=none: beam[(original index - k_begin) * T + t]
 };
 
 // ---------------------------------------------------------------------------
@@ -141,6 +143,18 @@ __host__ __device__ __forceinline__ unsigned bf16_bits(float x) {
 // w0 + c + SPA*m, w0 = t0 + the staging start.  The Lw words of a phase are
 // rounded up to whole 1 KiB rows (64 lanes x 16 bytes, one staging
 // instruction per wave).
+// Source order of a plan: out[i, :] = in[perm[i], :] over rows of D 32-bit
+// words (delays, weights).
+__global__ void gather_rows_kernel(const unsigned *__restrict__ in, const int *__restrict__ perm,
+                                   long long K, int D, unsigned *__restrict__ out) {
+  const long long n = K * D;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long k = i / D;
+    out[i] = in[(long long)perm[k] * D + (i - k * D)];
+  }
+}
+
 constexpr int kMaxWindow = 2048;  // largest LDS window (samples)
 __host__ __device__ __forceinline__ int window_slot_bytes(int Lw) { return (Lw + 63) / 64 * 1024; }
 
@@ -506,6 +520,12 @@ typedef const __attribute__((address_space(4))) int const_int;
 
 __device__ __forceinline__ const_int *as_const(const int *p) {
   return (const_int *)(p);
+}
+
+// Original index of the source at position k of a network's plan.
+template <class A>
+__device__ __forceinline__ int source_index(const A &a, int k) {
+  return a.perm ? as_const(a.perm)[k] : k;
 }
 
 typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
@@ -890,12 +910,14 @@ struct StackLds {
 #pragma unroll
       for (int kk = 0; kk < KW; ++kk) {
         const int k = kbase + kk;
-        if (k < a.k_begin || k >= a.k_end) continue;
+        if (k >= a.K) continue;
+        const int ko = source_index(a, k);
+        if (ko < a.k_begin || ko >= a.k_end) continue;
         int vlo = 0, vhi = a.T;
         if (!a.oob)
           strict_range(as_const(a.src_lohi)[2 * k], as_const(a.src_lohi)[2 * k + 1], a.T, vlo,
                        vhi);
-        float *row = a.beam + (size_t)(k - a.k_begin) * a.T;
+        float *row = a.beam + (size_t)(ko - a.k_begin) * a.T;
         if (vlo <= t0 && t0 + TT <= vhi && ((size_t)row & 15) == 0) {
           // the whole tile is valid for source k and the row 16-byte aligned
           // (wave-uniform): each run of SPA samples as 16-byte stores
@@ -919,10 +941,16 @@ struct StackLds {
         bv[j] = -FLT_MAX;
         bk[j] = 0;
       }
+      // Candidates in position order; a later position wins a tie only
+      // with a lower original index, so every lane keeps, of its sources,
+      // the largest value and among equal values the lowest original index
+      // (the reference's ascending scan with '>').
 #pragma unroll
       for (int kk = 0; kk < KW; ++kk) {
         const int k = kbase + kk;
-        if (k < a.k_begin || k >= a.k_end) continue;
+        if (k >= a.K) continue;
+        const int ko = source_index(a, k);
+        if (ko < a.k_begin || ko >= a.k_end) continue;
         int vlo = 0, vhi = a.T;
         if (!a.oob)
           strict_range(as_const(a.src_lohi)[2 * k], as_const(a.src_lohi)[2 * k + 1], a.T, vlo,
@@ -932,9 +960,9 @@ struct StackLds {
 #pragma unroll
           for (int j = 0; j < NACC; ++j) {
             const float v = acc[kk][j];
-            if (v > bv[j]) {
+            if (v > bv[j] || (v == bv[j] && ko < bk[j])) {
               bv[j] = v;
-              bk[j] = k;
+              bk[j] = ko;
             }
           }
         } else {
@@ -942,9 +970,9 @@ struct StackLds {
           for (int j = 0; j < NACC; ++j) {
             const int t = sample_of(t0, lane, j);
             const float v = acc[kk][j];
-            if (t >= vlo && t < vhi && v > bv[j]) {
+            if (t >= vlo && t < vhi && (v > bv[j] || (v == bv[j] && ko < bk[j]))) {
               bv[j] = v;
-              bk[j] = k;
+              bk[j] = ko;
             }
           }
         }
@@ -1099,12 +1127,14 @@ template <int FMT, bool NONE>
 __global__ __launch_bounds__(256) void stack_direct_kernel(StackArgs a, int chunk) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long tile = a.tiles[blockIdx.y];
-  const int k0 = (int)max((long long)a.k_begin, tile * chunk);
-  const int k1 = (int)min((long long)a.k_end, tile * chunk + chunk);
+  const int k0 = (int)(tile * chunk);
+  const int k1 = (int)min((long long)a.K, tile * chunk + chunk);
   const int S = a.S, P = a.P, T = a.T;
   float bv = -FLT_MAX;
   int bk = 0;
   for (int k = k0; k < k1; ++k) {
+    const int ko = a.perm ? a.perm[k] : k;
+    if (ko < a.k_begin || ko >= a.k_end) continue;
     int vlo = 0, vhi = T;
     if (!a.oob) strict_range(a.src_lohi[2 * k], a.src_lohi[2 * k + 1], T, vlo, vhi);
     if (t >= T || t < vlo || t >= vhi) continue;
@@ -1120,10 +1150,10 @@ __global__ __launch_bounds__(256) void stack_direct_kernel(StackArgs a, int chun
       }
     }
     if (NONE) {
-      a.beam[(size_t)(k - a.k_begin) * T + t] = acc;
-    } else if (acc > bv) {
+      a.beam[(size_t)(ko - a.k_begin) * T + t] = acc;
+    } else if (acc > bv || (acc == bv && ko < bk)) {
       bv = acc;
-      bk = k;
+      bk = ko;
     }
   }
   if (!NONE && t < T && bv > -FLT_MAX) atomicMax(a.keys + t, pack_key(bv, bk));

@@ -29,11 +29,22 @@ def _stream_ptr(stream):
     return ct.c_void_p(stream.cuda_stream)
 
 
+# Source order of a network's plan (bp_network_create_ordered).
+ORDERS = {"natural": 0, "similarity": 1, "auto": 2}
+
+
 class Network:
     """Moveouts (K,S,P) int32 and source weights (K,S) float32 resident on the
-    current device, with per-source and per-source-tile delay bounds."""
+    current device, with per-source and per-source-tile delay bounds.
 
-    def __init__(self, time_delays, weights_sources):
+    order: "auto" (the default: sources grouped into tiles by delay
+    similarity when that plan is cheaper), "similarity" (always) or
+    "natural" (tiles of consecutive source indices).  Results do not depend
+    on it: beams are indexed, and max ties broken, by the caller's index."""
+
+    def __init__(self, time_delays, weights_sources, order="auto"):
+        if order not in ORDERS:
+            raise ValueError(f"order must be one of {sorted(ORDERS)}, not {order!r}")
         self.lib = load_library("gpu")
         tau = np.ascontiguousarray(time_delays, dtype=np.int32)
         beta = np.ascontiguousarray(weights_sources, dtype=np.float32)
@@ -41,9 +52,10 @@ class Network:
             raise ValueError(f"shapes {tau.shape} / {beta.shape} do not match (K,S,P) / (K,S)")
         self.n_sources, self.n_stations, self.n_phases = tau.shape
         handle = ct.c_void_p()
-        check(self.lib.bp_network_create(tau.ctypes.data, beta.ctypes.data, self.n_sources,
-                                         self.n_stations, self.n_phases, ct.byref(handle)),
-              "bp_network_create", self.lib)
+        check(self.lib.bp_network_create_ordered(tau.ctypes.data, beta.ctypes.data,
+                                                 self.n_sources, self.n_stations, self.n_phases,
+                                                 ORDERS[order], ct.byref(handle)),
+              "bp_network_create_ordered", self.lib)
         self._h = handle
 
     def close(self):
@@ -63,6 +75,15 @@ class Network:
               self.lib)
         return dict(source_tile=tile.value, max_tile_spread=spread.value, n_tiles=n.value,
                     tiles_per_class=list(counts), window_per_class=list(windows))
+
+    def source_order(self):
+        """(permuted, order): whether the plan reorders the sources, and the
+        caller's source index at each plan position (int32, n_sources)."""
+        permuted = ct.c_int()
+        order = np.empty(self.n_sources, dtype=np.int32)
+        check(self.lib.bp_network_source_order(self._h, ct.byref(permuted), order.ctypes.data),
+              "bp_network_source_order", self.lib)
+        return bool(permuted.value), order
 
     def padding(self):
         """(pad_before, pad_after): zero padding of feature rows under which

@@ -326,7 +326,9 @@ def bench_differential(args, log):
     w = synthetic.differential_c6()
     S, C, T = w.waveform_features.shape
     K, _, P = w.time_delays.shape
-    net = bpd.Network(w.time_delays, w.weights_sources)
+    # the differential kernel does not use the stack plan's source order
+    # (its tables are in the caller's order), so none is computed
+    net = bpd.Network(w.time_delays, w.weights_sources, order="natural")
     F = bpd.prestack_planar(torch.from_numpy(w.waveform_features).cuda(),
                             torch.from_numpy(w.weights_phases).cuda())
     beam = torch.zeros(K, dtype=torch.float32, device="cuda")
@@ -440,6 +442,7 @@ def main():
     del u
     net = bpd.Network(w.time_delays, w.weights_sources)
     info = net.info()
+    info["source_order"] = "similarity" if net.source_order()[0] else "natural"
     # rows zero-padded for this network (as the drop-in entry points stage
     # them): every LDS tile takes the staged path
     pad = net.padding()
@@ -581,6 +584,7 @@ def main():
             "source_tile": info["source_tile"], "max_tile_spread": info["max_tile_spread"],
             "tiles_per_class": info["tiles_per_class"], "window_per_class": info["window_per_class"],
             "row_padding": list(pad),
+            "source_order": info["source_order"],
         },
         "events_recovered": f"{recovered}/{len(w.events)}",
         "roofline": roofline,
