@@ -941,10 +941,10 @@ struct StackLds {
         bv[j] = -FLT_MAX;
         bk[j] = 0;
       }
-      // Candidates in position order; a later position wins a tie only
-      // with a lower original index, so every lane keeps, of its sources,
+      // Candidates in position order, which within a tile is ascending
+      // original index: with a strict '>' every lane keeps, of its sources,
       // the largest value and among equal values the lowest original index
-      // (the reference's ascending scan with '>').
+      // (the reference's ascending scan).
 #pragma unroll
       for (int kk = 0; kk < KW; ++kk) {
         const int k = kbase + kk;
@@ -960,7 +960,7 @@ struct StackLds {
 #pragma unroll
           for (int j = 0; j < NACC; ++j) {
             const float v = acc[kk][j];
-            if (v > bv[j] || (v == bv[j] && ko < bk[j])) {
+            if (v > bv[j]) {
               bv[j] = v;
               bk[j] = ko;
             }
@@ -970,7 +970,7 @@ struct StackLds {
           for (int j = 0; j < NACC; ++j) {
             const int t = sample_of(t0, lane, j);
             const float v = acc[kk][j];
-            if (t >= vlo && t < vhi && (v > bv[j] || (v == bv[j] && ko < bk[j]))) {
+            if (t >= vlo && t < vhi && v > bv[j]) {
               bv[j] = v;
               bk[j] = ko;
             }
@@ -1151,7 +1151,7 @@ __global__ __launch_bounds__(256) void stack_direct_kernel(StackArgs a, int chun
     }
     if (NONE) {
       a.beam[(size_t)(ko - a.k_begin) * T + t] = acc;
-    } else if (acc > bv || (acc == bv && ko < bk)) {
+    } else if (acc > bv) {
       bv = acc;
       bk = ko;
     }

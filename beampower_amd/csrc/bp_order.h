@@ -17,11 +17,14 @@
 // multiple of the tile size and ends at one or at K, so every tile but the
 // last is whole, and each leaf is one tile.  Ties in the split key are
 // broken by the original index, so the order is a deterministic function
-// of the delays.
+// of the delays.  Within a tile, positions are in ascending original index:
+// the kernels scan a tile's sources in position order with a strict '>',
+// which then keeps the lowest original index among equal values, as the
+// reference's ascending scan does.
 //
 // The order only changes which sources share a tile.  Each source's beam
-// is the same chain of FMAs in any order, and the kernels break max ties on
-// the original index, so results do not depend on it.
+// is the same chain of FMAs in any order, and across tiles the kernels
+// break max ties on the original index, so results do not depend on it.
 #pragma once
 
 #include <algorithm>
@@ -47,7 +50,10 @@ inline std::vector<int> similarity_order(const int *tau, size_t K, size_t D, siz
     const size_t b = todo.back().first, e = todo.back().second;
     todo.pop_back();
     const size_t n = e - b;
-    if (n <= tile) continue;
+    if (n <= tile) {
+      std::sort(idx.begin() + (std::ptrdiff_t)b, idx.begin() + (std::ptrdiff_t)e);
+      continue;
+    }
     std::fill(lo.begin(), lo.end(), INT_MAX);
     std::fill(hi.begin(), hi.end(), INT_MIN);
     const size_t step = n > kSample ? n / kSample : 1;
@@ -81,31 +87,6 @@ inline std::vector<int> similarity_order(const int *tau, size_t K, size_t D, siz
     todo.push_back({b, b + nl});
   }
   return idx;
-}
-
-// Largest per-entry delay spread of each tile of `order` (the quantity the
-// plan classifies tiles by), over all entries: used to compare orders
-// before the device plan is built.
-inline std::vector<int> tile_spreads(const int *tau, size_t K, size_t D, size_t tile,
-                                     const std::vector<int> *order) {
-  const size_t nt = (K + tile - 1) / tile;
-  std::vector<int> out(nt, 0), lo(D), hi(D);
-  for (size_t j = 0; j < nt; ++j) {
-    std::fill(lo.begin(), lo.end(), INT_MAX);
-    std::fill(hi.begin(), hi.end(), INT_MIN);
-    const size_t e = std::min(K, (j + 1) * tile);
-    for (size_t i = j * tile; i < e; ++i) {
-      const int *row = tau + (size_t)(order ? (*order)[i] : (int)i) * D;
-      for (size_t d = 0; d < D; ++d) {
-        lo[d] = std::min(lo[d], row[d]);
-        hi[d] = std::max(hi[d], row[d]);
-      }
-    }
-    long long w = 0;
-    for (size_t d = 0; d < D; ++d) w = std::max(w, (long long)hi[d] - lo[d]);
-    out[j] = (int)std::min<long long>(w, INT_MAX);
-  }
-  return out;
 }
 
 }  // namespace bp
