@@ -639,11 +639,15 @@ struct StackLds {
     const int nrp = (Lw + 63) / 64;
     const int nw = Lw / SPA;
     rp.n = (nrp - wave + W - 1) / W;
+    // c = u / nw for u < Lw <= 2048 without an integer division:
+    // (u + 1/2) / nw lies at least 1/(2 nw) >= 2^-11 away from an integer,
+    // far beyond the float rounding of the reciprocal and the product
+    const float inv_nw = __builtin_amdgcn_rcpf((float)nw);
 #pragma unroll
     for (int i = 0; i < NRMAX; ++i) {
       const int j = wave + i * W;
       const int u = min(64 * j + lane, Lw - 1);
-      const int c = u / nw, m = u - c * nw;
+      const int c = (int)(((float)u + 0.5f) * inv_nw), m = u - c * nw;
       if (ES == 4) {
         rp.voff[i] = 4u * (unsigned)(c + SPA * m);
       } else {
@@ -941,22 +945,24 @@ struct StackLds {
         bv[j] = -FLT_MAX;
         bk[j] = 0;
       }
-      // Candidates in position order, which within a tile is ascending
-      // original index: with a strict '>' every lane keeps, of its sources,
-      // the largest value and among equal values the lowest original index
-      // (the reference's ascending scan).
+      // The common case, tested once per wave: every source of the wave
+      // exists, is written, and is valid over the whole time tile.  Then the
+      // running max needs no per-source or per-sample test.
+      bool whole = kbase + KW <= a.K && a.k_begin == 0 && a.k_end == a.K;
+      if (whole && !a.oob) {
 #pragma unroll
-      for (int kk = 0; kk < KW; ++kk) {
-        const int k = kbase + kk;
-        if (k >= a.K) continue;
-        const int ko = source_index(a, k);
-        if (ko < a.k_begin || ko >= a.k_end) continue;
-        int vlo = 0, vhi = a.T;
-        if (!a.oob)
+        for (int kk = 0; kk < KW; ++kk) {
+          const int k = kbase + kk;
+          int vlo, vhi;
           strict_range(as_const(a.src_lohi)[2 * k], as_const(a.src_lohi)[2 * k + 1], a.T, vlo,
                        vhi);
-        if (vlo <= t0 && t0 + TT <= vhi) {
-          // every sample of the tile is valid for source k (wave-uniform)
+          whole = whole && vlo <= t0 && t0 + TT <= vhi;
+        }
+      }
+      if (whole) {
+#pragma unroll
+        for (int kk = 0; kk < KW; ++kk) {
+          const int ko = source_index(a, kbase + kk);
 #pragma unroll
           for (int j = 0; j < NACC; ++j) {
             const float v = acc[kk][j];
@@ -965,14 +971,41 @@ struct StackLds {
               bk[j] = ko;
             }
           }
-        } else {
+        }
+      } else {
+        // Candidates in position order, which within a tile is ascending
+        // original index: with a strict '>' every lane keeps, of its sources,
+        // the largest value and among equal values the lowest original index
+        // (the reference's ascending scan).
 #pragma unroll
-          for (int j = 0; j < NACC; ++j) {
-            const int t = sample_of(t0, lane, j);
-            const float v = acc[kk][j];
-            if (t >= vlo && t < vhi && v > bv[j]) {
-              bv[j] = v;
-              bk[j] = ko;
+        for (int kk = 0; kk < KW; ++kk) {
+          const int k = kbase + kk;
+          if (k >= a.K) continue;
+          const int ko = source_index(a, k);
+          if (ko < a.k_begin || ko >= a.k_end) continue;
+          int vlo = 0, vhi = a.T;
+          if (!a.oob)
+            strict_range(as_const(a.src_lohi)[2 * k], as_const(a.src_lohi)[2 * k + 1], a.T, vlo,
+                         vhi);
+          if (vlo <= t0 && t0 + TT <= vhi) {
+            // every sample of the tile is valid for source k (wave-uniform)
+#pragma unroll
+            for (int j = 0; j < NACC; ++j) {
+              const float v = acc[kk][j];
+              if (v > bv[j]) {
+                bv[j] = v;
+                bk[j] = ko;
+              }
+            }
+          } else {
+#pragma unroll
+            for (int j = 0; j < NACC; ++j) {
+              const int t = sample_of(t0, lane, j);
+              const float v = acc[kk][j];
+              if (t >= vlo && t < vhi && v > bv[j]) {
+                bv[j] = v;
+                bk[j] = ko;
+              }
             }
           }
         }
