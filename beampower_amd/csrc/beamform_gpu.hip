@@ -56,9 +56,12 @@ int fail(int code, const char *fmt, ...) {
   } while (0)
 
 // Kernel configuration of the LDS shift-and-stack kernel (overridable at
-// build time for tuning sweeps).
+// build time for tuning sweeps).  6 sources per wave keep the accumulators
+// (6 x 8 FP32 per lane) within the 80 VGPRs of 6 waves per SIMD; on the
+// 50-station, 192k-source, 1 h workload 5 and 4 sources per wave measured
+// 8 % and 17 % slower (more staging per FMA).
 #ifndef BP_KW
-#define BP_KW 8
+#define BP_KW 6
 #endif
 #ifndef BP_W
 #define BP_W 8
@@ -74,6 +77,9 @@ constexpr int kNumCU = 256;        // MI355X
 constexpr int kTimeGroup = 8;      // block order: groups of 8 time tiles (bp_kernels.h, block_tile)
 constexpr double kSmallShare = 0.97;  // share of tiles the small window must cover
 constexpr int kDirectChunk = kTileSources;
+// Work-groups per launch of the LDS kernels: a dispatch's grid holds fewer
+// than 2^32 work-items; a multiple of 8 keeps the XCD ranges whole.
+constexpr long long kMaxLaunchBlocks = ((0xffffffffLL / (64 * kW)) / 8) * 8;
 
 template <int FMT, int P, int NB, bool NONE>
 using Lds = bp::StackLds<FMT, P, kKW, kW, NB, NONE>;
@@ -309,18 +315,27 @@ int ensure_schedule(bp_network *net, int fmt, const float *scales, hipStream_t s
 #ifndef BP_STAGE_BUFFERS
 #define BP_STAGE_BUFFERS 2
 #endif
+// `blocks` work-groups in launches of at most kMaxLaunchBlocks each.
+template <int FMT, int P, int NB, bool NONE>
+void launch_lds_chunks(bp::StackArgs a, long long blocks, hipStream_t st) {
+  for (long long b0 = 0; b0 < blocks; b0 += kMaxLaunchBlocks) {
+    a.block_base = (int)b0;
+    const unsigned n = (unsigned)std::min(kMaxLaunchBlocks, blocks - b0);
+    hipLaunchKernelGGL((bp::stack_lds_kernel<FMT, P, kKW, kW, NB, NONE>), dim3(n),
+                       dim3(64 * kW), (Lds<FMT, P, NB, NONE>::lds_bytes(a.Lw)), st, a);
+  }
+}
+
 template <int FMT, int P, bool NONE>
-void launch_lds_p(const bp::StackArgs &a, unsigned blocks, hipStream_t st) {
+void launch_lds_p(const bp::StackArgs &a, long long blocks, hipStream_t st) {
   if (BP_STAGE_BUFFERS == 3 && Lds<FMT, P, 3, NONE>::lds_bytes(a.Lw) <= kLdsPerCU / 2)
-    hipLaunchKernelGGL((bp::stack_lds_kernel<FMT, P, kKW, kW, 3, NONE>), dim3(blocks),
-                       dim3(64 * kW), (Lds<FMT, P, 3, NONE>::lds_bytes(a.Lw)), st, a);
+    launch_lds_chunks<FMT, P, 3, NONE>(a, blocks, st);
   else
-    hipLaunchKernelGGL((bp::stack_lds_kernel<FMT, P, kKW, kW, 2, NONE>), dim3(blocks),
-                       dim3(64 * kW), (Lds<FMT, P, 2, NONE>::lds_bytes(a.Lw)), st, a);
+    launch_lds_chunks<FMT, P, 2, NONE>(a, blocks, st);
 }
 
 template <int FMT, bool NONE>
-void launch_lds(int P, const bp::StackArgs &a, unsigned blocks, hipStream_t st) {
+void launch_lds(int P, const bp::StackArgs &a, long long blocks, hipStream_t st) {
   if (P == 1)
     launch_lds_p<FMT, 1, NONE>(a, blocks, st);
   else
@@ -368,7 +383,7 @@ int launch_stack(const bp_network *net, bp::StackArgs a, int k_begin, int k_end,
       a.n_time_tiles = (a.T + kTT - 1) / kTT;
       const long long blocks = (long long)a.n_time_tiles * n;
       if (blocks > INT32_MAX) return fail(BP_EINVAL, "grid too large (%lld blocks)", blocks);
-      launch_lds<FMT, NONE>(net->P, a, (unsigned)blocks, st);
+      launch_lds<FMT, NONE>(net->P, a, blocks, st);
     } else {
       if (n > 65535) return fail(BP_EINVAL, "too many direct-kernel tiles (%d)", n);
       const unsigned gx = (unsigned)((a.T + 255) / 256);
@@ -757,11 +772,9 @@ int launch_window_alone(const bp::StackArgs &a, int nb, hipStream_t st) {
   const long long blocks = (long long)a.n_time_tiles * a.n_list;
   if (blocks > INT32_MAX) return fail(BP_EINVAL, "grid too large (%lld blocks)", blocks);
   if (nb == 3)
-    hipLaunchKernelGGL((bp::stack_lds_kernel<FMT, P, kKW, kW, 3, false>), dim3((unsigned)blocks),
-                       dim3(64 * kW), (Lds<FMT, P, 3, false>::lds_bytes(a.Lw)), st, a);
+    launch_lds_chunks<FMT, P, 3, false>(a, blocks, st);
   else
-    hipLaunchKernelGGL((bp::stack_lds_kernel<FMT, P, kKW, kW, 2, false>), dim3((unsigned)blocks),
-                       dim3(64 * kW), (Lds<FMT, P, 2, false>::lds_bytes(a.Lw)), st, a);
+    launch_lds_chunks<FMT, P, 2, false>(a, blocks, st);
   HIP_TRY(hipGetLastError());
   return BP_OK;
 }
@@ -782,8 +795,8 @@ int launch_fused(const FusedSet &f, int nb, int blocks_per_cu, hipStream_t st) {
     const int per_cu = std::max(1, std::min(blocks_per_cu, kLdsPerCU / lds));
     want = std::min((long long)kNumCU * per_cu, (f.n_items + 7) / 8 * 8);
   }
-  if (want > INT32_MAX) return fail(BP_EINVAL, "batch grid too large (%lld)", want);
-  const unsigned grid = (unsigned)std::max<long long>(8, want);
+  // the kernel walks its XCD's items in a loop: any grid covers them
+  const unsigned grid = (unsigned)std::max<long long>(8, std::min(want, kMaxLaunchBlocks));
   if (nb == 3)
     hipLaunchKernelGGL((bp::stack_batch_kernel<FMT, P, kKW, kW, 3>), dim3(grid), dim3(64 * kW),
                        lds, st, f.dwins.as<bp::BatchWindow>(), (int)f.wins.size(),
@@ -1507,8 +1520,9 @@ int beamform(float *waveform_features, int *time_delays, float *weights_sources,
   BP_TRY(dbeam.alloc(K * T * sizeof(float)));
   // invalid strict cells read back as 0, as the wrapper's zero-filled buffer
   const long long n = (long long)(K * T);
-  hipLaunchKernelGGL(bp::fill_f32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                     st.s, dbeam.as<float>(), n, 0.0f);
+  const unsigned fill_blocks = (unsigned)std::min<long long>(65536, (n + 255) / 256);
+  hipLaunchKernelGGL(bp::fill_f32_kernel, dim3(fill_blocks), dim3(256), 0, st.s,
+                     dbeam.as<float>(), n, 0.0f);
   HIP_TRY(hipGetLastError());
   bp::StackArgs a = make_args_view(net.get(), v, out_of_bounds);
   a.beam = dbeam.as<float>();

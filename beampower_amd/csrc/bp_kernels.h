@@ -33,6 +33,8 @@ struct StackArgs {
   int Lw;                 // LDS window length (samples), multiple of 8
   int n_time_tiles;
   int time_group;         // block order: time tiles per group (>= 1)
+  int block_base;         // first block of this launch in the launch order (a class's blocks
+                          // may need several launches: see run())
   int k_begin, k_end;     // sources written, by original index: [k_begin, k_end)
   unsigned long long *keys;  // reduce=max: (T) packed keys of (value, original index)
   float *beam;               // reduce=none: beam[(original index - k_begin) * T + t]
@@ -393,9 +395,12 @@ __global__ void fill_u64_kernel(unsigned long long *__restrict__ p, long long n,
   if (i < n) p[i] = v;
 }
 
+// Grid-stride: an output of K x T floats can hold more elements than a
+// dispatch has work-items (2^32).
 __global__ void fill_f32_kernel(float *__restrict__ p, long long n, float v) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
 }
 
 // keys -> (beam_max, beam_argmax): beam_max written only where a candidate
@@ -464,8 +469,11 @@ __device__ __forceinline__ void block_tile(int L, int n_time_tiles, int n_list, 
   tt = g * TG + (rem - li * gsize);
 }
 
+// (source, phase) gather steps in flight.  One keeps the registers within
+// the 6-waves-per-SIMD budget below (two spill there, three ran 1.5 % slower
+// at 4 waves per SIMD).
 #ifndef BP_PIPE_DEPTH
-#define BP_PIPE_DEPTH 2  // (source, phase) gather steps in flight
+#define BP_PIPE_DEPTH 1
 #endif
 
 // Work-group barrier without the memory-model fence of __syncthreads(),
@@ -1022,19 +1030,23 @@ struct StackLds {
   }
 
   // One launch over the listed tiles of one network: block -> (time tile,
-  // listed tile) with XCD and L2 locality.
+  // listed tile) with XCD and L2 locality.  A dispatch holds fewer than 2^32
+  // work-items, so a class of more blocks than that allows runs as several
+  // launches of consecutive block ranges [block_base, block_base + grid).
   __device__ __forceinline__ static void run(const StackArgs &a, unsigned char *smem) {
-    const int L = xcd_remap(blockIdx.x, a.n_time_tiles * a.n_list);
+    const int L = a.block_base + xcd_remap(blockIdx.x, gridDim.x);
     int tt, li;
     block_tile(L, a.n_time_tiles, a.n_list, a.time_group, tt, li);
     tile_job(a, tt, as_const(a.tiles)[li], smem);
   }
 };
 
-// At least 4 waves per SIMD (VGPRs <= 128): two work-groups per CU, so one
-// computes while the other waits at its per-station barrier.
+// 6 waves per SIMD (VGPRs <= 80): three work-groups of 8 waves per CU when
+// their LDS fits (FP32 windows up to 704 samples), so two compute while one
+// waits at its per-station barrier.  With 6 sources per wave this measured
+// 6.9 % faster than 8 sources per wave at 4 waves per SIMD (two work-groups).
 #ifndef BP_WAVES_PER_EU
-#define BP_WAVES_PER_EU 4
+#define BP_WAVES_PER_EU 6
 #endif
 template <int FMT, int P, int KW, int W, int NB, bool NONE>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(BP_WAVES_PER_EU)))
