@@ -131,6 +131,10 @@ def poisson_run(server, requests, rate, seed=0):
     span = max(t.info["done_s"] for t in tickets) - min(t.info["submit_s"] for t in tickets)
     samples = sum(t.features.shape[2] for t in tickets)
     work = sum(server.networks[t.network][3] * t.features.shape[2] for t in tickets)
+    # gather-FMAs (source * station * phase * sample): work comparable across networks
+    # of different station counts
+    fmas = sum(np.prod(np.array(server.networks[t.network])[[0, 2, 3]], dtype=np.float64)
+               * t.features.shape[2] for t in tickets)
     batch = np.array([t.info["batch_windows"] for t in tickets])
     summary = {
         "rate_offered": rate, "requests": len(requests),
@@ -139,6 +143,7 @@ def poisson_run(server, requests, rate, seed=0):
         "mean_ms": float(lat.mean() * 1e3),
         "throughput_req_s": len(requests) / span,
         "source_samples_per_s": work / span,
+        "gather_fma_per_s": fmas / span,
         "samples": samples,
         "mean_batch_windows": float(batch.mean()),
     }

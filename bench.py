@@ -129,7 +129,7 @@ def bench_batched(args, log):
 
     n = args.batch_windows
     log(f"batched form: generating {n} windows")
-    wins, work = [], 0
+    wins, work, fmas = [], 0, 0
     for i in range(n):
         w = synthetic.batched_c4_window(1000 + i)
         F = bpd.prestack_planar(torch.from_numpy(w.waveform_features).cuda(),
@@ -137,6 +137,7 @@ def bench_batched(args, log):
         net = bpd.Network(w.time_delays, w.weights_sources)
         K, S, P = w.time_delays.shape
         work += K * F.shape[2]
+        fmas += K * S * P * F.shape[2]
         wins.append((net, F, 0))
         del w
     planar = wins
@@ -178,6 +179,7 @@ def bench_batched(args, log):
                and np.array_equal(b32.beam_argmax[i].cpu().numpy(), ref[i][1])
                for i in range(n))
     out["f32"] = {"ms": t_launch * 1e3, "graph_ms": t_graph * 1e3, "value": work / t_graph,
+                  "gather_fma_per_s": fmas / t_graph,
                   "windows_one_by_one_ms": t_alone * 1e3,
                   "identical_to_windows_alone": bool(same)}
     log(f"batched f32: {t_graph * 1e3:.1f} ms (graph), {t_alone * 1e3:.1f} ms one by one")
@@ -586,6 +588,7 @@ def main():
             "row_padding": list(pad),
             "source_order": info["source_order"],
         },
+        "gather_fma_per_s": value * S * P,
         "events_recovered": f"{recovered}/{len(w.events)}",
         "roofline": roofline,
         "dropin_host_pointers": dropin_host,

@@ -109,6 +109,17 @@ def test_c_abi_rejects_bad_arguments_without_a_device():
     assert lib.bp_detect_threshold(None, 10, 5, 0.66, 8.0, None, None) == -22
     n = ct.c_size_t(7)
     assert lib.bp_detect_peaks(None, None, 10, 1, None, 0, ct.byref(n), None) == -22
+    assert lib.bp_network_create_ordered(tau.ctypes.data, beta.ctypes.data, 2, 1, 2, 7,
+                                         ct.byref(out)) == -22
+    assert "order" in core.last_error(lib)
+    assert lib.bp_network_source_order(None, None, None) == -22
+
+
+def test_network_order_names_are_checked():
+    from beampower_amd import device as bpd
+    assert bpd.ORDERS == {"natural": 0, "similarity": 1, "auto": 2}
+    with pytest.raises(ValueError, match="order"):
+        bpd.Network(np.zeros((2, 1, 2), np.int32), np.ones((2, 1), np.float32), order="sorted")
 
 
 def test_degenerate_shapes_follow_reference_semantics():
