@@ -4,7 +4,11 @@
 
 Compiles the HIP sources under beampower_amd/csrc with hipcc for gfx950
 only.  Packed-FP32 instruction selection is disabled (-packed-fp32-ops).
+The stack kernel's compile-time shape is the sources' defaults unless
+beampower_amd/lib/build_flags.json (written by `python -m beampower_amd.tune
+--apply`) lists macro definitions to use instead.
 """
+import json
 import os
 import shutil
 import subprocess
@@ -27,6 +31,30 @@ def hipcc():
     return path
 
 
+FLAGS_PATH = os.path.join(LIB_DIR, "build_flags.json")
+# macros of the kernel shape the tuner may set (bp_kernels.h, beamform_gpu.hip)
+SHAPE_MACROS = ("BP_KW", "BP_W", "BP_WAVES_PER_EU", "BP_PIPE_DEPTH", "BP_STAGE_BUFFERS",
+                "BP_TIME_GROUP")
+
+
+def shape_flags(macros):
+    """-D flags of a {macro: int} kernel shape (unknown names are refused)."""
+    out = []
+    for name, value in sorted(macros.items()):
+        if name not in SHAPE_MACROS:
+            raise ValueError(f"{name} is not a kernel shape macro ({', '.join(SHAPE_MACROS)})")
+        out.append(f"-D{name}={int(value)}")
+    return out
+
+
+def tuned_flags():
+    """-D flags recorded by the tuner, or none."""
+    if not os.path.exists(FLAGS_PATH):
+        return []
+    with open(FLAGS_PATH) as f:
+        return shape_flags(json.load(f)["macros"])
+
+
 def command(out=LIB_PATH, extra=()):
     return [
         hipcc(), "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
@@ -40,7 +68,7 @@ def command(out=LIB_PATH, extra=()):
 def build(verbose=False):
     os.makedirs(LIB_DIR, exist_ok=True)
     tmp = LIB_PATH + ".tmp"
-    cmd = command(tmp)
+    cmd = command(tmp, tuned_flags())
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)

@@ -74,7 +74,10 @@ constexpr int kSmallLw = 1024;   // windows of the common tiles
 constexpr int kLargeLw = bp::kMaxWindow;  // largest window
 constexpr int kLdsPerCU = 160 * 1024;
 constexpr int kNumCU = 256;        // MI355X
-constexpr int kTimeGroup = 8;      // block order: groups of 8 time tiles (bp_kernels.h, block_tile)
+#ifndef BP_TIME_GROUP
+#define BP_TIME_GROUP 8
+#endif
+constexpr int kTimeGroup = BP_TIME_GROUP;  // block order: groups of time tiles (bp_kernels.h, block_tile)
 constexpr double kSmallShare = 0.97;  // share of tiles the small window must cover
 constexpr int kDirectChunk = kTileSources;
 // Work-groups per launch of the LDS kernels: a dispatch's grid holds fewer

@@ -1,0 +1,203 @@
+"""Tuner of the stack kernel's compile-time shape on a workload of one's own.
+
+    python -m beampower_amd.tune [--stations 50] [--grid 80 80 30] [--box-km 100]
+                                 [--samples 90000] [--fmt f32|bf16|f16] [--reps 6] [--apply]
+
+The shift-and-stack kernel keeps a tile's accumulators in registers, so its
+source tile (sources per wave x waves per work-group), the waves per SIMD it
+is compiled for (and with them how many work-groups share a CU), the gather
+pipeline depth, the staging buffers and the block-order time group are fixed
+at build time (macros of bp_kernels.h / beamform_gpu.hip).  This tool builds
+the library once per candidate shape into a scratch directory, loads every
+build into this process, plans the same network with each (the plan depends
+on the source tile), and times reduce='max' on the same device-resident,
+row-padded features with HIP events, the builds interleaved over `reps`
+rounds.  All builds must give bit-identical maxima and argmax.  --apply
+writes the fastest shape to beampower_amd/lib/build_flags.json, which
+`python -m beampower_amd.build` (and __graft_entry__.build()) then use.
+
+The run-time configuration of a batch (block order, staging buffers, grid)
+is tuned separately by bp_batch_tune.
+"""
+import argparse
+import concurrent.futures
+import ctypes as ct
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+from . import build as bp_build
+from .core import FeatureViewC
+
+# Candidate shapes; the first is the sources' default.
+CANDIDATES = (
+    {"BP_KW": 6, "BP_WAVES_PER_EU": 6, "BP_PIPE_DEPTH": 1},
+    {"BP_KW": 6, "BP_WAVES_PER_EU": 6, "BP_PIPE_DEPTH": 2},
+    {"BP_KW": 5, "BP_WAVES_PER_EU": 6, "BP_PIPE_DEPTH": 1},
+    {"BP_KW": 4, "BP_WAVES_PER_EU": 6, "BP_PIPE_DEPTH": 2},
+    {"BP_KW": 8, "BP_WAVES_PER_EU": 4, "BP_PIPE_DEPTH": 1},
+    {"BP_KW": 8, "BP_WAVES_PER_EU": 4, "BP_PIPE_DEPTH": 2},
+    {"BP_KW": 6, "BP_WAVES_PER_EU": 6, "BP_PIPE_DEPTH": 1, "BP_TIME_GROUP": 4},
+    {"BP_KW": 6, "BP_WAVES_PER_EU": 6, "BP_PIPE_DEPTH": 1, "BP_TIME_GROUP": 16},
+)
+FORMATS = {"f32": 0, "bf16": 1, "f16": 2}
+
+
+def name_of(macros):
+    return ",".join(f"{k[3:].lower()}={v}" for k, v in sorted(macros.items()))
+
+
+def build_all(candidates, out_dir, jobs=4, log=print):
+    """Build one library per candidate into out_dir; returns the paths."""
+    def one(i):
+        path = os.path.join(out_dir, f"beamform_gpu_{i}.so")
+        res = subprocess.run(bp_build.command(path, bp_build.shape_flags(candidates[i])),
+                             capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"build of {name_of(candidates[i])} failed:\n{res.stderr[-2000:]}")
+        return path
+    with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as ex:
+        paths = list(ex.map(one, range(len(candidates))))
+    log(f"built {len(paths)} candidates")
+    return paths
+
+
+class Candidate:
+    """One build loaded into this process, its network plan and buffers."""
+
+    def __init__(self, path, tau, beta, F, fmt):
+        import torch
+        vp, sz = ct.c_void_p, ct.c_size_t
+        lib = ct.CDLL(path)
+        lib.bp_last_error.restype = ct.c_char_p
+        lib.bp_network_create.argtypes = [vp, vp, sz, sz, sz, ct.POINTER(vp)]
+        lib.bp_network_destroy.argtypes = [vp]
+        lib.bp_network_padding.argtypes = [vp, ct.POINTER(sz), ct.POINTER(sz)]
+        lib.bp_pack_features.argtypes = [vp, sz, sz, sz, ct.c_int, sz, sz, vp, vp, vp]
+        lib.bp_network_beamform_max_view.argtypes = [vp, ct.POINTER(FeatureViewC), ct.c_int,
+                                                     vp, vp, vp, vp]
+        self.lib = lib
+        K, S, P = tau.shape
+        T = F.shape[2]
+        self.h = vp()
+        self._check(lib.bp_network_create(tau.ctypes.data, beta.ctypes.data, K, S, P,
+                                          ct.byref(self.h)))
+        pb, pa = sz(), sz()
+        self._check(lib.bp_network_padding(self.h, ct.byref(pb), ct.byref(pa)))
+        before = pb.value + (pb.value & 1 if fmt else 0)
+        stride = before + T + pa.value
+        stride += stride & 1 if fmt else 0
+        self.scale = np.ones(S, np.float32)
+        if fmt == 0:
+            self.buf = torch.zeros(S * P * stride, dtype=torch.float32, device=F.device)
+        else:
+            self.buf = torch.zeros(S * P * 2 * stride, dtype=torch.int16, device=F.device)
+        self._check(lib.bp_pack_features(F.data_ptr(), S, P, T, fmt, before, stride,
+                                         self.scale.ctypes.data if fmt else None,
+                                         self.buf.data_ptr(), None))
+        v = FeatureViewC()
+        v.data, v.fmt, v.n_samples = self.buf.data_ptr(), fmt, T
+        v.row_stride, v.pad_before = stride, before
+        v.station_scale = self.scale.ctypes.data if fmt else None
+        self.view = v
+        self.bmax = torch.zeros(T, dtype=torch.float32, device=F.device)
+        self.barg = torch.empty(T, dtype=torch.int32, device=F.device)
+        self.keys = torch.empty(T, dtype=torch.int64, device=F.device)
+        torch.cuda.synchronize()
+
+    def _check(self, rc):
+        if rc != 0:
+            raise RuntimeError(self.lib.bp_last_error().decode())
+
+    def run(self):
+        self._check(self.lib.bp_network_beamform_max_view(
+            self.h, ct.byref(self.view), 0, self.bmax.data_ptr(), self.barg.data_ptr(),
+            self.keys.data_ptr(), None))
+
+    def close(self):
+        if self.h:
+            self.lib.bp_network_destroy(self.h)
+            self.h = None
+
+
+def time_all(cands, reps, calls=3):
+    """Median milliseconds per call of each candidate, interleaved rounds."""
+    import torch
+    for c in cands:
+        c.run()
+    torch.cuda.synchronize()
+    times = [[] for _ in cands]
+    for _ in range(reps):
+        for i, c in enumerate(cands):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(calls):
+                c.run()
+            e1.record()
+            torch.cuda.synchronize()
+            times[i].append(e0.elapsed_time(e1) / calls)
+    return [float(np.median(t)) for t in times]
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--stations", type=int, default=50)
+    ap.add_argument("--grid", type=int, nargs=3, default=(80, 80, 30))
+    ap.add_argument("--box-km", type=float, default=100.0)
+    ap.add_argument("--samples", type=int, default=90_000)
+    ap.add_argument("--fmt", choices=sorted(FORMATS), default="f32")
+    ap.add_argument("--reps", type=int, default=6)
+    ap.add_argument("--jobs", type=int, default=4, help="parallel builds")
+    ap.add_argument("--apply", action="store_true", help="record the fastest shape for build()")
+    args = ap.parse_args(argv)
+    import torch
+    from . import synthetic
+    if not torch.cuda.is_available():
+        sys.exit("the tuner times builds on the GPU: no GPU is visible")
+    log = lambda m: print(f"[tune] {m}", flush=True)
+    _, _, tau = synthetic.make_network(2, args.stations, args.box_km, tuple(args.grid))
+    beta = np.ones(tau.shape[:2], np.float32)
+    rng = np.random.default_rng(2)
+    S, P = args.stations, tau.shape[2]
+    u = torch.from_numpy(synthetic.boxcar_envelopes(rng, S, 3, args.samples)).cuda()
+    alpha = torch.from_numpy(synthetic.tutorial_phase_weights(S)).cuda()
+    F = (u[:, :, None, :] * alpha[:, :, :, None]).sum(1).contiguous()  # (S, P, T)
+    del u
+    log(f"workload: {tau.shape[0]} sources, {S} stations, {args.samples} samples, {args.fmt}")
+    with tempfile.TemporaryDirectory(prefix="bp_tune_") as d:
+        paths = build_all(CANDIDATES, d, args.jobs, log)
+        cands = []
+        for p, m in zip(paths, CANDIDATES):
+            cands.append(Candidate(p, tau, beta, F, FORMATS[args.fmt]))
+            log(f"planned {name_of(m)}")
+        ms = time_all(cands, args.reps)
+        ref = (cands[0].bmax.cpu().numpy().view(np.uint32), cands[0].barg.cpu().numpy())
+        same = [np.array_equal(c.bmax.cpu().numpy().view(np.uint32), ref[0])
+                and np.array_equal(c.barg.cpu().numpy(), ref[1]) for c in cands]
+        for c in cands:
+            c.close()
+    work = tau.shape[0] * args.samples
+    rows = sorted(zip(ms, CANDIDATES, same), key=lambda r: r[0])
+    for t, m, ok in rows:
+        print(f"{name_of(m):48s} {t:9.3f} ms  {work / (t * 1e-3):.3e} source·samples/s"
+              f"{'' if ok else '  RESULTS DIFFER'}", flush=True)
+    if not all(same):
+        sys.exit("builds disagree: not applying")
+    best = rows[0][1]
+    if args.apply:
+        os.makedirs(bp_build.LIB_DIR, exist_ok=True)
+        with open(bp_build.FLAGS_PATH, "w") as f:
+            json.dump({"macros": best, "ms": rows[0][0],
+                       "workload": {"stations": S, "grid": list(args.grid),
+                                    "samples": args.samples, "fmt": args.fmt}}, f, indent=1)
+        log(f"wrote {bp_build.FLAGS_PATH}: rebuild with `python -m beampower_amd.build`")
+    return rows
+
+
+if __name__ == "__main__":
+    main()

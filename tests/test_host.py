@@ -165,3 +165,25 @@ def test_bench_refuses_more_than_one_gpu():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_tuner_candidates_and_build_flags():
+    """The tuner's first candidate is the sources' default shape, every
+    candidate differs, and build() turns a recorded shape into -D flags."""
+    from beampower_amd import build as bp_build, tune
+    src = ""
+    for f in ("bp_kernels.h", "beamform_gpu.hip"):
+        with open(os.path.join(bp_build.CSRC, f)) as fh:
+            src += fh.read()
+    defaults = {m: int(v) for m, v in re.findall(r"#define (BP_[A-Z_]+) (\d+)\b", src)
+                if m in bp_build.SHAPE_MACROS}
+    assert set(defaults) == set(bp_build.SHAPE_MACROS)
+    first = tune.CANDIDATES[0]
+    assert all(defaults[m] == v for m, v in first.items())
+    keys = [tuple(sorted(c.items())) for c in tune.CANDIDATES]
+    assert len(set(keys)) == len(keys)
+    flags = bp_build.shape_flags({"BP_KW": 5, "BP_PIPE_DEPTH": 2})
+    assert flags == ["-DBP_KW=5", "-DBP_PIPE_DEPTH=2"]
+    assert all(f in bp_build.command("x.so", flags) for f in flags)
+    with pytest.raises(ValueError):
+        bp_build.shape_flags({"BP_ANYTHING": 1})
