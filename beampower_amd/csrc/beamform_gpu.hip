@@ -313,7 +313,8 @@ int ensure_schedule(bp_network *net, int fmt, const float *scales, hipStream_t s
 
 // Staging buffers of the LDS kernel: 2 (copies of station s + 1 in flight
 // while station s is consumed) measured faster than 3 on the 50-station,
-// 192k-source, 1 h workload (91.8 vs 97.7 ms); 3 is taken only when asked
+// 192k-source, 1 h workload (66.2 against 85.6 ms: three buffers leave room
+// for two work-groups per CU instead of three); 3 is taken only when asked
 // for at build time and two work-groups still fit a CU's LDS.
 #ifndef BP_STAGE_BUFFERS
 #define BP_STAGE_BUFFERS 2
