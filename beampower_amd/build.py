@@ -34,7 +34,7 @@ def hipcc():
 FLAGS_PATH = os.path.join(LIB_DIR, "build_flags.json")
 # macros of the kernel shape the tuner may set (bp_kernels.h, beamform_gpu.hip)
 SHAPE_MACROS = ("BP_KW", "BP_W", "BP_WAVES_PER_EU", "BP_PIPE_DEPTH", "BP_STAGE_BUFFERS",
-                "BP_TIME_GROUP")
+                "BP_TIME_GROUP", "BP_NACC", "BP_SMALL_WINDOW", "BP_SMALL_SHARE_PCT")
 
 
 def shape_flags(macros):

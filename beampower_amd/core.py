@@ -53,7 +53,7 @@ GPU_SYMBOLS = (
     "beamform", "beamform_max", "prestack_waveform_features", "beamform_differential",
     "bp_last_error", "bp_version",
     "bp_network_create", "bp_network_destroy", "bp_network_info", "bp_network_classes",
-    "bp_network_create_ordered", "bp_network_source_order",
+    "bp_network_create_ordered", "bp_network_source_order", "bp_kernel_shape",
     "bp_network_beamform_max", "bp_network_beamform",
     "bp_interleaved_to_planar", "bp_prestack_planar",
     "bp_parity_stride", "bp_quantize_planar",
@@ -115,6 +115,7 @@ def _declare_gpu_extension(lib):
     lib.bp_network_create_ordered.argtypes = [vp, vp, _SZ, _SZ, _SZ, ct.c_int,
                                               ct.POINTER(vp)]
     lib.bp_network_source_order.argtypes = [vp, _P_I, vp]
+    lib.bp_kernel_shape.argtypes = [_P_I, _P_I, _P_I]
     lib.bp_network_info.argtypes = [vp, _P_I, _P_I, _P_I]
     lib.bp_network_classes.argtypes = [vp, _P_I, _P_I]
     lib.bp_network_beamform_max.argtypes = [vp, vp, ct.c_int, vp, _SZ, ct.c_int, vp, vp, vp,

@@ -68,9 +68,18 @@ int fail(int code, const char *fmt, ...) {
 #endif
 constexpr int kKW = BP_KW;  // sources per wave
 constexpr int kW = BP_W;    // waves per work-group -> source tile KW*W
-constexpr int kTT = 512;    // time tile (samples)
+constexpr int kTT = bp::kTimeTile;  // time tile (samples)
 constexpr int kTileSources = kKW * kW;
-constexpr int kSmallLw = 1024;   // windows of the common tiles
+// Window classes: the small window covers BP_SMALL_SHARE_PCT % of the tiles
+// (or as many as fit BP_SMALL_WINDOW samples), the large one the rest up to
+// kMaxWindow; wider tiles go to the direct kernel.
+#ifndef BP_SMALL_WINDOW
+#define BP_SMALL_WINDOW (bp::kTimeTile + 512)
+#endif
+#ifndef BP_SMALL_SHARE_PCT
+#define BP_SMALL_SHARE_PCT 97
+#endif
+constexpr int kSmallLw = BP_SMALL_WINDOW;  // largest window of the common tiles
 constexpr int kLargeLw = bp::kMaxWindow;  // largest window
 constexpr int kLdsPerCU = 160 * 1024;
 constexpr int kNumCU = 256;        // MI355X
@@ -78,7 +87,7 @@ constexpr int kNumCU = 256;        // MI355X
 #define BP_TIME_GROUP 8
 #endif
 constexpr int kTimeGroup = BP_TIME_GROUP;  // block order: groups of time tiles (bp_kernels.h, block_tile)
-constexpr double kSmallShare = 0.97;  // share of tiles the small window must cover
+constexpr double kSmallShare = BP_SMALL_SHARE_PCT / 100.0;  // tiles the small window covers
 constexpr int kDirectChunk = kTileSources;
 // Work-groups per launch of the LDS kernels: a dispatch's grid holds fewer
 // than 2^32 work-items; a multiple of 8 keeps the XCD ranges whole.
@@ -1027,6 +1036,13 @@ int bp_network_source_order(const bp_network *net, int *permuted, int *order) {
   if (permuted) *permuted = net->permuted ? 1 : 0;
   if (order)
     for (int i = 0; i < net->K; ++i) order[i] = net->permuted ? net->order[i] : i;
+  return BP_OK;
+}
+
+int bp_kernel_shape(int *source_tile, int *time_tile, int *waves_per_simd) {
+  if (source_tile) *source_tile = kTileSources;
+  if (time_tile) *time_tile = kTT;
+  if (waves_per_simd) *waves_per_simd = BP_WAVES_PER_EU;
   return BP_OK;
 }
 

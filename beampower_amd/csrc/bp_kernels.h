@@ -157,6 +157,12 @@ __global__ void gather_rows_kernel(const unsigned *__restrict__ in, const int *_
   }
 }
 
+// Samples per lane and source of the LDS kernel (8 or 16): its time tile is
+// 64 * BP_NACC samples.
+#ifndef BP_NACC
+#define BP_NACC 8
+#endif
+constexpr int kTimeTile = 64 * BP_NACC;
 constexpr int kMaxWindow = 2048;  // largest LDS window (samples)
 __host__ __device__ __forceinline__ int window_slot_bytes(int Lw) { return (Lw + 63) / 64 * 1024; }
 
@@ -423,10 +429,11 @@ __global__ void finalize_max_kernel(const unsigned long long *__restrict__ keys,
 // Shift-and-stack, LDS-staged (the hot kernel).
 //
 // A work-group of W waves covers one source tile (KT = W*KW sources; wave w
-// owns KW of them) and one time tile of TT = 512 samples.  A feature sample
+// owns KW of them) and one time tile of TT = 64 * NACC samples (512 by
+// default: NACC = 8 accumulators per lane and source).  A feature sample
 // is ES bytes (4 for float32, 2 for the 16-bit storage formats), so one
 // 16-byte LDS access holds SPA = 16/ES consecutive samples.  Lane l owns
-// R = 8/SPA runs of SPA consecutive samples, starting at
+// R = NACC/SPA runs of SPA consecutive samples, starting at
 // t0 + SPA*l + 64*SPA*r, so a wave's gather of one (source, station, phase)
 // term is R ds_read_b128 of 64 consecutive 16-byte words: conflict-free, and
 // SPA samples per LDS lane-access, which is what the gfx950 LDS pipe needs
@@ -439,7 +446,7 @@ __global__ void finalize_max_kernel(const unsigned long long *__restrict__ keys,
 // LDS as SPA copies shifted by 0..SPA-1 samples, in a ring of NB buffers:
 // the global->LDS copies of station s + NB - 1 are in flight while station
 // s is consumed.  Positions outside [0, T) are staged as exact zeros.
-// Accumulators acc[KW][8] stay in registers for the whole tile.  The
+// Accumulators acc[KW][NACC] stay in registers for the whole tile.  The
 // per-(source, station) LDS offsets and weights come from the precomputed
 // schedule as wave-uniform scalar loads.
 // ---------------------------------------------------------------------------
@@ -606,12 +613,13 @@ struct StackLds {
   static constexpr int ES = F::ES;
   static constexpr int SPA = F::SPA;
   static constexpr int NC = SPA;          // shifted copies of a window
-  static constexpr int R = 8 / SPA;       // 16-byte words per lane per term
+  static constexpr int NACC = BP_NACC;    // accumulators per lane and source
+  static_assert(NACC == 8 || NACC == 16, "8 or 16 samples per lane");
+  static constexpr int R = NACC / SPA;    // 16-byte words per lane per term
   static constexpr int PL = ES == 4 ? 1 : 2;  // feature planes per (s, p) in HBM
-  static constexpr int TT = 512;
+  static constexpr int TT = kTimeTile;
   static constexpr int KT = KW * W;
   static constexpr int NT = 64 * W;
-  static constexpr int NACC = 8;
   static constexpr int ZERO_BYTES = 1024 * R;  // all-zero block read by unused terms
 
   // LDS of one buffer: P phase slots (window_slot_bytes each), then the

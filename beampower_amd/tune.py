@@ -4,10 +4,12 @@
                                  [--samples 90000] [--fmt f32|bf16|f16] [--reps 6] [--apply]
 
 The shift-and-stack kernel keeps a tile's accumulators in registers, so its
-source tile (sources per wave x waves per work-group), the waves per SIMD it
-is compiled for (and with them how many work-groups share a CU), the gather
-pipeline depth, the staging buffers and the block-order time group are fixed
-at build time (macros of bp_kernels.h / beamform_gpu.hip).  This tool builds
+source tile (sources per wave x waves per work-group), its time tile (64 x
+samples per lane), the waves per SIMD it is compiled for (and with them how
+many work-groups share a CU), the gather pipeline depth, the staging
+buffers, the block-order time group and the window classes (the small LDS
+window's share of the tiles) are fixed at build time (macros of
+bp_kernels.h / beamform_gpu.hip).  This tool builds
 the library once per candidate shape into a scratch directory, loads every
 build into this process, plans the same network with each (the plan depends
 on the source tile), and times reduce='max' on the same device-resident,
@@ -43,6 +45,12 @@ CANDIDATES = (
     {"BP_KW": 8, "BP_WAVES_PER_EU": 4, "BP_PIPE_DEPTH": 2},
     {"BP_KW": 6, "BP_WAVES_PER_EU": 6, "BP_PIPE_DEPTH": 1, "BP_TIME_GROUP": 4},
     {"BP_KW": 6, "BP_WAVES_PER_EU": 6, "BP_PIPE_DEPTH": 1, "BP_TIME_GROUP": 16},
+    # 1,024-sample time tiles (16 samples per lane): half the address adds per FMA
+    {"BP_KW": 4, "BP_NACC": 16, "BP_WAVES_PER_EU": 4, "BP_PIPE_DEPTH": 1},
+    {"BP_KW": 3, "BP_NACC": 16, "BP_WAVES_PER_EU": 4, "BP_PIPE_DEPTH": 1},
+    # window classes: the small window's share of the tiles
+    {"BP_KW": 6, "BP_WAVES_PER_EU": 6, "BP_PIPE_DEPTH": 1, "BP_SMALL_SHARE_PCT": 90},
+    {"BP_KW": 6, "BP_WAVES_PER_EU": 6, "BP_PIPE_DEPTH": 1, "BP_SMALL_SHARE_PCT": 100},
 )
 FORMATS = {"f32": 0, "bf16": 1, "f16": 2}
 

@@ -125,6 +125,10 @@ int bp_network_source_order(const bp_network *net, int *permuted, int *order);
 /* Largest per-(station, phase) delay spread inside one source tile. */
 int bp_network_info(const bp_network *net, int *source_tile, int *max_tile_spread,
                     int *n_tiles);
+/* Compile-time shape of the LDS kernel of this build (no counterpart in the
+ * reference): sources per tile, samples per time tile, and the waves per SIMD
+ * it is compiled for (beampower_amd.tune chooses among builds). */
+int bp_kernel_shape(int *source_tile, int *time_tile, int *waves_per_simd);
 /* Source tiles per kernel class (small LDS window, large LDS window,
  * direct gathers) and the LDS window length of each class (0 = direct). */
 int bp_network_classes(const bp_network *net, int *tiles_per_class, int *window_per_class);
