@@ -23,10 +23,13 @@ def test_class_split_over_several_launches():
         pytest.skip("no GPU")
     from beampower_amd import device as bpd
     rng = np.random.default_rng(77)
+    import ctypes as ct
+    from beampower_amd import load_library
+    tile_c, tt_c = ct.c_int(), ct.c_int()
+    assert load_library("gpu").bp_kernel_shape(ct.byref(tile_c), ct.byref(tt_c), None) == 0
+    tile, time_tile = tile_c.value, tt_c.value
     S, P, T, W = 2, 1, 1_100_000, 2000
-    n_tt = (T + 511) // 512
-    probe = bpd.Network(np.zeros((1, S, P), np.int32), np.ones((1, S), np.float32))
-    tile = probe.info()["source_tile"]
+    n_tt = (T + time_tile - 1) // time_tile
     n_tiles = DISPATCH_BLOCKS // n_tt + 100
     K = n_tiles * tile
     tau = rng.integers(0, 50, (K, S, P)).astype(np.int32)
