@@ -13,6 +13,7 @@ dominant shift-and-stack kernel by its algorithmic LDS gather bytes;
 and flags) on this host on a bounded time slice of the same workload.
 """
 import argparse
+import ctypes as ct
 import json
 import os
 import sys
@@ -445,6 +446,9 @@ def main():
     net = bpd.Network(w.time_delays, w.weights_sources)
     info = net.info()
     info["source_order"] = "similarity" if net.source_order()[0] else "natural"
+    shape = [ct.c_int() for _ in range(3)]
+    net.lib.bp_kernel_shape(*[ct.byref(x) for x in shape])
+    info["time_tile"], info["waves_per_simd"] = shape[1].value, shape[2].value
     # rows zero-padded for this network (as the drop-in entry points stage
     # them): every LDS tile takes the staged path
     pad = net.padding()
@@ -500,7 +504,6 @@ def main():
 
     # the host-pointer drop-in entry point on the same data: network plan,
     # host->device copies of F, tau and beta, the kernels and the copy back
-    import ctypes as ct
     from beampower_amd import load_library
     lib = load_library("gpu")
     F_host = np.ascontiguousarray(F_plain.permute(0, 2, 1).cpu().numpy())  # (S, T, P)
@@ -586,7 +589,8 @@ def main():
             "source_tile": info["source_tile"], "max_tile_spread": info["max_tile_spread"],
             "tiles_per_class": info["tiles_per_class"], "window_per_class": info["window_per_class"],
             "row_padding": list(pad),
-            "source_order": info["source_order"],
+            "source_order": info["source_order"], "time_tile": info["time_tile"],
+            "waves_per_simd": info["waves_per_simd"],
         },
         "gather_fma_per_s": value * S * P,
         "events_recovered": f"{recovered}/{len(w.events)}",
