@@ -224,8 +224,10 @@ struct bp_network {
   std::vector<int> tiles[3];  // ascending tile ids per class (host copy)
   DevBuf tau, beta, lohi, tile_lo, tile_all, tile_lw, dtiles[3];
   DevBuf sched[3], scale[3];
-  DevBuf diff_tau, diff_beta;  // differential mode: wave-transposed tables, built on first use
-  bool diff_ok = false;
+  // differential mode: wave-transposed tables in the caller's source order,
+  // each built on first use and kept: weights, int32 delays, paired int16 delays
+  DevBuf diff_tau, diff_beta, diff_tau_pairs;
+  bool diff_ok = false, diff_beta_ok = false, diff_pairs_ok = false;
   // source order: tau, beta and the plan are in position order; perm maps a
   // position to the caller's index (device and host), inv the reverse (host)
   bool permuted = false;
@@ -895,30 +897,51 @@ int batch_configure(bp_batch *b, const bp_batch_config &c) {
 // (K, S, P) delays and (K, S) weights on the device -> the wave-transposed
 // tables of the differential kernel (64-source blocks, entry-major), padded
 // to bp::diff_padded_stations(S, P) stations.
-int build_diff_tables(const int *tau, const float *beta, long long K, int S, int P,
-                      DevBuf &tauT, DevBuf &betaT, hipStream_t st) {
+// (K, E) row-major 4-byte table -> wave-transposed (ceil(K/64), E_pad, 64).
+int transpose_diff_table(const void *in, long long K, int E, int E_pad, DevBuf &out,
+                         hipStream_t st) {
   const long long nblk = (K + bp::kDiffBlock - 1) / bp::kDiffBlock;
-  const int S_pad = bp::diff_padded_stations(S, P);
-  const long long sp = (long long)S * P, sp_pad = (long long)S_pad * P;
-  BP_TRY(tauT.alloc((size_t)(nblk * sp_pad * bp::kDiffBlock) * sizeof(int)));
-  BP_TRY(betaT.alloc((size_t)(nblk * S_pad * bp::kDiffBlock) * sizeof(float)));
-  auto tiles = [](long long e) {
-    return (unsigned)std::min<long long>((e + bp::kDiffBlock - 1) / bp::kDiffBlock, 1024);
-  };
-  hipLaunchKernelGGL(bp::diff_tables_kernel, dim3((unsigned)nblk, tiles(sp_pad)), dim3(256), 0,
-                     st, reinterpret_cast<const unsigned *>(tau), K, (int)sp, (int)sp_pad,
-                     tauT.as<unsigned>());
-  HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(bp::diff_tables_kernel, dim3((unsigned)nblk, tiles(S_pad)), dim3(256), 0,
-                     st, reinterpret_cast<const unsigned *>(beta), K, S, S_pad,
-                     betaT.as<unsigned>());
+  BP_TRY(out.alloc((size_t)(nblk * E_pad * bp::kDiffBlock) * 4));
+  const unsigned tiles =
+      (unsigned)std::min<long long>((E_pad + bp::kDiffBlock - 1) / bp::kDiffBlock, 1024);
+  hipLaunchKernelGGL(bp::diff_tables_kernel, dim3((unsigned)nblk, tiles), dim3(256), 0, st,
+                     static_cast<const unsigned *>(in), K, E, E_pad, out.as<unsigned>());
   HIP_TRY(hipGetLastError());
   return BP_OK;
 }
 
+// Tables of the differential kernel: weights (ceil(K/64), S_pad, 64) float32
+// and delays either (ceil(K/64), S_pad * P, 64) int32 or, for two phases,
+// (ceil(K/64), S_pad, 64) words of two saturated int16 (`pairs`).
+int build_diff_tau(const int *tau, long long K, int S, int P, bool pairs, DevBuf &tauT,
+                   hipStream_t st) {
+  const int S_pad = bp::diff_padded_stations(S, P);
+  if (!pairs) return transpose_diff_table(tau, K, S * P, S_pad * P, tauT, st);
+  DevBuf wide;
+  BP_TRY(transpose_diff_table(tau, K, S * 2, S_pad * 2, wide, st));
+  const long long nblk = (K + bp::kDiffBlock - 1) / bp::kDiffBlock;
+  const long long n = nblk * S_pad * bp::kDiffBlock;
+  BP_TRY(tauT.alloc((size_t)n * 4));
+  hipLaunchKernelGGL(bp::diff_pack_pairs_kernel,
+                     dim3((unsigned)std::min<long long>(65536, (n + 255) / 256)), dim3(256), 0,
+                     st, wide.as<unsigned>(), nblk, S_pad, tauT.as<unsigned>());
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(st));  // before `wide` is freed
+  return BP_OK;
+}
+
+int build_diff_beta(const float *beta, long long K, int S, int P, DevBuf &betaT,
+                    hipStream_t st) {
+  return transpose_diff_table(beta, K, S, bp::diff_padded_stations(S, P), betaT, st);
+}
+
+// Two-phase delays fit the paired int16 tables for windows of at most 32768
+// samples (the saturated delays then stay outside the window).
+bool diff_pairs_ok(long long P, long long T) { return P == 2 && T <= 32768; }
+
 template <int FMT>
 void launch_differential_p(const bp::StackArgs &a, const int *tauT, const float *betaT,
-                           float *beam, hipStream_t st) {
+                           float *beam, bool pairs, hipStream_t st) {
   const dim3 grid((unsigned)((a.K + bp::kDiffSources - 1) / bp::kDiffSources));
   const dim3 block(bp::kDiffSources);
   const int S_pad = bp::diff_padded_stations(a.S, a.P);
@@ -928,8 +951,12 @@ void launch_differential_p(const bp::StackArgs &a, const int *tauT, const float 
                          S_pad, beam);
       break;
     case 2:
-      hipLaunchKernelGGL((bp::differential_kernel<FMT, 2>), grid, block, 0, st, a, tauT, betaT,
-                         S_pad, beam);
+      if (pairs)
+        hipLaunchKernelGGL((bp::differential_kernel<FMT, 2, true>), grid, block, 0, st, a, tauT,
+                           betaT, S_pad, beam);
+      else
+        hipLaunchKernelGGL((bp::differential_kernel<FMT, 2>), grid, block, 0, st, a, tauT,
+                           betaT, S_pad, beam);
       break;
     case 3:
       hipLaunchKernelGGL((bp::differential_kernel<FMT, 3>), grid, block, 0, st, a, tauT, betaT,
@@ -957,13 +984,14 @@ int check_differential(int fmt, long long S, long long P, long long ldf) {
 }
 
 int launch_differential(int fmt, const bp::StackArgs &a, const int *tauT, const float *betaT,
-                        float *beam, hipStream_t st) {
+                        float *beam, bool pairs, hipStream_t st) {
+  if (pairs && a.P != 2) return fail(BP_EINVAL, "paired delay tables need two phases");
   if (fmt == BP_F32)
-    launch_differential_p<BP_F32>(a, tauT, betaT, beam, st);
+    launch_differential_p<BP_F32>(a, tauT, betaT, beam, pairs, st);
   else if (fmt == BP_F16)
-    launch_differential_p<BP_F16>(a, tauT, betaT, beam, st);
+    launch_differential_p<BP_F16>(a, tauT, betaT, beam, pairs, st);
   else
-    launch_differential_p<BP_BF16>(a, tauT, betaT, beam, st);
+    launch_differential_p<BP_BF16>(a, tauT, betaT, beam, pairs, st);
   HIP_TRY(hipGetLastError());
   return BP_OK;
 }
@@ -1243,17 +1271,20 @@ int bp_network_beamform_differential(bp_network *net, const bp_feature_view *fea
   const int fmt = features->fmt;
   BP_TRY(check_differential(fmt, net->S, net->P, (long long)features->row_stride));
   if (fmt != BP_F32) BP_TRY(ensure_schedule(net, fmt, features->station_scale, st));
+  // The tables are in the caller's source order even when the stack plan
+  // orders sources by similarity: a wave's 64 lanes are 64 sources, and on
+  // a grid 64 consecutive indices gather from fewer cache lines than a
+  // similarity tile (0.51 against 0.59 ms at 1,225 pairs x 192,000 sources).
+  const bool pairs = diff_pairs_ok(net->P, (long long)features->n_samples);
   {
-    // the transposed tables, built on first use and kept with the network
-    // The tables are in the caller's source order even when the stack plan
-    // orders sources by similarity: a wave's 64 lanes are 64 sources, and on
-    // a grid 64 consecutive indices gather from fewer cache lines than a
-    // similarity tile (0.51 against 0.59 ms at 1,225 pairs x 192,000 sources).
     std::lock_guard<std::mutex> lock(net->mu);
-    if (!net->diff_ok) {
+    bool &tau_ok = pairs ? net->diff_pairs_ok : net->diff_ok;
+    if (!tau_ok || !net->diff_beta_ok) {
+      const size_t K = net->K, S = net->S, P = net->P;
+      const int *tau_src = net->tau.as<int>();
+      const float *beta_src = net->beta.as<float>();
+      DevBuf inv, tau, beta;
       if (net->permuted) {
-        const size_t K = net->K, S = net->S, P = net->P;
-        DevBuf inv, tau, beta;
         BP_TRY(inv.alloc(K * sizeof(int)));
         BP_TRY(tau.alloc(K * S * P * sizeof(int)));
         BP_TRY(beta.alloc(K * S * sizeof(float)));
@@ -1268,20 +1299,24 @@ int bp_network_beamform_differential(bp_network *net, const bp_feature_view *fea
                            net->beta.as<unsigned>(), inv.as<int>(), (long long)K, (int)S,
                            beta.as<unsigned>());
         HIP_TRY(hipGetLastError());
-        BP_TRY(build_diff_tables(tau.as<int>(), beta.as<float>(), net->K, net->S, net->P,
-                                 net->diff_tau, net->diff_beta, st));
-        HIP_TRY(hipStreamSynchronize(st));  // before the temporaries are freed
-      } else {
-        BP_TRY(build_diff_tables(net->tau.as<int>(), net->beta.as<float>(), net->K, net->S,
-                                 net->P, net->diff_tau, net->diff_beta, st));
+        tau_src = tau.as<int>();
+        beta_src = beta.as<float>();
       }
-      HIP_TRY(hipStreamSynchronize(st));  // ready for calls on any stream
-      net->diff_ok = true;
+      if (!net->diff_beta_ok) BP_TRY(build_diff_beta(beta_src, net->K, net->S, net->P,
+                                                     net->diff_beta, st));
+      if (!tau_ok)
+        BP_TRY(build_diff_tau(tau_src, net->K, net->S, net->P, pairs,
+                              pairs ? net->diff_tau_pairs : net->diff_tau, st));
+      // ready for calls on any stream, and before the temporaries are freed
+      HIP_TRY(hipStreamSynchronize(st));
+      net->diff_beta_ok = true;
+      tau_ok = true;
     }
   }
   bp::StackArgs a = make_args_view(net, *features, 0);
   a.perm = nullptr;  // tables in the caller's order
-  return launch_differential(fmt, a, net->diff_tau.as<int>(), net->diff_beta.as<float>(), beam,
+  const DevBuf &tauT = pairs ? net->diff_tau_pairs : net->diff_tau;
+  return launch_differential(fmt, a, tauT.as<int>(), net->diff_beta.as<float>(), beam, pairs,
                              st);
 }
 
@@ -1329,10 +1364,11 @@ int beamform_differential(float *waveform_features, int *time_delays, float *wei
   a.K = (int)K;
   a.S = (int)S;
   a.P = (int)P;
-  BP_TRY(build_diff_tables(tau.as<int>(), beta.as<float>(), (long long)K, (int)S, (int)P, tauT,
-                           betaT, st.s));
+  const bool pairs = diff_pairs_ok((long long)P, (long long)T);
+  BP_TRY(build_diff_beta(beta.as<float>(), (long long)K, (int)S, (int)P, betaT, st.s));
+  BP_TRY(build_diff_tau(tau.as<int>(), (long long)K, (int)S, (int)P, pairs, tauT, st.s));
   BP_TRY(launch_differential(BP_F32, a, tauT.as<int>(), betaT.as<float>(), dbeam.as<float>(),
-                             st.s));
+                             pairs, st.s));
   HIP_TRY(hipMemcpyAsync(beam, dbeam.p, K * sizeof(float), hipMemcpyDeviceToHost, st.s));
   HIP_TRY(hipStreamSynchronize(st.s));
   return BP_OK;

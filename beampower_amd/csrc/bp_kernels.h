@@ -1255,6 +1255,29 @@ __global__ __launch_bounds__(256) void diff_tables_kernel(const unsigned *__rest
   }
 }
 
+// Two phases' delays of a station as one word of int16 halves (phase 0 low,
+// phase 1 high), saturated to the int16 range, from wave-transposed int32
+// entries (ceil(K/64), 2 * S_pad, 64) into (ceil(K/64), S_pad, 64).  For
+// windows of at most 32768 samples (centre t <= 16383) a saturated delay
+// still points outside [0, T), and every delay inside it is exact, so the
+// kernel's terms and bounds are unchanged.
+__device__ __forceinline__ unsigned sat16(unsigned v) {
+  const int d = (int)v;
+  return (unsigned)(d < -32768 ? -32768 : d > 32767 ? 32767 : d) & 0xffffu;
+}
+
+__global__ void diff_pack_pairs_kernel(const unsigned *__restrict__ in, long long n_blk,
+                                       int S_pad, unsigned *__restrict__ out) {
+  const long long n = n_blk * S_pad * kDiffBlock;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long row = i / kDiffBlock, lane = i - row * kDiffBlock;  // row = b * S_pad + s
+    const long long b = row / S_pad, s = row - b * S_pad;
+    const long long src = ((b * S_pad + s) * 2) * kDiffBlock + lane;  // entry 2s of block b
+    out[i] = sat16(in[src]) | (sat16(in[src + kDiffBlock]) << 16);
+  }
+}
+
 // Stations per chunk of the differential kernel for P phases (1-4), and the
 // station count of its tables: S rounded up to two chunks, plus one chunk
 // that the last prefetch reads.  Padding stations have zero weights.
@@ -1285,10 +1308,15 @@ __device__ __forceinline__ float feature_row(const StackArgs &a, unsigned row, u
 // the chain is the reference's.  The bounds are private to each source
 // (the reference shares them between OpenMP threads,
 // beamform.c:353-354,370-380; this is its one-thread behaviour).
-template <int FMT, int PT>
+//
+// PAIRS (two phases only): the delays come as one word of two int16 per
+// (source, station) (diff_pack_pairs_kernel), one load instead of two and
+// 8 instead of 12 table bytes per (source, station) with the weight.
+template <int FMT, int PT, bool PAIRS = false>
 __global__ __launch_bounds__(kDiffSources) void differential_kernel(
     StackArgs a, const int *__restrict__ tauT, const float *__restrict__ betaT, int S_pad,
     float *__restrict__ beam) {
+  static_assert(!PAIRS || PT == 2, "paired delays are for two phases");
   constexpr int CS = diff_chunk_stations(PT);  // stations per chunk (PT > 0)
   constexpr int NE = PT > 0 ? CS * PT : 1;
   const int lane = threadIdx.x % kDiffBlock;
@@ -1299,7 +1327,7 @@ __global__ __launch_bounds__(kDiffSources) void differential_kernel(
   const int S = a.S, P = PT > 0 ? PT : a.P;
   const long long T = a.T, t = (T - 1) / 2;
   // 32-bit offsets inside the wave's block: 64 * S_pad * P < 2^30 (checked by the host)
-  const int *tb = tauT + blk * S_pad * P * kDiffBlock + lane;
+  const int *tb = tauT + blk * S_pad * (PAIRS ? 1 : P) * kDiffBlock + lane;
   const float *wb = betaT + blk * S_pad * kDiffBlock + lane;
   float acc = 0.0f;
   bool used = false, inside = true;
@@ -1342,9 +1370,15 @@ __global__ __launch_bounds__(kDiffSources) void differential_kernel(
 #pragma unroll
     for (int c = 0; c < CS; ++c) {
       ch.b[c] = wb[(unsigned)(s0 + c) * kDiffBlock];
+      if (PAIRS) {
+        const unsigned w = (unsigned)tb[(unsigned)(s0 + c) * kDiffBlock];
+        ch.d[c * PT] = (int)(short)(w & 0xffffu);
+        ch.d[c * PT + 1] = (int)(short)(w >> 16);
+      } else {
 #pragma unroll
-      for (int p = 0; p < PT; ++p)
-        ch.d[c * PT + p] = tb[(unsigned)((s0 + c) * PT + p) * kDiffBlock];
+        for (int p = 0; p < PT; ++p)
+          ch.d[c * PT + p] = tb[(unsigned)((s0 + c) * PT + p) * kDiffBlock];
+      }
     }
   };
   auto gather = [&](int s0, const Chunk &ch, float (&x)[NE]) {

@@ -347,7 +347,10 @@ def bench_differential(args, log):
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / n
     recovered = int(int(beam.argmax().item()) == w.events[0][0])
-    table_bytes = K * S * P * 4 + K * S * 4 + K * 4
+    # table bytes the kernel streams: two-phase delays of windows up to 32768
+    # samples as one word of two int16 per (source, station pair), else int32
+    pairs = P == 2 and T <= 32768
+    table_bytes = K * S * (4 if pairs else P * 4) + K * S * 4 + K * 4
     feature_bytes = S * P * T * 4
     achieved = (table_bytes + feature_bytes) / (ms * 1e-3)
     out = {
@@ -358,6 +361,7 @@ def bench_differential(args, log):
                      "unit": "GB/s", "frac": achieved / HBM_PEAK,
                      "frac_of_measured_stream": achieved / HBM_READ_MEASURED,
                      "kernel": "differential_kernel",
+                     "delay_table": "int16 pairs" if pairs else "int32",
                      "algorithmic_bytes_per_launch": table_bytes + feature_bytes},
     }
     log(f"differential: {ms:.3f} ms per step, {achieved / 1e9:.0f} GB/s")

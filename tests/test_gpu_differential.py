@@ -148,3 +148,33 @@ def test_c6_full_size_vs_oracle_and_direct(gpu_lib):
         k1 = min(K, k0 + 16000)
         d = net.beamform(Fd, 0, source_begin=k0, source_end=k1)[:, t].contiguous()
         assert bits_equal(d.cpu().numpy(), bh[k0:k1]), k0
+
+
+@pytest.mark.parametrize("T", [32767, 32768, 40001])
+def test_delay_table_widths_around_the_int16_limit(gpu_lib, T):
+    """Two-phase delays go in paired int16 tables for windows of at most
+    32,768 samples (saturated delays stay outside the window) and in int32
+    tables for longer windows.  Delays beyond the int16 range, and sources
+    with only in-range delays, both give the oracle's beams; one network
+    serves a short and a long window in turn."""
+    torch = _torch()
+    from beampower_amd import device as bpd
+    rng = np.random.default_rng(T)
+    S, P, K = 5, 2, 300
+    F = rng.standard_normal((S, T, P)).astype(np.float32)
+    c = (T - 1) // 2
+    tau = rng.integers(-c, T - 1 - c + 1, (K, S, P)).astype(np.int32)   # in range
+    tau[:40] = rng.integers(-70000, 70000, (40, S, P))                  # mostly out of range
+    tau[40:50, 0, 0] = 32767 + rng.integers(0, 3, 10)                   # around the int16 edge
+    tau[50:60, 1, 1] = -32768 - rng.integers(0, 3, 10)
+    beta = rng.uniform(0.5, 1.5, (K, S)).astype(np.float32)
+    beta[rng.random((K, S)) < 0.2] = 0.0
+    ref = oracle.beamform_differential(F, tau, beta, flavour="fma")
+    assert bits_equal(abi_differential(gpu_lib, F, tau, beta, np.zeros(K)), ref)
+    net = bpd.Network(tau, beta)
+    for Fw, refw in ((F[:, :2001], None), (F, ref)):
+        if refw is None:
+            refw = oracle.beamform_differential(np.ascontiguousarray(Fw), tau, beta,
+                                                flavour="fma")
+        Fd = torch.from_numpy(np.ascontiguousarray(Fw.transpose(0, 2, 1))).cuda()
+        assert bits_equal(net.beamform_differential(Fd).cpu().numpy(), refw)
