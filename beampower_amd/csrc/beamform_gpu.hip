@@ -183,7 +183,9 @@ hipError_t set_lds_attr() {
 template <int FMT, bool NONE>
 hipError_t set_lds_attrs() {
   hipError_t r[] = {set_lds_attr<FMT, 1, 2, NONE>(), set_lds_attr<FMT, 2, 2, NONE>(),
-                    set_lds_attr<FMT, 1, 3, NONE>(), set_lds_attr<FMT, 2, 3, NONE>()};
+                    set_lds_attr<FMT, 3, 2, NONE>(), set_lds_attr<FMT, 4, 2, NONE>(),
+                    set_lds_attr<FMT, 1, 3, NONE>(), set_lds_attr<FMT, 2, 3, NONE>(),
+                    set_lds_attr<FMT, 3, 3, NONE>(), set_lds_attr<FMT, 4, 3, NONE>()};
   for (hipError_t x : r)
     if (x != hipSuccess) return x;
   return hipSuccess;
@@ -351,10 +353,12 @@ void launch_lds_p(const bp::StackArgs &a, long long blocks, hipStream_t st) {
 
 template <int FMT, bool NONE>
 void launch_lds(int P, const bp::StackArgs &a, long long blocks, hipStream_t st) {
-  if (P == 1)
-    launch_lds_p<FMT, 1, NONE>(a, blocks, st);
-  else
-    launch_lds_p<FMT, 2, NONE>(a, blocks, st);
+  switch (P) {
+    case 1: launch_lds_p<FMT, 1, NONE>(a, blocks, st); break;
+    case 2: launch_lds_p<FMT, 2, NONE>(a, blocks, st); break;
+    case 3: launch_lds_p<FMT, 3, NONE>(a, blocks, st); break;
+    default: launch_lds_p<FMT, 4, NONE>(a, blocks, st);
+  }
 }
 
 // Launch the stack kernels for sources [k_begin, k_end): every class's
@@ -451,6 +455,26 @@ bp_feature_view plain_view(const void *F, int fmt, const float *scale, size_t T)
 // multiples of 8 samples (whole 16-byte words of 16-bit samples) of at
 // least TT + spread + 1 (the staging start of 16-bit samples is rounded
 // down to even, one sample before the least delay at most).
+// Largest LDS window of P phases whose staging buffers (float32, two
+// buffers, the reduce='max' layout: the largest of the kernel's variants)
+// fit one CU's LDS: 2,048 samples for one or two phases, less for more.
+template <int P>
+int lds_window_limit() {
+  int lw = kLargeLw;
+  while (lw > kTT && Lds<BP_F32, P, 2, false>::lds_bytes(lw) > kLdsPerCU) lw -= 64;
+  return lw;
+}
+
+int lds_window_limit(int P) {
+  switch (P) {
+    case 1: return lds_window_limit<1>();
+    case 2: return lds_window_limit<2>();
+    case 3: return lds_window_limit<3>();
+    case 4: return lds_window_limit<4>();
+    default: return 0;
+  }
+}
+
 void classify_tiles(bp_network *net, const std::vector<int> &tile_all) {
   const int n = net->n_tiles;
   std::vector<int> spreads;
@@ -459,13 +483,14 @@ void classify_tiles(bp_network *net, const std::vector<int> &tile_all) {
   std::vector<int> sorted = spreads;
   std::sort(sorted.begin(), sorted.end());
   net->max_spread = sorted.empty() ? 0 : sorted.back();
-  const bool lds = net->P == 1 || net->P == 2;
+  const int lw_max = lds_window_limit(net->P);  // 0: no LDS kernel for this phase count
+  const bool lds = lw_max > 0;
   const int q = sorted.empty() ? 0 : sorted[std::min<size_t>(sorted.size() - 1,
                                                    (size_t)(kSmallShare * sorted.size()))];
   // the small window covers kSmallShare of the tiles, or as many as fit
-  const int lw0 = std::min(kSmallLw, round_up8((long long)kTT + q + 1));
+  const int lw0 = std::min({kSmallLw, lw_max, round_up8((long long)kTT + q + 1)});
   int lw1 = round_up8((long long)kTT + net->max_spread + 1);
-  lw1 = std::min(lw1, kLargeLw);
+  lw1 = std::min(lw1, lw_max);
   if (lw0 > 0 && lw1 <= lw0) lw1 = 0;
   net->Lw[0] = lw0;
   net->Lw[1] = lw1;
@@ -664,11 +689,17 @@ int init_batch_attrs() {
   std::call_once(g_batch_attr_once, [] {
     hipError_t r[] = {
         set_batch_attr<BP_F32, 1, 2>(),  set_batch_attr<BP_F32, 2, 2>(),
+        set_batch_attr<BP_F32, 3, 2>(),  set_batch_attr<BP_F32, 4, 2>(),
         set_batch_attr<BP_F32, 1, 3>(),  set_batch_attr<BP_F32, 2, 3>(),
+        set_batch_attr<BP_F32, 3, 3>(),  set_batch_attr<BP_F32, 4, 3>(),
         set_batch_attr<BP_F16, 1, 2>(),  set_batch_attr<BP_F16, 2, 2>(),
+        set_batch_attr<BP_F16, 3, 2>(),  set_batch_attr<BP_F16, 4, 2>(),
         set_batch_attr<BP_F16, 1, 3>(),  set_batch_attr<BP_F16, 2, 3>(),
+        set_batch_attr<BP_F16, 3, 3>(),  set_batch_attr<BP_F16, 4, 3>(),
         set_batch_attr<BP_BF16, 1, 2>(), set_batch_attr<BP_BF16, 2, 2>(),
-        set_batch_attr<BP_BF16, 1, 3>(), set_batch_attr<BP_BF16, 2, 3>()};
+        set_batch_attr<BP_BF16, 3, 2>(), set_batch_attr<BP_BF16, 4, 2>(),
+        set_batch_attr<BP_BF16, 1, 3>(), set_batch_attr<BP_BF16, 2, 3>(),
+        set_batch_attr<BP_BF16, 3, 3>(), set_batch_attr<BP_BF16, 4, 3>()};
     for (hipError_t e : r)
       if (e != hipSuccess) {
         g_batch_attr_rc = fail(BP_EHIP, "hipFuncSetAttribute: %s", hipGetErrorString(e));
@@ -836,10 +867,13 @@ int launch_batch_class(bp_batch *b, int c, hipStream_t st) {
 template <int FMT>
 int launch_batch_fmt(bp_batch *b, hipStream_t st) {
   for (int c = 0; c < 2; ++c) {
-    if (b->P == 1)
-      BP_TRY((launch_batch_class<FMT, 1>(b, c, st)));
-    else if (b->P == 2)
-      BP_TRY((launch_batch_class<FMT, 2>(b, c, st)));
+    switch (b->P) {
+      case 1: BP_TRY((launch_batch_class<FMT, 1>(b, c, st))); break;
+      case 2: BP_TRY((launch_batch_class<FMT, 2>(b, c, st))); break;
+      case 3: BP_TRY((launch_batch_class<FMT, 3>(b, c, st))); break;
+      case 4: BP_TRY((launch_batch_class<FMT, 4>(b, c, st))); break;
+      default: break;  // no LDS tiles: every tile is a direct one
+    }
   }
   for (size_t i = 0; i < b->direct.size(); ++i) {
     const bp::StackArgs &a = b->direct[i];

@@ -607,7 +607,7 @@ struct Fmt<BP_BF16> {
 
 template <int FMT, int P, int KW, int W, int NB, bool NONE>
 struct StackLds {
-  static_assert(P == 1 || P == 2, "the LDS kernel takes one or two phases");
+  static_assert(P >= 1 && P <= 4, "the LDS kernel takes one to four phases");
   static_assert(NB == 2 || NB == 3, "two or three staging buffers");
   using F = Fmt<FMT>;
   static constexpr int ES = F::ES;
@@ -1146,7 +1146,7 @@ __global__ void finalize_max_batch_kernel(const BatchKeys *w) {
 
 // ---------------------------------------------------------------------------
 // Shift-and-stack, direct gathers from global memory.  Used for source tiles
-// the LDS kernel does not take (n_phases outside {1,2}, or a tile whose
+// the LDS kernel does not take (n_phases outside 1..4, or a tile whose
 // delay spread exceeds the largest LDS window).  Same FMA order, same bits.
 // One thread per time sample; blockIdx.y selects a listed source tile of
 // `chunk` sources.

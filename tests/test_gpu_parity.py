@@ -105,7 +105,8 @@ def test_caller_beam_max_kept_where_no_source_is_valid(gpu_lib):
     (31, 20, 2, 3000, 2500, 500, 0),
     (32, 20, 2, 3000, 2500, 500, 1),
     (33, 13, 1, 2000, 1000, 300, 0),
-    (34, 9, 3, 1500, 700, 200, 1),      # three phases -> direct kernel
+    (34, 9, 3, 1500, 700, 200, 1),      # three phases
+    (38, 6, 5, 1200, 300, 150, 0),      # five phases -> direct kernel
     (35, 10, 2, 6000, 400, 4000, 0),    # spread beyond the LDS window -> direct kernel
     (36, 10, 2, 6000, 400, 4000, 1),
     (37, 50, 2, 1100, 130, 1030, 1),    # windows cross both ends
@@ -183,3 +184,30 @@ def test_device_api_matches_drop_in(gpu_lib):
     b = net.beamform(Fd, 1, source_begin=1000, source_end=1300).cpu().numpy()
     bo = oracle.beamform(F, w.time_delays[1000:1300], w.weights_sources[1000:1300], 1)
     assert bits_equal(b, bo)
+
+
+@pytest.mark.parametrize("P", [3, 4])
+def test_three_and_four_phases_in_every_tile_class(gpu_lib, P):
+    """The LDS kernel takes one to four phases; with more phases its largest
+    window shrinks to fit a CU's LDS, so tile spreads put tiles in the small
+    window, the large window and the direct kernel."""
+    from beampower_amd import device as bpd
+    rng = np.random.default_rng(60 + P)
+    S, C, T, K = 9, 3, 4000, 48 * 40
+    u = rng.standard_normal((S, C, T)).astype(np.float32)
+    alpha = rng.uniform(-1, 1, (S, C, P)).astype(np.float32)
+    tau = rng.integers(0, 50, (K, S, P)).astype(np.int32)
+    tau[48 * 7:48 * 9] = rng.integers(0, 600, (96, S, P))      # large window
+    tau[48 * 20:48 * 21] = rng.integers(0, 2500, (48, S, P))   # direct
+    beta = rng.uniform(0.5, 1.5, (K, S)).astype(np.float32)
+    beta[rng.random((K, S)) < 0.1] = 0.0
+    net = bpd.Network(tau, beta, order="natural")
+    cls = net.info()["tiles_per_class"]
+    assert cls[0] > 0 and cls[1] > 0 and cls[2] > 0, cls
+    F = oracle.prestack(u, alpha)
+    for oob in (0, 1):
+        m, a = abi_max(gpu_lib, F, tau, beta, oob)
+        mo, ao = oracle.beamform_max(F, tau, beta, oob)
+        assert bits_equal(m, mo) and np.array_equal(a, ao), (P, oob)
+        b = abi_none(gpu_lib, F, tau, beta, oob)
+        assert bits_equal(b, oracle.beamform(F, tau, beta, oob)), (P, oob)
