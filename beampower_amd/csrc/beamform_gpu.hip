@@ -569,13 +569,39 @@ double plan_cost(const bp_network *net) {
   return c + (double)net->tiles[2].size() * 10.0 * (440.0 + 1024.0);
 }
 
-// Order policy of the per-call entry points: the similarity order is tried
-// only when the natural plan's common window is wide (the per-call plan
-// then pays for itself); otherwise as BP_ORDER_NATURAL.
+// Order policy of the per-call entry points, which plan a network for one
+// call of n_samples: the similarity order is computed only when the most it
+// can save on that call exceeds what computing it costs; otherwise as
+// BP_ORDER_NATURAL.
 constexpr int kOrderAutoWide = 3;
 
+// Seconds per plan-cost unit, sample and station-phase of the stack
+// kernels: 65.7 ms for 4,000 tiles of 440 + 592 units, 100 station-phases
+// and 90,000 samples (50 stations, 192,000 sources, 1 h at 25 Hz).
+constexpr double kUnitSeconds = 1.75e-15;
+
+// The order cannot make a tile's window narrower than TT + 8 samples, so
+// the natural plan's cost above that floor bounds the saving (53 ms at
+// 90,000 samples for a shuffled 192,000-source grid, whose similarity plan
+// saves about 50).  Computing the order takes about 10 ns per delay entry
+// divided by the host threads (19 ms for 192,000 x 100 entries and 48 ms
+// for 300,000 x 240 on 16 threads, 100 and 322 ms on one), plus a second
+// plan (a few ms).
+bool order_pays(const bp_network *net, size_t n_samples) {
+  if (net->K <= kTileSources || !has_lds_tiles(net)) return false;
+  const double floor = (double)net->n_tiles * (440.0 + kTT + 8);
+  const double sp = (double)net->S * net->P;
+  const double saving = (plan_cost(net) - floor) * sp * (double)n_samples * kUnitSeconds;
+  const double entries = (double)net->K * sp;
+  const double threads = (double)bp::order_detail::order_threads((size_t)entries);
+  const double cost = entries * 1e-8 / threads + 5e-3;
+  return saving > cost;
+}
+
+// n_samples: the samples of the one call a kOrderAutoWide plan is made for.
 int network_create(const int *tau, const float *beta, size_t K, size_t S, size_t P,
-                   int order, hipStream_t st, std::unique_ptr<bp_network> &out) {
+                   int order, hipStream_t st, std::unique_ptr<bp_network> &out,
+                   size_t n_samples = 0) {
   auto make = [&]() {
     auto n = std::make_unique<bp_network>();
     n->K = (int)K;
@@ -593,8 +619,7 @@ int network_create(const int *tau, const float *beta, size_t K, size_t S, size_t
   BP_TRY(build_plan(net.get(), st));
 
   bool try_order = order == BP_ORDER_SIMILARITY || (order == BP_ORDER_AUTO && K > kTileSources);
-  if (order == kOrderAutoWide)
-    try_order = K > kTileSources && has_lds_tiles(net.get()) && net->Lw[0] > kTT + 256;
+  if (order == kOrderAutoWide) try_order = order_pays(net.get(), n_samples);
   if (try_order) {
     std::vector<int> ord = bp::similarity_order(tau, K, S * P, kTileSources);
     auto alt = make();
@@ -1560,7 +1585,7 @@ int beamform_max(float *waveform_features, int *time_delays, float *weights_sour
   BP_TRY(st.create());
   std::unique_ptr<bp_network> net;
   BP_TRY(network_create(time_delays, weights_sources, n_sources, n_stations, n_phases,
-                        kOrderAutoWide, st.s, net));
+                        kOrderAutoWide, st.s, net, n_samples));
   const size_t T = n_samples, nf = n_stations * n_samples * n_phases;
   bp_feature_view v{};
   DevBuf f_stp, f_pad, keys, bmax, barg;
@@ -1602,7 +1627,7 @@ int beamform(float *waveform_features, int *time_delays, float *weights_sources,
   BP_TRY(st.create());
   std::unique_ptr<bp_network> net;
   BP_TRY(network_create(time_delays, weights_sources, n_sources, n_stations, n_phases,
-                        kOrderAutoWide, st.s, net));
+                        kOrderAutoWide, st.s, net, n_samples));
   bp_feature_view v{};
   DevBuf f_stp, f_pad, dbeam;
   BP_TRY(stage_dropin_features(net.get(), waveform_features, n_stations, T, n_phases, f_stp,

@@ -30,30 +30,39 @@
 #include <algorithm>
 #include <climits>
 #include <cstddef>
+#include <cstdlib>
 #include <numeric>
+#include <thread>
 #include <utility>
 #include <vector>
 
 namespace bp {
 
-inline std::vector<int> similarity_order(const int *tau, size_t K, size_t D, size_t tile) {
-  std::vector<int> idx(K);
-  std::iota(idx.begin(), idx.end(), 0);
-  if (K <= tile || D == 0) return idx;
-  std::vector<std::pair<size_t, size_t>> todo{{0, K}};
+namespace order_detail {
+
+// Bisection state of one thread: scratch for the split of one range.
+struct Splitter {
+  const int *tau;
+  size_t D, tile;
   std::vector<std::pair<int, int>> key;
-  std::vector<int> lo(D), hi(D);
-  // the split entry is chosen on at most this many evenly spaced sources of
-  // a range (all of them once ranges are small)
-  constexpr size_t kSample = 512;
-  while (!todo.empty()) {
-    const size_t b = todo.back().first, e = todo.back().second;
-    todo.pop_back();
+  std::vector<int> lo, hi;
+
+  Splitter(const int *t, size_t d, size_t tl, size_t max_range)
+      : tau(t), D(d), tile(tl), lo(d), hi(d) {
+    key.reserve(max_range);
+  }
+
+  // Splits idx[b, e) in place at a whole number of tiles and returns the
+  // left part's length, or sorts a leaf (one tile at most) and returns 0.
+  size_t split(int *idx, size_t b, size_t e) {
     const size_t n = e - b;
     if (n <= tile) {
-      std::sort(idx.begin() + (std::ptrdiff_t)b, idx.begin() + (std::ptrdiff_t)e);
-      continue;
+      std::sort(idx + b, idx + e);
+      return 0;
     }
+    // the split entry is chosen on at most this many evenly spaced sources
+    // of a range (all of them once ranges are small)
+    constexpr size_t kSample = 512;
     std::fill(lo.begin(), lo.end(), INT_MAX);
     std::fill(hi.begin(), hi.end(), INT_MIN);
     const size_t step = n > kSample ? n / kSample : 1;
@@ -83,10 +92,82 @@ inline std::vector<int> similarity_order(const int *tau, size_t K, size_t D, siz
     }
     std::nth_element(key.begin(), key.begin() + (std::ptrdiff_t)nl, key.end());
     for (size_t i = 0; i < n; ++i) idx[b + i] = key[i].second;
-    todo.push_back({b + nl, e});
-    todo.push_back({b, b + nl});
+    return nl;
   }
+
+  // Bisects idx[b, e) down to its leaves.
+  void run(int *idx, size_t b, size_t e) {
+    std::vector<std::pair<size_t, size_t>> todo{{b, e}};
+    while (!todo.empty()) {
+      const auto r = todo.back();
+      todo.pop_back();
+      const size_t nl = split(idx, r.first, r.second);
+      if (nl == 0) continue;
+      todo.push_back({r.first + nl, r.second});
+      todo.push_back({r.first, r.first + nl});
+    }
+  }
+};
+
+// Host threads for an order over `entries` delay entries: one below about
+// a million entries (a thread start costs more than it saves there), else
+// the hardware's threads, at most OMP_NUM_THREADS when that is set, and at
+// most 16.
+inline unsigned order_threads(size_t entries) {
+  if (entries < ((size_t)1 << 20)) return 1;
+  unsigned n = std::thread::hardware_concurrency();
+  if (const char *env = std::getenv("OMP_NUM_THREADS")) {
+    const long v = std::strtol(env, nullptr, 10);
+    if (v > 0) n = std::min<unsigned>(n, (unsigned)v);
+  }
+  return std::max(1u, std::min(n, 16u));
+}
+
+// Bisects idx[b, e) with the splitters parts[t0, t0 + threads): the range
+// is split on this thread, and its halves go to two threads (this one and
+// a new one) each with half of the splitters, until one splitter is left.
+inline void bisect(std::vector<Splitter> &parts, int *idx, size_t b, size_t e, unsigned t0,
+                   unsigned threads) {
+  Splitter &sp = parts[t0];
+  if (threads <= 1) {
+    sp.run(idx, b, e);
+    return;
+  }
+  const size_t nl = sp.split(idx, b, e);
+  if (nl == 0) return;
+  const unsigned lt = threads / 2;
+  std::thread right;
+  try {
+    right = std::thread(bisect, std::ref(parts), idx, b + nl, e, t0 + lt, threads - lt);
+  } catch (...) {
+    sp.run(idx, b + nl, e);  // no thread: the right half is bisected here
+  }
+  bisect(parts, idx, b, b + nl, t0, lt);
+  if (right.joinable()) right.join();
+}
+
+}  // namespace order_detail
+
+// Each range is split by a function of its own sources and their order
+// only, and ranges are disjoint, so subtrees are bisected on `threads` host
+// threads with the same result as on one thread.
+inline std::vector<int> similarity_order(const int *tau, size_t K, size_t D, size_t tile,
+                                         unsigned threads) {
+  std::vector<int> idx(K);
+  std::iota(idx.begin(), idx.end(), 0);
+  if (K <= tile || D == 0) return idx;
+  const unsigned nt = std::max(1u, std::min(threads, 64u));
+  // the first range of every splitter but the first is at most half of K
+  // and a tile
+  std::vector<order_detail::Splitter> parts;
+  parts.reserve(nt);
+  for (unsigned t = 0; t < nt; ++t) parts.emplace_back(tau, D, tile, t ? K / 2 + tile : K);
+  order_detail::bisect(parts, idx.data(), 0, K, 0, nt);
   return idx;
+}
+
+inline std::vector<int> similarity_order(const int *tau, size_t K, size_t D, size_t tile) {
+  return similarity_order(tau, K, D, tile, order_detail::order_threads(K * D));
 }
 
 }  // namespace bp

@@ -174,8 +174,10 @@ def test_batch_of_ordered_networks_equals_natural_windows():
 
 @pytest.mark.parametrize("oob", [0, 1])
 def test_dropin_on_shuffled_sources(gpu_lib, oob):
-    """The per-call drop-in plans a similarity order when the caller's order
-    gives wide tiles (here every tile spans the whole volume)."""
+    """The per-call drop-in on shuffled sources with ties far apart.  At this
+    size its plan keeps the caller's order (an order would cost the call more
+    than it saves); the similarity-order plans of larger calls are those the
+    Network tests above check in every order."""
     u, alpha, tau, beta = shuffled_network(7, grid=(20, 12, 10), box_km=100.0, T=3500)
     F = oracle.prestack(u, alpha, flavour="fma")
     S, T, P = F.shape
