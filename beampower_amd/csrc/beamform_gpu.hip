@@ -15,8 +15,11 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <memory>
 #include <mutex>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <utility>
 #include <vector>
@@ -31,13 +34,17 @@ namespace {
 
 thread_local std::string g_err;
 
-int fail(int code, const char *fmt, ...) {
+int fail(int code, const char *fmt, ...) noexcept {
   char buf[512];
   va_list ap;
   va_start(ap, fmt);
   vsnprintf(buf, sizeof buf, fmt, ap);
   va_end(ap);
-  g_err = buf;
+  try {
+    g_err = buf;
+  } catch (...) {
+    g_err.clear();  // no memory for the message: the status still tells
+  }
   return code;
 }
 
@@ -54,6 +61,24 @@ int fail(int code, const char *fmt, ...) {
     int rc_ = (expr);         \
     if (rc_ != BP_OK) return rc_; \
   } while (0)
+
+// Handlers of the function-try-block of every int entry point: a host
+// exception (a std::vector that cannot grow, a thread that cannot start)
+// never crosses the C ABI, it becomes a status and a message.  The
+// entry point's device buffers are released as its locals unwind.
+#define BP_CATCH                                                        \
+  catch (const std::bad_alloc &) {                                      \
+    return fail(BP_ENOMEM, "host memory allocation failed");            \
+  }                                                                     \
+  catch (const std::length_error &) {                                   \
+    return fail(BP_ENOMEM, "host allocation too large");                \
+  }                                                                     \
+  catch (const std::exception &e_) {                                    \
+    return fail(BP_EINVAL, "host exception: %s", e_.what());            \
+  }                                                                     \
+  catch (...) {                                                         \
+    return fail(BP_EINVAL, "unknown host exception");                   \
+  }
 
 // Kernel configuration of the LDS shift-and-stack kernel (overridable at
 // build time for tuning sweeps).  6 sources per wave keep the accumulators
@@ -1063,12 +1088,12 @@ const char *bp_last_error(void) { return g_err.c_str(); }
 
 const char *bp_version(void) { return BP_VERSION; }
 
-int bp_profiling(int enable) {
+int bp_profiling(int enable) try {
   g_prof.on = enable != 0;
   return BP_OK;
-}
+} BP_CATCH
 
-int bp_profiling_read(double *stack_ms_total, long long *stack_launches) {
+int bp_profiling_read(double *stack_ms_total, long long *stack_launches) try {
   for (auto &e : g_prof.pending) {
     HIP_TRY(hipEventSynchronize(e.second));
     float ms = 0.f;
@@ -1084,17 +1109,17 @@ int bp_profiling_read(double *stack_ms_total, long long *stack_launches) {
   g_prof.ms = 0.0;
   g_prof.launches = 0;
   return BP_OK;
-}
+} BP_CATCH
 
 int bp_network_create(const int *time_delays, const float *weights_sources, size_t n_sources,
-                      size_t n_stations, size_t n_phases, bp_network **out) {
+                      size_t n_stations, size_t n_phases, bp_network **out) try {
   return bp_network_create_ordered(time_delays, weights_sources, n_sources, n_stations,
                                     n_phases, BP_ORDER_AUTO, out);
-}
+} BP_CATCH
 
 int bp_network_create_ordered(const int *time_delays, const float *weights_sources,
                               size_t n_sources, size_t n_stations, size_t n_phases, int order,
-                              bp_network **out) {
+                              bp_network **out) try {
   if (!out) return fail(BP_EINVAL, "out is NULL");
   if (order != BP_ORDER_NATURAL && order != BP_ORDER_SIMILARITY && order != BP_ORDER_AUTO)
     return fail(BP_EINVAL, "unknown source order policy %d", order);
@@ -1111,50 +1136,50 @@ int bp_network_create_ordered(const int *time_delays, const float *weights_sourc
                         st.s, net));
   *out = net.release();
   return BP_OK;
-}
+} BP_CATCH
 
-int bp_network_destroy(bp_network *net) {
+int bp_network_destroy(bp_network *net) try {
   delete net;
   return BP_OK;
-}
+} BP_CATCH
 
-int bp_network_source_order(const bp_network *net, int *permuted, int *order) {
+int bp_network_source_order(const bp_network *net, int *permuted, int *order) try {
   if (!net) return fail(BP_EINVAL, "NULL network");
   if (permuted) *permuted = net->permuted ? 1 : 0;
   if (order)
     for (int i = 0; i < net->K; ++i) order[i] = net->permuted ? net->order[i] : i;
   return BP_OK;
-}
+} BP_CATCH
 
-int bp_kernel_shape(int *source_tile, int *time_tile, int *waves_per_simd) {
+int bp_kernel_shape(int *source_tile, int *time_tile, int *waves_per_simd) try {
   if (source_tile) *source_tile = kTileSources;
   if (time_tile) *time_tile = kTT;
   if (waves_per_simd) *waves_per_simd = BP_WAVES_PER_EU;
   return BP_OK;
-}
+} BP_CATCH
 
 int bp_network_info(const bp_network *net, int *source_tile, int *max_tile_spread,
-                    int *n_tiles) {
+                    int *n_tiles) try {
   if (!net) return fail(BP_EINVAL, "NULL network");
   if (source_tile) *source_tile = kTileSources;
   if (max_tile_spread) *max_tile_spread = net->max_spread;
   if (n_tiles) *n_tiles = net->n_tiles;
   return BP_OK;
-}
+} BP_CATCH
 
-int bp_network_classes(const bp_network *net, int *tiles_per_class, int *window_per_class) {
+int bp_network_classes(const bp_network *net, int *tiles_per_class, int *window_per_class) try {
   if (!net) return fail(BP_EINVAL, "NULL network");
   for (int c = 0; c < 3; ++c) {
     if (tiles_per_class) tiles_per_class[c] = (int)net->tiles[c].size();
     if (window_per_class) window_per_class[c] = c < 2 ? net->Lw[c] : 0;
   }
   return BP_OK;
-}
+} BP_CATCH
 
 int bp_network_beamform_max(bp_network *net, const void *features, int fmt,
                             const float *station_scale, size_t n_samples, int out_of_bounds,
                             float *beam_max, int *beam_argmax, unsigned long long *keys,
-                            void *stream) {
+                            void *stream) try {
   if (!net || !features || !beam_max || !beam_argmax || !keys)
     return fail(BP_EINVAL, "NULL argument");
   if (out_of_bounds != 0 && out_of_bounds != 1)
@@ -1168,11 +1193,11 @@ int bp_network_beamform_max(bp_network *net, const void *features, int fmt,
   if (n_samples == 0) return BP_OK;
   return network_max(net, plain_view(features, fmt, station_scale, n_samples), out_of_bounds,
                      beam_max, beam_argmax, keys, static_cast<hipStream_t>(stream));
-}
+} BP_CATCH
 
 int bp_network_beamform_max_view(bp_network *net, const bp_feature_view *features,
                                  int out_of_bounds, float *beam_max, int *beam_argmax,
-                                 unsigned long long *keys, void *stream) {
+                                 unsigned long long *keys, void *stream) try {
   if (!net || !features || !beam_max || !beam_argmax || !keys)
     return fail(BP_EINVAL, "NULL argument");
   if (out_of_bounds != 0 && out_of_bounds != 1)
@@ -1182,9 +1207,9 @@ int bp_network_beamform_max_view(bp_network *net, const bp_feature_view *feature
   if (features->n_samples == 0) return BP_OK;
   return network_max(net, *features, out_of_bounds, beam_max, beam_argmax, keys,
                      static_cast<hipStream_t>(stream));
-}
+} BP_CATCH
 
-int bp_network_prepare(bp_network *net, int fmt, const float *station_scale) {
+int bp_network_prepare(bp_network *net, int fmt, const float *station_scale) try {
   if (!net) return fail(BP_EINVAL, "NULL network");
   if (fmt != BP_F32 && fmt != BP_F16 && fmt != BP_BF16)
     return fail(BP_EINVAL, "unknown feature format %d", fmt);
@@ -1194,20 +1219,20 @@ int bp_network_prepare(bp_network *net, int fmt, const float *station_scale) {
   Stream st;
   BP_TRY(st.create());
   return ensure_schedule(net, fmt, station_scale, st.s);
-}
+} BP_CATCH
 
-int bp_network_padding(const bp_network *net, size_t *pad_before, size_t *pad_after) {
+int bp_network_padding(const bp_network *net, size_t *pad_before, size_t *pad_after) try {
   if (!net) return fail(BP_EINVAL, "NULL network");
   if (pad_before) *pad_before = (size_t)net->pad_lo;
   if (pad_after) *pad_after = (size_t)net->pad_hi;
   return BP_OK;
-}
+} BP_CATCH
 
 size_t bp_parity_stride(size_t n_samples) { return parity_stride(n_samples); }
 
 int bp_pack_features(const float *F_spt, size_t n_stations, size_t n_phases, size_t n_samples,
                      int fmt, size_t pad_before, size_t row_stride, float *station_scale,
-                     void *out, void *stream) {
+                     void *out, void *stream) try {
   if (!F_spt || !out) return fail(BP_EINVAL, "NULL argument");
   if (fmt != BP_F32 && fmt != BP_F16 && fmt != BP_BF16)
     return fail(BP_EINVAL, "unknown feature format %d", fmt);
@@ -1264,10 +1289,10 @@ int bp_pack_features(const float *F_spt, size_t n_stations, size_t n_phases, siz
   HIP_TRY(hipGetLastError());
   if (fmt == BP_F16) HIP_TRY(hipStreamSynchronize(st));  // dscale is freed on return
   return BP_OK;
-}
+} BP_CATCH
 
 int bp_quantize_planar(const float *F_spt, size_t n_stations, size_t n_phases, size_t n_samples,
-                       int fmt, float *station_scale, void *out, void *stream) {
+                       int fmt, float *station_scale, void *out, void *stream) try {
   if (!F_spt || !station_scale || !out) return fail(BP_EINVAL, "NULL argument");
   if (fmt != BP_F16 && fmt != BP_BF16)
     return fail(BP_EINVAL, "quantization targets BP_F16 or BP_BF16, not %d", fmt);
@@ -1275,11 +1300,11 @@ int bp_quantize_planar(const float *F_spt, size_t n_stations, size_t n_phases, s
   if (n_stations == 0 || n_phases == 0 || n_samples == 0) return BP_OK;
   return bp_pack_features(F_spt, n_stations, n_phases, n_samples, fmt, 0,
                           parity_stride(n_samples), station_scale, out, stream);
-}
+} BP_CATCH
 
 int bp_network_beamform(bp_network *net, const float *features, size_t n_samples,
                         int out_of_bounds, size_t source_begin, size_t source_end,
-                        float *beam, void *stream) {
+                        float *beam, void *stream) try {
   if (!net || !features || !beam) return fail(BP_EINVAL, "NULL argument");
   if (out_of_bounds != 0 && out_of_bounds != 1)
     return fail(BP_EINVAL, "out_of_bounds must be 0 (strict) or 1 (flexible), not %d",
@@ -1293,11 +1318,11 @@ int bp_network_beamform(bp_network *net, const float *features, size_t n_samples
   a.beam = beam;
   return launch_stack<BP_F32, true>(net, a, (int)source_begin, (int)source_end,
                                     static_cast<hipStream_t>(stream));
-}
+} BP_CATCH
 
 int bp_network_beamform_view(bp_network *net, const bp_feature_view *features,
                              int out_of_bounds, size_t source_begin, size_t source_end,
-                             float *beam, void *stream) {
+                             float *beam, void *stream) try {
   if (!net || !features || !beam) return fail(BP_EINVAL, "NULL argument");
   if (out_of_bounds != 0 && out_of_bounds != 1)
     return fail(BP_EINVAL, "out_of_bounds must be 0 (strict) or 1 (flexible), not %d",
@@ -1317,12 +1342,12 @@ int bp_network_beamform_view(bp_network *net, const bp_feature_view *features,
   if (fmt == BP_F16)
     return launch_stack<BP_F16, true>(net, a, (int)source_begin, (int)source_end, st);
   return launch_stack<BP_BF16, true>(net, a, (int)source_begin, (int)source_end, st);
-}
+} BP_CATCH
 
 // ---- differential mode (beamform.c:338-394) -------------------------------
 
 int bp_network_beamform_differential(bp_network *net, const bp_feature_view *features,
-                                     float *beam, void *stream) {
+                                     float *beam, void *stream) try {
   if (!net || !features || !beam) return fail(BP_EINVAL, "NULL argument");
   BP_TRY(check_view(net, *features));
   if (features->n_samples == 0 || net->K == 0) return BP_OK;
@@ -1377,7 +1402,7 @@ int bp_network_beamform_differential(bp_network *net, const bp_feature_view *fea
   const DevBuf &tauT = pairs ? net->diff_tau_pairs : net->diff_tau;
   return launch_differential(fmt, a, tauT.as<int>(), net->diff_beta.as<float>(), beam, pairs,
                              st);
-}
+} BP_CATCH
 
 // The CPU library's entry point (beamform_cpu.h:6, beamform.c:338-394) on
 // the GPU, host pointers, synchronous; num_threads is accepted and unused.
@@ -1385,7 +1410,7 @@ int bp_network_beamform_differential(bp_network *net, const bp_feature_view *fea
 // passes zeros, beampower.py:198).
 int beamform_differential(float *waveform_features, int *time_delays, float *weights,
                           size_t n_samples, size_t n_sources, size_t n_stations,
-                          size_t n_phases, int num_threads, float *beam) {
+                          size_t n_phases, int num_threads, float *beam) try {
   (void)num_threads;
   BP_TRY(check_shapes(n_samples, n_sources, n_stations, n_phases));
   // no sample, no source, or no station/phase (no source uses a station)
@@ -1431,12 +1456,12 @@ int beamform_differential(float *waveform_features, int *time_delays, float *wei
   HIP_TRY(hipMemcpyAsync(beam, dbeam.p, K * sizeof(float), hipMemcpyDeviceToHost, st.s));
   HIP_TRY(hipStreamSynchronize(st.s));
   return BP_OK;
-}
+} BP_CATCH
 
 // ---- detection consumer (notebooks/3_beampower.ipynb:497-626) ------------
 
 int bp_detect_threshold(const float *beam_max, size_t n_samples, size_t window, double overlap,
-                        double num_dev, float *threshold, void *stream) {
+                        double num_dev, float *threshold, void *stream) try {
   if (!beam_max || !threshold) return fail(BP_EINVAL, "NULL argument");
   const long long n = (long long)n_samples, W = (long long)window;
   const long long shift = (long long)((1.0 - overlap) * (double)window);
@@ -1474,11 +1499,11 @@ int bp_detect_threshold(const float *beam_max, size_t n_samples, size_t window, 
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(st));  // thr and dthr are released on return
   return BP_OK;
-}
+} BP_CATCH
 
 int bp_detect_peaks(const float *beam_max, const float *threshold, size_t n_samples,
                     size_t distance, int *peaks, size_t capacity, size_t *n_peaks,
-                    void *stream) {
+                    void *stream) try {
   if (!beam_max || !n_peaks || (capacity && !peaks)) return fail(BP_EINVAL, "NULL argument");
   *n_peaks = 0;
   if (!size_ok(n_samples) || distance < 1)
@@ -1533,10 +1558,10 @@ int bp_detect_peaks(const float *beam_max, const float *threshold, size_t n_samp
   }
   *n_peaks = m;
   return BP_OK;
-}
+} BP_CATCH
 
 int bp_interleaved_to_planar(const float *F_stp, size_t n_stations, size_t n_samples,
-                             size_t n_phases, float *F_spt, void *stream) {
+                             size_t n_phases, float *F_spt, void *stream) try {
   if (!F_stp || !F_spt) return fail(BP_EINVAL, "NULL argument");
   BP_TRY(check_shapes(n_samples, 0, n_stations, n_phases));
   if (n_samples == 0 || n_stations == 0 || n_phases == 0) return BP_OK;
@@ -1546,11 +1571,11 @@ int bp_interleaved_to_planar(const float *F_stp, size_t n_stations, size_t n_sam
                      (int)n_samples, (int)n_phases, (long long)n_samples, 0LL, F_spt);
   HIP_TRY(hipGetLastError());
   return BP_OK;
-}
+} BP_CATCH
 
 int bp_prestack_planar(const float *u_sct, const float *weights_phases, size_t n_stations,
                        size_t n_channels, size_t n_samples, size_t n_phases, float *F_spt,
-                       void *stream) {
+                       void *stream) try {
   if (!u_sct || !weights_phases || !F_spt) return fail(BP_EINVAL, "NULL argument");
   BP_TRY(check_shapes(n_samples, n_channels, n_stations, n_phases));
   if (n_samples == 0 || n_stations == 0 || n_phases == 0) return BP_OK;
@@ -1561,13 +1586,13 @@ int bp_prestack_planar(const float *u_sct, const float *weights_phases, size_t n
                      F_spt);
   HIP_TRY(hipGetLastError());
   return BP_OK;
-}
+} BP_CATCH
 
 // ---- drop-in entry points ------------------------------------------------
 
 int beamform_max(float *waveform_features, int *time_delays, float *weights_sources,
                  size_t n_samples, size_t n_sources, size_t n_stations, size_t n_phases,
-                 int out_of_bounds, float *beam_max, int *beam_argmax) {
+                 int out_of_bounds, float *beam_max, int *beam_argmax) try {
   if (out_of_bounds != 0 && out_of_bounds != 1)
     return fail(BP_EINVAL, "out_of_bounds must be 0 (strict) or 1 (flexible), not %d",
                 out_of_bounds);
@@ -1603,11 +1628,11 @@ int beamform_max(float *waveform_features, int *time_delays, float *weights_sour
   HIP_TRY(hipMemcpyAsync(beam_argmax, barg.p, T * sizeof(int), hipMemcpyDeviceToHost, st.s));
   HIP_TRY(hipStreamSynchronize(st.s));
   return BP_OK;
-}
+} BP_CATCH
 
 int beamform(float *waveform_features, int *time_delays, float *weights_sources,
              size_t n_samples, size_t n_sources, size_t n_stations, size_t n_phases,
-             int out_of_bounds, float *beam) {
+             int out_of_bounds, float *beam) try {
   if (out_of_bounds != 0 && out_of_bounds != 1)
     return fail(BP_EINVAL, "out_of_bounds must be 0 (strict) or 1 (flexible), not %d",
                 out_of_bounds);
@@ -1645,11 +1670,11 @@ int beamform(float *waveform_features, int *time_delays, float *weights_sources,
   HIP_TRY(hipMemcpyAsync(beam, dbeam.p, K * T * sizeof(float), hipMemcpyDeviceToHost, st.s));
   HIP_TRY(hipStreamSynchronize(st.s));
   return BP_OK;
-}
+} BP_CATCH
 
 int prestack_waveform_features(float *waveform_features, float *weights_phases,
                                size_t n_samples, size_t n_stations, size_t n_channels,
-                               size_t n_phases, int num_threads, float *prestacked) {
+                               size_t n_phases, int num_threads, float *prestacked) try {
   (void)num_threads;
   BP_TRY(check_shapes(n_samples, n_channels, n_stations, n_phases));
   const size_t S = n_stations, C = n_channels, T = n_samples, P = n_phases;
@@ -1679,12 +1704,12 @@ int prestack_waveform_features(float *waveform_features, float *weights_phases,
                          hipMemcpyDeviceToHost, st.s));
   HIP_TRY(hipStreamSynchronize(st.s));
   return BP_OK;
-}
+} BP_CATCH
 
 int bp_batch_create(int n_windows, bp_network *const *networks,
                     const bp_feature_view *features, const int *out_of_bounds,
                     float *const *beam_max, int *const *beam_argmax,
-                    unsigned long long *const *keys, bp_batch **out) {
+                    unsigned long long *const *keys, bp_batch **out) try {
   if (!out) return fail(BP_EINVAL, "out is NULL");
   *out = nullptr;
   if (n_windows <= 0 || !networks || !features || !out_of_bounds || !beam_max ||
@@ -1774,14 +1799,14 @@ int bp_batch_create(int n_windows, bp_network *const *networks,
   HIP_TRY(hipStreamSynchronize(st.s));
   *out = b.release();
   return BP_OK;
-}
+} BP_CATCH
 
-int bp_batch_launch(bp_batch *b, void *stream) {
+int bp_batch_launch(bp_batch *b, void *stream) try {
   if (!b) return fail(BP_EINVAL, "NULL batch");
   return batch_enqueue(b, static_cast<hipStream_t>(stream), true);
-}
+} BP_CATCH
 
-int bp_batch_graph_launch(bp_batch *b, void *stream) {
+int bp_batch_graph_launch(bp_batch *b, void *stream) try {
   if (!b) return fail(BP_EINVAL, "NULL batch");
   if (!b->exec) {
     if (!b->cap) HIP_TRY(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
@@ -1799,21 +1824,21 @@ int bp_batch_graph_launch(bp_batch *b, void *stream) {
   }
   HIP_TRY(hipGraphLaunch(b->exec, static_cast<hipStream_t>(stream)));
   return BP_OK;
-}
+} BP_CATCH
 
-int bp_batch_get_config(const bp_batch *b, bp_batch_config *cfg) {
+int bp_batch_get_config(const bp_batch *b, bp_batch_config *cfg) try {
   if (!b || !cfg) return fail(BP_EINVAL, "NULL argument");
   *cfg = b->cfg;
   return BP_OK;
-}
+} BP_CATCH
 
-int bp_batch_set_config(bp_batch *b, const bp_batch_config *cfg) {
+int bp_batch_set_config(bp_batch *b, const bp_batch_config *cfg) try {
   if (!b || !cfg) return fail(BP_EINVAL, "NULL argument");
   return batch_configure(b, *cfg);
-}
+} BP_CATCH
 
 int bp_batch_tune(bp_batch *b, int reps, bp_batch_config *tried, int max_tried, int *n_tried,
-                  bp_batch_config *best, void *stream) {
+                  bp_batch_config *best, void *stream) try {
   if (!b) return fail(BP_EINVAL, "NULL batch");
   if (reps < 1) reps = 1;
   hipStream_t st = static_cast<hipStream_t>(stream);
@@ -1855,11 +1880,11 @@ int bp_batch_tune(bp_batch *b, int reps, bp_batch_config *tried, int max_tried, 
   }
   if (best) *best = win;
   return BP_OK;
-}
+} BP_CATCH
 
-int bp_batch_destroy(bp_batch *b) {
+int bp_batch_destroy(bp_batch *b) try {
   delete b;
   return BP_OK;
-}
+} BP_CATCH
 
 }  // extern "C"

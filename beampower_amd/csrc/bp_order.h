@@ -31,7 +31,9 @@
 #include <climits>
 #include <cstddef>
 #include <cstdlib>
+#include <exception>
 #include <numeric>
+#include <system_error>
 #include <thread>
 #include <utility>
 #include <vector>
@@ -136,14 +138,29 @@ inline void bisect(std::vector<Splitter> &parts, int *idx, size_t b, size_t e, u
   const size_t nl = sp.split(idx, b, e);
   if (nl == 0) return;
   const unsigned lt = threads / 2;
+  // an exception of the new thread is carried to this one and rethrown
+  // after the join (none may leave a thread's function)
+  std::exception_ptr right_error;
   std::thread right;
   try {
-    right = std::thread(bisect, std::ref(parts), idx, b + nl, e, t0 + lt, threads - lt);
-  } catch (...) {
+    right = std::thread([&parts, &right_error, idx, b, e, nl, t0, lt, threads] {
+      try {
+        bisect(parts, idx, b + nl, e, t0 + lt, threads - lt);
+      } catch (...) {
+        right_error = std::current_exception();
+      }
+    });
+  } catch (const std::system_error &) {
     sp.run(idx, b + nl, e);  // no thread: the right half is bisected here
   }
-  bisect(parts, idx, b, b + nl, t0, lt);
+  try {
+    bisect(parts, idx, b, b + nl, t0, lt);
+  } catch (...) {
+    if (right.joinable()) right.join();
+    throw;
+  }
   if (right.joinable()) right.join();
+  if (right_error) std::rethrow_exception(right_error);
 }
 
 }  // namespace order_detail

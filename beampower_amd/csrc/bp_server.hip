@@ -25,8 +25,11 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <exception>
 #include <map>
+#include <memory>
 #include <mutex>
+#include <new>
 #include <string>
 #include <thread>
 #include <vector>
@@ -71,11 +74,18 @@ class Arena {
     }
     return -1;
   }
-  void release(long long off, size_t bytes) {
+  // Never throws: a block whose free-list node cannot be allocated stays
+  // counted as used (lost to the budget) rather than ending the process.
+  void release(long long off, size_t bytes) noexcept {
     if (off < 0) return;
     bytes = align_up(bytes);
+    std::map<size_t, size_t>::iterator it;
+    try {
+      it = free_.emplace((size_t)off, bytes).first;
+    } catch (...) {
+      return;
+    }
     used_ -= bytes;
-    auto it = free_.emplace((size_t)off, bytes).first;
     auto next = std::next(it);
     if (next != free_.end() && it->first + it->second == next->first) {
       it->second += next->second;
@@ -135,10 +145,29 @@ struct Flight {
 
 thread_local std::string t_err;
 
-int fail(int code, const std::string &msg) {
-  t_err = msg;
+int fail(int code, const char *msg) noexcept {
+  try {
+    t_err = msg;
+  } catch (...) {
+    t_err.clear();  // no memory for the message: the status still tells
+  }
   return code;
 }
+
+int fail(int code, const std::string &msg) noexcept { return fail(code, msg.c_str()); }
+
+// Handlers of the function-try-block of every int entry point: a host
+// exception never crosses the C ABI, it becomes a status and a message.
+#define BP_SERVER_CATCH                                                 \
+  catch (const std::bad_alloc &) {                                      \
+    return fail(BP_ENOMEM, "host memory allocation failed");            \
+  }                                                                     \
+  catch (const std::exception &e_) {                                    \
+    return fail(BP_EINVAL, e_.what());                                  \
+  }                                                                     \
+  catch (...) {                                                         \
+    return fail(BP_EINVAL, "unknown host exception");                   \
+  }
 
 }  // namespace
 
@@ -156,6 +185,7 @@ struct bp_server {
   std::vector<hipStream_t> compute;
   std::thread loader, scheduler;
   bool stop = false;
+  bool broken = false;  // a worker thread failed: submits are refused
   long long next_id = 1;
   long long n_batches = 0, n_batched_windows = 0, n_completed = 0;
   int launch_index = 0;
@@ -173,10 +203,52 @@ struct bp_server {
 
   void loader_loop();
   void scheduler_loop();
+  void guarded(void (bp_server::*loop)());
+  void fail_request(Request *r, int status, const char *msg) noexcept;
   int admit(Request *r);
   void launch(std::vector<Request *> batch);
   void complete(Flight &f);
 };
+
+// Called with the lock held: request r ends with `status`, its device
+// memory back in the arena.
+void bp_server::fail_request(Request *r, int status, const char *msg) noexcept {
+  r->info.status = status;
+  try {
+    r->err = msg;
+  } catch (...) {
+  }
+  arena.release(r->off_tmp, r->bytes_tmp);
+  arena.release(r->off_keep, r->bytes_keep);
+  r->off_tmp = r->off_keep = -1;
+  r->done = true;
+}
+
+// Runs a worker loop.  A host exception that leaves it (a container that
+// cannot grow) would end the process; instead the server is marked broken,
+// every request not yet launched fails with BP_ENOMEM, the batches in
+// flight are completed, and later submits are refused.
+void bp_server::guarded(void (bp_server::*loop)()) {
+  try {
+    (this->*loop)();
+    return;
+  } catch (...) {
+  }
+  std::deque<Flight> left;
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    broken = true;
+    for (std::deque<Request *> *q : {&pending, &ready}) {
+      for (Request *r : *q) fail_request(r, BP_ENOMEM, "server worker failed: host memory");
+      q->clear();
+    }
+    left.swap(flights);
+  }
+  for (Flight &f : left) complete(f);
+  cv_done.notify_all();
+  cv_load.notify_all();
+  cv_sched.notify_all();
+}
 
 int bp_server::admit(Request *r) {
   const Network &n = nets[r->net];
@@ -243,64 +315,76 @@ void bp_server::loader_loop() {
     r = pending.front();
     pending.pop_front();
     if (r->off_tmp < 0) {
-      r->info.status = fail(BP_ENOMEM, "window needs more device memory than the budget");
-      r->err = t_err;
-      r->done = true;
+      fail_request(r, BP_ENOMEM, "window needs more device memory than the budget");
       cv_done.notify_all();
       continue;
     }
     r->info.admit_s = now_s();
     lock.unlock();
-    const int rc = admit(r);
+    int rc;
+    try {
+      rc = admit(r);
+    } catch (...) {
+      rc = fail(BP_ENOMEM, "admission: host memory allocation failed");
+    }
     lock.lock();
     arena.release(r->off_tmp, r->bytes_tmp);
     r->off_tmp = -1;
     r->info.ready_s = now_s();
     if (rc != BP_OK) {
-      r->info.status = rc;
-      r->err = t_err;
-      arena.release(r->off_keep, r->bytes_keep);
-      r->off_keep = -1;
-      r->done = true;
+      fail_request(r, rc, t_err.c_str());
       cv_done.notify_all();
       cv_load.notify_all();
       continue;
     }
-    ready.push_back(r);
+    try {
+      ready.push_back(r);
+    } catch (...) {
+      fail_request(r, BP_ENOMEM, "admission: host memory allocation failed");
+      cv_done.notify_all();
+      cv_load.notify_all();
+      continue;
+    }
     cv_sched.notify_all();
   }
 }
 
 void bp_server::launch(std::vector<Request *> batch) {
   const int n = (int)batch.size();
-  std::vector<bp_network *> nv(n);
-  std::vector<bp_feature_view> views(n);
-  std::vector<int> oob(n);
-  std::vector<float *> bmax(n);
-  std::vector<int *> barg(n);
-  std::vector<unsigned long long *> keys(n);
-  for (int i = 0; i < n; ++i) {
-    Request *r = batch[i];
-    nv[i] = nets[r->net].net;
-    views[i] = r->view;
-    oob[i] = r->oob;
-    bmax[i] = r->d_bmax;
-    barg[i] = r->d_barg;
-    keys[i] = r->d_keys;
-  }
   Flight f;
-  f.reqs = batch;
-  hipStream_t st = compute[launch_index++ % compute.size()];
-  int rc = bp_batch_create(n, nv.data(), views.data(), oob.data(), bmax.data(), barg.data(),
-                           keys.data(), &f.batch);
-  if (rc != BP_OK) fail(rc, bp_last_error());
-  const double t = now_s();
-  if (rc == BP_OK && (rc = bp_batch_launch(f.batch, st)) != BP_OK) fail(rc, bp_last_error());
-  if (rc == BP_OK && hipEventCreateWithFlags(&f.ev, hipEventDisableTiming) != hipSuccess)
-    rc = fail(BP_EHIP, "hipEventCreate failed");
-  if (rc == BP_OK && hipEventRecord(f.ev, st) != hipSuccess)
-    rc = fail(BP_EHIP, "hipEventRecord failed");
-  std::lock_guard<std::mutex> lock(mu);
+  int rc = BP_OK;
+  double t = now_s();
+  try {
+    std::vector<bp_network *> nv(n);
+    std::vector<bp_feature_view> views(n);
+    std::vector<int> oob(n);
+    std::vector<float *> bmax(n);
+    std::vector<int *> barg(n);
+    std::vector<unsigned long long *> keys(n);
+    for (int i = 0; i < n; ++i) {
+      Request *r = batch[i];
+      nv[i] = nets[r->net].net;
+      views[i] = r->view;
+      oob[i] = r->oob;
+      bmax[i] = r->d_bmax;
+      barg[i] = r->d_barg;
+      keys[i] = r->d_keys;
+    }
+    f.reqs = batch;
+    hipStream_t st = compute[launch_index++ % compute.size()];
+    rc = bp_batch_create(n, nv.data(), views.data(), oob.data(), bmax.data(), barg.data(),
+                         keys.data(), &f.batch);
+    if (rc != BP_OK) fail(rc, bp_last_error());
+    t = now_s();
+    if (rc == BP_OK && (rc = bp_batch_launch(f.batch, st)) != BP_OK) fail(rc, bp_last_error());
+    if (rc == BP_OK && hipEventCreateWithFlags(&f.ev, hipEventDisableTiming) != hipSuccess)
+      rc = fail(BP_EHIP, "hipEventCreate failed");
+    if (rc == BP_OK && hipEventRecord(f.ev, st) != hipSuccess)
+      rc = fail(BP_EHIP, "hipEventRecord failed");
+  } catch (...) {
+    rc = fail(BP_ENOMEM, "batch set-up: host memory allocation failed");
+  }
+  std::unique_lock<std::mutex> lock(mu);
   for (Request *r : batch) {
     r->info.launch_s = t;
     r->info.batch_windows = n;
@@ -323,7 +407,12 @@ void bp_server::launch(std::vector<Request *> batch) {
     cv_load.notify_all();
     return;
   }
-  flights.push_back(std::move(f));
+  try {
+    flights.push_back(std::move(f));
+  } catch (...) {
+    lock.unlock();
+    complete(f);  // no room to keep it in flight: finish it here
+  }
 }
 
 // Called with the lock released; the flight has been removed from the list.
@@ -363,6 +452,9 @@ void bp_server::scheduler_loop() {
   for (;;) {
     if (!ready.empty() && (int)flights.size() < cfg.in_flight) {
       std::vector<Request *> batch;
+      // reserved before any request leaves `ready`, so a failed allocation
+      // leaves them all there
+      batch.reserve(std::min<size_t>(ready.size(), (size_t)cfg.max_batch));
       while (!ready.empty() && (int)batch.size() < cfg.max_batch) {
         batch.push_back(ready.front());
         ready.pop_front();
@@ -395,7 +487,7 @@ extern "C" {
 
 const char *bp_server_last_error(void) { return t_err.c_str(); }
 
-int bp_server_create(const bp_server_config *cfg, bp_server **out) {
+int bp_server_create(const bp_server_config *cfg, bp_server **out) try {
   if (!out) return fail(BP_EINVAL, "out is NULL");
   *out = nullptr;
   if (!cfg) return fail(BP_EINVAL, "NULL configuration");
@@ -405,31 +497,39 @@ int bp_server_create(const bp_server_config *cfg, bp_server **out) {
   if (cfg->hbm_budget < (1u << 20) || cfg->max_batch < 1 || cfg->max_batch > 4096 ||
       cfg->in_flight < 1 || cfg->in_flight > 4)
     return fail(BP_EINVAL, "hbm_budget >= 1 MiB, max_batch in [1, 4096], in_flight in [1, 4]");
-  auto s = new bp_server();
-  s->cfg = *cfg;
-  hipError_t e = hipGetDevice(&s->device);
-  if (e == hipSuccess) e = s->arena.init(cfg->hbm_budget);
-  if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->load_stream, hipStreamNonBlocking);
-  for (int i = 0; e == hipSuccess && i < cfg->in_flight; ++i) {
-    hipStream_t st = nullptr;
-    e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
-    if (e == hipSuccess) s->compute.push_back(st);
+  bp_server *s = new bp_server();  // a failure here reaches BP_SERVER_CATCH: nothing to free
+  hipError_t e = hipSuccess;
+  try {
+    s->cfg = *cfg;
+    s->compute.reserve((size_t)cfg->in_flight);
+    e = hipGetDevice(&s->device);
+    if (e == hipSuccess) e = s->arena.init(cfg->hbm_budget);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->load_stream, hipStreamNonBlocking);
+    for (int i = 0; e == hipSuccess && i < cfg->in_flight; ++i) {
+      hipStream_t st = nullptr;
+      e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+      if (e == hipSuccess) s->compute.push_back(st);
+    }
+    if (e == hipSuccess) {
+      s->loader = std::thread([s] { s->guarded(&bp_server::loader_loop); });
+      s->scheduler = std::thread([s] { s->guarded(&bp_server::scheduler_loop); });
+    }
+  } catch (...) {
+    bp_server_destroy(s);  // stops and joins a thread that did start
+    return fail(BP_ENOMEM, "server set-up: host memory or worker threads unavailable");
   }
   if (e != hipSuccess) {
-    const int rc = fail(e == hipErrorOutOfMemory ? BP_ENOMEM : BP_EHIP,
-                        std::string("server set-up: ") + hipGetErrorString(e));
     bp_server_destroy(s);
-    return rc;
+    return fail(e == hipErrorOutOfMemory ? BP_ENOMEM : BP_EHIP,
+                std::string("server set-up: ") + hipGetErrorString(e));
   }
-  s->loader = std::thread([s] { s->loader_loop(); });
-  s->scheduler = std::thread([s] { s->scheduler_loop(); });
   *out = s;
   return BP_OK;
-}
+} BP_SERVER_CATCH
 
 int bp_server_add_network(bp_server *s, const int *time_delays, const float *weights_sources,
                           const float *weights_phases, size_t n_sources, size_t n_stations,
-                          size_t n_channels, size_t n_phases, int *network_id) {
+                          size_t n_channels, size_t n_phases, int *network_id) try {
   if (!s || !network_id || !weights_phases) return fail(BP_EINVAL, "NULL argument");
   Network n;
   int rc = bp_network_create(time_delays, weights_sources, n_sources, n_stations, n_phases,
@@ -453,23 +553,31 @@ int bp_server_add_network(bp_server *s, const int *time_delays, const float *wei
     return fail(BP_ENOMEM, "phase weights upload failed");
   }
   std::lock_guard<std::mutex> lock(s->mu);
-  s->nets.push_back(n);
+  try {
+    s->nets.push_back(n);
+  } catch (...) {
+    (void)hipFree(n.alpha);
+    bp_network_destroy(n.net);
+    throw;
+  }
   *network_id = (int)s->nets.size() - 1;
   return BP_OK;
-}
+} BP_SERVER_CATCH
 
 int bp_server_submit(bp_server *s, int network_id, const float *waveform_features,
                      size_t n_samples, int out_of_bounds, float *beam_max, int *beam_argmax,
-                     long long *request_id) {
+                     long long *request_id) try {
   if (!s || !waveform_features || !beam_max || !beam_argmax || !request_id)
     return fail(BP_EINVAL, "NULL argument");
   if (out_of_bounds != 0 && out_of_bounds != 1)
     return fail(BP_EINVAL, "out_of_bounds must be 0 or 1");
   if (n_samples == 0 || n_samples > (1u << 30)) return fail(BP_EINVAL, "bad n_samples");
   std::lock_guard<std::mutex> lock(s->mu);
+  if (s->broken) return fail(BP_ENOMEM, "server stopped after a worker failed (host memory)");
   if (network_id < 0 || network_id >= (int)s->nets.size())
     return fail(BP_EINVAL, "unknown network " + std::to_string(network_id));
-  auto *r = new Request();
+  std::unique_ptr<Request> owned(new Request());
+  Request *r = owned.get();
   r->id = s->next_id++;
   r->net = network_id;
   r->u_host = waveform_features;
@@ -479,13 +587,19 @@ int bp_server_submit(bp_server *s, int network_id, const float *waveform_feature
   r->barg_host = beam_argmax;
   r->info.submit_s = now_s();
   s->reqs[r->id] = r;
-  s->pending.push_back(r);
+  try {
+    s->pending.push_back(r);
+  } catch (...) {
+    s->reqs.erase(r->id);
+    throw;
+  }
+  owned.release();  // owned by the server until bp_server_wait
   *request_id = r->id;
   s->cv_load.notify_all();
   return BP_OK;
-}
+} BP_SERVER_CATCH
 
-int bp_server_wait(bp_server *s, long long request_id, double timeout_s, bp_request_info *info) {
+int bp_server_wait(bp_server *s, long long request_id, double timeout_s, bp_request_info *info) try {
   if (!s) return fail(BP_EINVAL, "NULL server");
   std::unique_lock<std::mutex> lock(s->mu);
   auto it = s->reqs.find(request_id);
@@ -498,13 +612,13 @@ int bp_server_wait(bp_server *s, long long request_id, double timeout_s, bp_requ
     return 1;  // not done yet
   if (info) *info = r->info;
   const int status = r->info.status;
-  if (status != BP_OK) t_err = r->err;
+  if (status != BP_OK) fail(status, r->err);
   s->reqs.erase(it);
   delete r;
   return status;
-}
+} BP_SERVER_CATCH
 
-int bp_server_stats(bp_server *s, bp_server_counters *st) {
+int bp_server_stats(bp_server *s, bp_server_counters *st) try {
   if (!s || !st) return fail(BP_EINVAL, "NULL argument");
   std::lock_guard<std::mutex> lock(s->mu);
   st->pending = (long long)s->pending.size();
@@ -517,9 +631,9 @@ int bp_server_stats(bp_server *s, bp_server_counters *st) {
   st->arena_used = s->arena.used();
   st->arena_peak = s->arena.peak();
   return BP_OK;
-}
+} BP_SERVER_CATCH
 
-int bp_server_destroy(bp_server *s) {
+int bp_server_destroy(bp_server *s) try {
   if (!s) return BP_OK;
   {
     std::lock_guard<std::mutex> lock(s->mu);
@@ -538,6 +652,6 @@ int bp_server_destroy(bp_server *s) {
   }
   delete s;
   return BP_OK;
-}
+} BP_SERVER_CATCH
 
 }  // extern "C"
