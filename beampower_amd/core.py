@@ -59,7 +59,7 @@ GPU_SYMBOLS = (
     "bp_parity_stride", "bp_quantize_planar",
     "bp_profiling", "bp_profiling_read",
     "bp_network_beamform_max_view", "bp_network_beamform_view", "bp_network_padding",
-    "bp_network_beamform_differential",
+    "bp_network_beamform_differential", "bp_network_diff_table_bytes",
     "bp_pack_features",
     "bp_batch_create", "bp_batch_launch", "bp_batch_graph_launch", "bp_batch_tune",
     "bp_batch_get_config", "bp_batch_set_config", "bp_batch_destroy",
@@ -133,6 +133,7 @@ def _declare_gpu_extension(lib):
     lib.bp_network_beamform_view.argtypes = [vp, ct.POINTER(FeatureViewC), ct.c_int, _SZ, _SZ,
                                              vp, vp]
     lib.bp_network_beamform_differential.argtypes = [vp, ct.POINTER(FeatureViewC), vp, vp]
+    lib.bp_network_diff_table_bytes.argtypes = [vp, ct.POINTER(_SZ), ct.POINTER(_SZ), _P_I]
     lib.bp_pack_features.argtypes = [vp, _SZ, _SZ, _SZ, ct.c_int, _SZ, _SZ, vp, vp, vp]
     lib.bp_batch_create.argtypes = [ct.c_int, vp, ct.POINTER(FeatureViewC), vp, vp, vp, vp,
                                     ct.POINTER(vp)]

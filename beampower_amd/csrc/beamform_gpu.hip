@@ -124,6 +124,7 @@ using Lds = bp::StackLds<FMT, P, kKW, kW, NB, NONE>;
 // Device buffer owned by RAII.
 struct DevBuf {
   void *p = nullptr;
+  size_t bytes = 0;  // as allocated
   DevBuf() = default;
   DevBuf(const DevBuf &) = delete;
   DevBuf &operator=(const DevBuf &) = delete;
@@ -132,12 +133,18 @@ struct DevBuf {
   }
   int alloc(size_t bytes) {
     if (bytes == 0) bytes = 16;
+    if (p) {  // a buffer that is allocated again releases the old memory
+      (void)hipFree(p);
+      p = nullptr;
+      this->bytes = 0;
+    }
     hipError_t e = hipMalloc(&p, bytes);
     if (e != hipSuccess) {
       p = nullptr;
       return fail(e == hipErrorOutOfMemory ? BP_ENOMEM : BP_EHIP,
                   "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
     }
+    this->bytes = bytes;
     return BP_OK;
   }
   template <class T>
@@ -252,9 +259,12 @@ struct bp_network {
   DevBuf tau, beta, lohi, tile_lo, tile_all, tile_lw, dtiles[3];
   DevBuf sched[3], scale[3];
   // differential mode: wave-transposed tables in the caller's source order,
-  // each built on first use and kept: weights, int32 delays, paired int16 delays
+  // each built on first use and kept: weights (float32, or used bits with
+  // the one weight value diff_cw), int32 delays, paired int16 delays
   DevBuf diff_tau, diff_beta, diff_tau_pairs;
-  bool diff_ok = false, diff_beta_ok = false, diff_pairs_ok = false;
+  bool diff_ok = false, diff_pairs_ok = false;
+  int diff_wform = 0;  // weights: 0 not built, 1 float32 table, 2 used bits
+  float diff_cw = 0.0f;
   // source order: tau, beta and the plan are in position order; perm maps a
   // position to the caller's index (device and host), inv the reverse (host)
   bool permuted = false;
@@ -262,7 +272,7 @@ struct bp_network {
   std::vector<int> order, inv;
   bool sched_ok[3] = {false, false, false};
   std::vector<float> sched_scale[3];
-  std::mutex mu;  // schedule (re)builds
+  mutable std::mutex mu;  // schedule and table (re)builds
 };
 
 namespace {
@@ -1014,52 +1024,89 @@ int build_diff_tau(const int *tau, long long K, int S, int P, bool pairs, DevBuf
   return BP_OK;
 }
 
-int build_diff_beta(const float *beta, long long K, int S, int P, DevBuf &betaT,
-                    hipStream_t st) {
-  return transpose_diff_table(beta, K, S, bp::diff_padded_stations(S, P), betaT, st);
+// Weights of the differential kernel.  For 1-4 phases, when every nonzero
+// weight has the same bits (wform 2, that value in cw; no nonzero weight at
+// all counts too), the used-bit table (ceil(K/64), diff_mask_words, 64);
+// otherwise the float32 table (ceil(K/64), S_pad, 64) (wform 1).
+int build_diff_weights(const float *beta, long long K, int S, int P, DevBuf &out, int &wform,
+                       float &cw, hipStream_t st) {
+  const long long n = K * S;
+  if (P >= 1 && P <= 4 && n > 0) {
+    DevBuf bits;
+    BP_TRY(bits.alloc(2 * sizeof(unsigned)));
+    HIP_TRY(hipMemsetAsync(bits.p, 0xff, sizeof(unsigned), st));
+    HIP_TRY(hipMemsetAsync(bits.as<unsigned>() + 1, 0, sizeof(unsigned), st));
+    hipLaunchKernelGGL(bp::diff_weight_bits_kernel,
+                       dim3((unsigned)std::min<long long>(2048, (n + 255) / 256)), dim3(256), 0,
+                       st, reinterpret_cast<const unsigned *>(beta), n, bits.as<unsigned>());
+    HIP_TRY(hipGetLastError());
+    unsigned h[2] = {0u, 0u};
+    HIP_TRY(hipMemcpyAsync(h, bits.p, sizeof h, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (h[0] >= h[1]) {  // one nonzero weight value, or none
+      cw = h[0] == h[1] ? __builtin_bit_cast(float, h[0]) : 1.0f;
+      const int W = bp::diff_mask_words(S, P);
+      const long long nw = (K + bp::kDiffBlock - 1) / bp::kDiffBlock * W * bp::kDiffBlock;
+      BP_TRY(out.alloc((size_t)nw * sizeof(unsigned)));
+      hipLaunchKernelGGL(bp::diff_mask_kernel,
+                         dim3((unsigned)std::min<long long>(65536, (nw + 255) / 256)), dim3(256),
+                         0, st, beta, K, S, W, out.as<unsigned>());
+      HIP_TRY(hipGetLastError());
+      wform = 2;
+      return BP_OK;
+    }
+  }
+  BP_TRY(transpose_diff_table(beta, K, S, bp::diff_padded_stations(S, P), out, st));
+  wform = 1;
+  cw = 0.0f;
+  return BP_OK;
 }
 
 // Two-phase delays fit the paired int16 tables for windows of at most 32768
 // samples (the saturated delays then stay outside the window).
 bool diff_pairs_ok(long long P, long long T) { return P == 2 && T <= 32768; }
 
-template <int FMT>
-void launch_differential_p(const bp::StackArgs &a, const int *tauT, const float *betaT,
-                           float *beam, bool pairs, hipStream_t st) {
+template <int FMT, int PT, bool PAIRS>
+void launch_diff_form(const bp::StackArgs &a, const int *tauT, const float *wT, float *beam,
+                      bool mask, float cw, hipStream_t st) {
   const dim3 grid((unsigned)((a.K + bp::kDiffSources - 1) / bp::kDiffSources));
   const dim3 block(bp::kDiffSources);
   const int S_pad = bp::diff_padded_stations(a.S, a.P);
+  if constexpr (PT > 0) {
+    if (mask) {
+      hipLaunchKernelGGL((bp::differential_kernel<FMT, PT, PAIRS, true>), grid, block, 0, st, a,
+                         tauT, wT, S_pad, beam, cw);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((bp::differential_kernel<FMT, PT, PAIRS, false>), grid, block, 0, st, a,
+                     tauT, wT, S_pad, beam, 0.0f);
+}
+
+// wT: the weight table of build_diff_weights, used bits when `mask`.
+template <int FMT>
+void launch_differential_p(const bp::StackArgs &a, const int *tauT, const float *wT,
+                           float *beam, bool pairs, bool mask, float cw, hipStream_t st) {
   switch (a.P) {
-    case 1:
-      hipLaunchKernelGGL((bp::differential_kernel<FMT, 1>), grid, block, 0, st, a, tauT, betaT,
-                         S_pad, beam);
-      break;
+    case 1: launch_diff_form<FMT, 1, false>(a, tauT, wT, beam, mask, cw, st); break;
     case 2:
       if (pairs)
-        hipLaunchKernelGGL((bp::differential_kernel<FMT, 2, true>), grid, block, 0, st, a, tauT,
-                           betaT, S_pad, beam);
+        launch_diff_form<FMT, 2, true>(a, tauT, wT, beam, mask, cw, st);
       else
-        hipLaunchKernelGGL((bp::differential_kernel<FMT, 2>), grid, block, 0, st, a, tauT,
-                           betaT, S_pad, beam);
+        launch_diff_form<FMT, 2, false>(a, tauT, wT, beam, mask, cw, st);
       break;
-    case 3:
-      hipLaunchKernelGGL((bp::differential_kernel<FMT, 3>), grid, block, 0, st, a, tauT, betaT,
-                         S_pad, beam);
-      break;
-    case 4:
-      hipLaunchKernelGGL((bp::differential_kernel<FMT, 4>), grid, block, 0, st, a, tauT, betaT,
-                         S_pad, beam);
-      break;
-    default:
-      hipLaunchKernelGGL((bp::differential_kernel<FMT, 0>), grid, block, 0, st, a, tauT, betaT,
-                         S_pad, beam);
+    case 3: launch_diff_form<FMT, 3, false>(a, tauT, wT, beam, mask, cw, st); break;
+    case 4: launch_diff_form<FMT, 4, false>(a, tauT, wT, beam, mask, cw, st); break;
+    default: launch_diff_form<FMT, 0, false>(a, tauT, wT, beam, false, 0.0f, st);
   }
 }
 
 // Size limits of the kernel's 32-bit offsets.
 int check_differential(int fmt, long long S, long long P, long long ldf) {
   const long long sp = S * P, rows = sp * (fmt == BP_F32 ? 1 : 2);
-  if ((S + 32) * P * bp::kDiffBlock >= (1ll << 30))
+  // offsets inside a wave's block of the padded tables
+  if (S >= (1ll << 24) || P >= (1ll << 24) ||
+      (long long)bp::diff_padded_stations((int)S, (int)P) * P * bp::kDiffBlock >= (1ll << 30))
     return fail(BP_EINVAL, "n_stations * n_phases = %lld is too large for differential mode", sp);
   if (rows * ldf >= (1ll << 30))
     return fail(BP_EINVAL, "differential mode takes features under 2^30 elements (%lld rows of "
@@ -1067,15 +1114,17 @@ int check_differential(int fmt, long long S, long long P, long long ldf) {
   return BP_OK;
 }
 
-int launch_differential(int fmt, const bp::StackArgs &a, const int *tauT, const float *betaT,
-                        float *beam, bool pairs, hipStream_t st) {
+int launch_differential(int fmt, const bp::StackArgs &a, const int *tauT, const float *wT,
+                        int wform, float cw, float *beam, bool pairs, hipStream_t st) {
   if (pairs && a.P != 2) return fail(BP_EINVAL, "paired delay tables need two phases");
+  const bool mask = wform == 2;
+  if (mask && (a.P < 1 || a.P > 4)) return fail(BP_EINVAL, "used-bit weights need 1-4 phases");
   if (fmt == BP_F32)
-    launch_differential_p<BP_F32>(a, tauT, betaT, beam, pairs, st);
+    launch_differential_p<BP_F32>(a, tauT, wT, beam, pairs, mask, cw, st);
   else if (fmt == BP_F16)
-    launch_differential_p<BP_F16>(a, tauT, betaT, beam, pairs, st);
+    launch_differential_p<BP_F16>(a, tauT, wT, beam, pairs, mask, cw, st);
   else
-    launch_differential_p<BP_BF16>(a, tauT, betaT, beam, pairs, st);
+    launch_differential_p<BP_BF16>(a, tauT, wT, beam, pairs, mask, cw, st);
   HIP_TRY(hipGetLastError());
   return BP_OK;
 }
@@ -1346,6 +1395,16 @@ int bp_network_beamform_view(bp_network *net, const bp_feature_view *features,
 
 // ---- differential mode (beamform.c:338-394) -------------------------------
 
+int bp_network_diff_table_bytes(const bp_network *net, size_t *delay_bytes,
+                                size_t *weight_bytes, int *weight_form) try {
+  if (!net) return fail(BP_EINVAL, "NULL network");
+  std::lock_guard<std::mutex> lock(net->mu);
+  if (delay_bytes) *delay_bytes = net->diff_tau.bytes + net->diff_tau_pairs.bytes;
+  if (weight_bytes) *weight_bytes = net->diff_wform ? net->diff_beta.bytes : 0;
+  if (weight_form) *weight_form = net->diff_wform;
+  return BP_OK;
+} BP_CATCH
+
 int bp_network_beamform_differential(bp_network *net, const bp_feature_view *features,
                                      float *beam, void *stream) try {
   if (!net || !features || !beam) return fail(BP_EINVAL, "NULL argument");
@@ -1363,7 +1422,7 @@ int bp_network_beamform_differential(bp_network *net, const bp_feature_view *fea
   {
     std::lock_guard<std::mutex> lock(net->mu);
     bool &tau_ok = pairs ? net->diff_pairs_ok : net->diff_ok;
-    if (!tau_ok || !net->diff_beta_ok) {
+    if (!tau_ok || net->diff_wform == 0) {
       const size_t K = net->K, S = net->S, P = net->P;
       const int *tau_src = net->tau.as<int>();
       const float *beta_src = net->beta.as<float>();
@@ -1386,22 +1445,27 @@ int bp_network_beamform_differential(bp_network *net, const bp_feature_view *fea
         tau_src = tau.as<int>();
         beta_src = beta.as<float>();
       }
-      if (!net->diff_beta_ok) BP_TRY(build_diff_beta(beta_src, net->K, net->S, net->P,
-                                                     net->diff_beta, st));
+      if (net->diff_wform == 0) {
+        int wform = 0;
+        float cw = 0.0f;
+        BP_TRY(build_diff_weights(beta_src, net->K, net->S, net->P, net->diff_beta, wform, cw,
+                                  st));
+        net->diff_wform = wform;
+        net->diff_cw = cw;
+      }
       if (!tau_ok)
         BP_TRY(build_diff_tau(tau_src, net->K, net->S, net->P, pairs,
                               pairs ? net->diff_tau_pairs : net->diff_tau, st));
       // ready for calls on any stream, and before the temporaries are freed
       HIP_TRY(hipStreamSynchronize(st));
-      net->diff_beta_ok = true;
       tau_ok = true;
     }
   }
   bp::StackArgs a = make_args_view(net, *features, 0);
   a.perm = nullptr;  // tables in the caller's order
   const DevBuf &tauT = pairs ? net->diff_tau_pairs : net->diff_tau;
-  return launch_differential(fmt, a, tauT.as<int>(), net->diff_beta.as<float>(), beam, pairs,
-                             st);
+  return launch_differential(fmt, a, tauT.as<int>(), net->diff_beta.as<float>(), net->diff_wform,
+                             net->diff_cw, beam, pairs, st);
 } BP_CATCH
 
 // The CPU library's entry point (beamform_cpu.h:6, beamform.c:338-394) on
@@ -1449,10 +1513,13 @@ int beamform_differential(float *waveform_features, int *time_delays, float *wei
   a.S = (int)S;
   a.P = (int)P;
   const bool pairs = diff_pairs_ok((long long)P, (long long)T);
-  BP_TRY(build_diff_beta(beta.as<float>(), (long long)K, (int)S, (int)P, betaT, st.s));
+  int wform = 0;
+  float cw = 0.0f;
+  BP_TRY(build_diff_weights(beta.as<float>(), (long long)K, (int)S, (int)P, betaT, wform, cw,
+                            st.s));
   BP_TRY(build_diff_tau(tau.as<int>(), (long long)K, (int)S, (int)P, pairs, tauT, st.s));
-  BP_TRY(launch_differential(BP_F32, a, tauT.as<int>(), betaT.as<float>(), dbeam.as<float>(),
-                             pairs, st.s));
+  BP_TRY(launch_differential(BP_F32, a, tauT.as<int>(), betaT.as<float>(), wform, cw,
+                             dbeam.as<float>(), pairs, st.s));
   HIP_TRY(hipMemcpyAsync(beam, dbeam.p, K * sizeof(float), hipMemcpyDeviceToHost, st.s));
   HIP_TRY(hipStreamSynchronize(st.s));
   return BP_OK;

@@ -121,9 +121,15 @@ __global__ void tile_bounds_kernel(const int *__restrict__ tau,
   __syncthreads();
   if (threadIdx.x == 0) {
     int *o = tile_all + 8 * tile;
-    o[0] = s_lo;
-    o[1] = s_hi;
-    o[2] = s_maxlo;
+    // A tile none of whose sources uses a station reads no feature (every
+    // weight is 0 and every gather reads the zero block), but the LDS kernel
+    // still places its staging window at t0 + the tile's least delay: that
+    // window goes to delay 0, where the staging addresses stay in the rows
+    // (INT_MAX there would put the buffer base gigabytes past them).
+    const bool unused = s_lo == INT_MAX;
+    o[0] = unused ? 0 : s_lo;
+    o[1] = unused ? 0 : s_hi;
+    o[2] = unused ? 0 : s_maxlo;
     o[3] = s_spread;
     o[4] = s_nonfinite;
     o[5] = o[6] = o[7] = 0;
@@ -1278,13 +1284,76 @@ __global__ void diff_pack_pairs_kernel(const unsigned *__restrict__ in, long lon
   }
 }
 
+// Uniform weights.  When every nonzero weight of a network has the same
+// bits (all ones, or any 0/1 station mask), the weight of (source, station)
+// is that value c where the station is used (beta != 0) and 0 elsewhere:
+// one bit instead of a float32.  The kernel then applies the same weight c
+// (the same FMA chain) from a bit table, 1/32 of the bytes.
+//
+// diff_weight_bits_kernel: min (out[0]) and max (out[1]) of the bit
+// patterns of the nonzero weights (-0.0 is zero); the host initialises
+// out to {0xffffffff, 0}.  Equal bounds: uniform.  No nonzero weight
+// leaves out[0] > out[1].
+__global__ __launch_bounds__(256) void diff_weight_bits_kernel(const unsigned *__restrict__ beta,
+                                                               long long n,
+                                                               unsigned *__restrict__ out) {
+  unsigned lo = 0xffffffffu, hi = 0u;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const unsigned v = beta[i];
+    if (__uint_as_float(v) != 0.0f) {
+      lo = min(lo, v);
+      hi = max(hi, v);
+    }
+  }
+  // wave reduction, then one atomic per wave
+  for (int off = 32; off > 0; off >>= 1) {
+    lo = min(lo, (unsigned)__shfl_xor((int)lo, off, 64));
+    hi = max(hi, (unsigned)__shfl_xor((int)hi, off, 64));
+  }
+  if (threadIdx.x % 64 == 0) {
+    if (lo != 0xffffffffu) atomicMin(out, lo);
+    if (hi != 0u) atomicMax(out + 1, hi);
+  }
+}
+
+// (K, S) row-major weights -> wave-transposed used-bit words
+// (ceil(K/64), W, 64): bit (s % 32) of word s / 32 of source k is
+// beta[k, s] != 0; bits past station S and sources past K are 0.  One
+// thread per output word, the grid striding over them.
+__global__ __launch_bounds__(256) void diff_mask_kernel(const float *__restrict__ beta,
+                                                        long long K, int S, int W,
+                                                        unsigned *__restrict__ out) {
+  const long long n = (K + 63) / 64 * W * 64;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long row = i / 64, lane = i - row * 64;  // row = b * W + w
+    const long long b = row / W, w = row - b * W;
+    const long long k = b * 64 + lane;
+    unsigned m = 0u;
+    if (k < K) {
+      const int s0 = (int)w * 32, s1 = min(S, s0 + 32);
+      const float *r = beta + k * S;
+      for (int s = s0; s < s1; ++s) m |= (r[s] != 0.0f ? 1u : 0u) << (s - s0);
+    }
+    out[i] = m;
+  }
+}
+
 // Stations per chunk of the differential kernel for P phases (1-4), and the
 // station count of its tables: S rounded up to two chunks, plus one chunk
-// that the last prefetch reads.  Padding stations have zero weights.
+// that the last prefetch reads.  Padding stations have zero weights.  A
+// chunk's 16 gathers are in flight together (32 measured 8 % slower at
+// 1,225 pairs x 192,000 sources; so did an XCD-grouped block order, +0 %).
 __host__ __device__ constexpr int diff_chunk_stations(int P) { return P >= 1 && P <= 4 ? 16 / P : 1; }
 __host__ __device__ inline int diff_padded_stations(int S, int P) {
   const int cs = diff_chunk_stations(P);
   return (S + 2 * cs - 1) / (2 * cs) * (2 * cs) + cs;
+}
+// Words per source of the used-bit table: the padded stations, and one more
+// word that a chunk straddling a word boundary reads (5 stations per chunk).
+__host__ __device__ inline int diff_mask_words(int S, int P) {
+  return (diff_padded_stations(S, P) + 31) / 32 + 1;
 }
 
 // Feature element t of planar row `row` (= s * P + p): float32 rows, or
@@ -1312,12 +1381,17 @@ __device__ __forceinline__ float feature_row(const StackArgs &a, unsigned row, u
 // PAIRS (two phases only): the delays come as one word of two int16 per
 // (source, station) (diff_pack_pairs_kernel), one load instead of two and
 // 8 instead of 12 table bytes per (source, station) with the weight.
-template <int FMT, int PT, bool PAIRS = false>
+//
+// MASK (1-4 phases): uniform weights; betaT is the used-bit table of
+// diff_mask_kernel, and a used station's weight is cw.
+template <int FMT, int PT, bool PAIRS = false, bool MASK = false>
 __global__ __launch_bounds__(kDiffSources) void differential_kernel(
     StackArgs a, const int *__restrict__ tauT, const float *__restrict__ betaT, int S_pad,
-    float *__restrict__ beam) {
+    float *__restrict__ beam, float cw = 0.0f) {
   static_assert(!PAIRS || PT == 2, "paired delays are for two phases");
+  static_assert(!MASK || PT > 0, "used-bit weights are for 1-4 phases");
   constexpr int CS = diff_chunk_stations(PT);  // stations per chunk (PT > 0)
+  static_assert(!MASK || CS <= 32, "a chunk's used bits span two words at most");
   constexpr int NE = PT > 0 ? CS * PT : 1;
   const int lane = threadIdx.x % kDiffBlock;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kDiffBlock);
@@ -1329,6 +1403,8 @@ __global__ __launch_bounds__(kDiffSources) void differential_kernel(
   // 32-bit offsets inside the wave's block: 64 * S_pad * P < 2^30 (checked by the host)
   const int *tb = tauT + blk * S_pad * (PAIRS ? 1 : P) * kDiffBlock + lane;
   const float *wb = betaT + blk * S_pad * kDiffBlock + lane;
+  const unsigned *mb = reinterpret_cast<const unsigned *>(betaT) +
+                       blk * diff_mask_words(S, P) * kDiffBlock + lane;
   float acc = 0.0f;
   bool used = false, inside = true;
 
@@ -1367,9 +1443,22 @@ __global__ __launch_bounds__(kDiffSources) void differential_kernel(
     float b[CS];
   };
   auto load = [&](int s0, Chunk &ch) {
+    // used bits of the chunk's stations: one word, or two when the chunk
+    // straddles a word boundary (chunks start at multiples of CS)
+    unsigned m0 = 0u, m1 = 0u;
+    if (MASK) {
+      m0 = mb[(unsigned)(s0 >> 5) * kDiffBlock];
+      if (32 % CS != 0) m1 = mb[(unsigned)((s0 >> 5) + 1) * kDiffBlock];
+    }
 #pragma unroll
     for (int c = 0; c < CS; ++c) {
-      ch.b[c] = wb[(unsigned)(s0 + c) * kDiffBlock];
+      if (MASK) {
+        const int sc = s0 + c;
+        const unsigned m = (32 % CS == 0 || (sc >> 5) == (s0 >> 5)) ? m0 : m1;
+        ch.b[c] = ((m >> (sc & 31)) & 1u) ? cw : 0.0f;
+      } else {
+        ch.b[c] = wb[(unsigned)(s0 + c) * kDiffBlock];
+      }
       if (PAIRS) {
         const unsigned w = (unsigned)tb[(unsigned)(s0 + c) * kDiffBlock];
         ch.d[c * PT] = (int)(short)(w & 0xffffu);

@@ -85,6 +85,18 @@ class Network:
               "bp_network_source_order", self.lib)
         return bool(permuted.value), order
 
+    def diff_tables(self):
+        """Device bytes of the differential tables built so far:
+        dict(delay_bytes, weight_bytes, weight_form), weight_form "none"
+        (not built), "float32", or "used_bits" (every nonzero weight has the
+        same value: one bit per source and station)."""
+        d, w, f = ct.c_size_t(), ct.c_size_t(), ct.c_int()
+        check(self.lib.bp_network_diff_table_bytes(self._h, ct.byref(d), ct.byref(w),
+                                                   ct.byref(f)),
+              "bp_network_diff_table_bytes", self.lib)
+        return dict(delay_bytes=d.value, weight_bytes=w.value,
+                    weight_form=("none", "float32", "used_bits")[f.value])
+
     def padding(self):
         """(pad_before, pad_after): zero padding of feature rows under which
         no LDS tile of this network takes the guarded edge path."""

@@ -183,7 +183,7 @@ def differential_workload(name, seed, n_stations, box_km, grid_shape, margin=10,
     tt = build_time_delays(sources_km, stations_km, (VP_KM_S, VS_KM_S), SAMPLING_RATE_HZ,
                            relative=False)
     i, j = np.triu_indices(n_stations, k=1)
-    diff = (tt[:, j, :] - tt[:, i, :]).astype(np.int32)
+    diff = np.ascontiguousarray((tt[:, j, :] - tt[:, i, :]).astype(np.int32))
     half = int(np.abs(diff).max()) + margin
     T = 2 * half + 1
     rng = np.random.default_rng(seed + 1)

@@ -348,9 +348,13 @@ def bench_differential(args, log):
     ms = e0.elapsed_time(e1) / n
     recovered = int(int(beam.argmax().item()) == w.events[0][0])
     # table bytes the kernel streams: two-phase delays of windows up to 32768
-    # samples as one word of two int16 per (source, station pair), else int32
+    # samples as one word of two int16 per (source, station pair), else int32;
+    # weights as float32, or one used bit each when every nonzero weight has
+    # the same value (the form the library chose is read back)
+    tables = net.diff_tables()
     pairs = P == 2 and T <= 32768
-    table_bytes = K * S * (4 if pairs else P * 4) + K * S * 4 + K * 4
+    weight_bytes = K * S / 8 if tables["weight_form"] == "used_bits" else K * S * 4
+    table_bytes = K * S * (4 if pairs else P * 4) + weight_bytes + K * 4
     feature_bytes = S * P * T * 4
     achieved = (table_bytes + feature_bytes) / (ms * 1e-3)
     out = {
@@ -362,6 +366,8 @@ def bench_differential(args, log):
                      "frac_of_measured_stream": achieved / HBM_READ_MEASURED,
                      "kernel": "differential_kernel",
                      "delay_table": "int16 pairs" if pairs else "int32",
+                     "weight_table": tables["weight_form"],
+                     "resident_table_bytes": tables["delay_bytes"] + tables["weight_bytes"],
                      "algorithmic_bytes_per_launch": table_bytes + feature_bytes},
     }
     log(f"differential: {ms:.3f} ms per step, {achieved / 1e9:.0f} GB/s")

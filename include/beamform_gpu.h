@@ -192,6 +192,15 @@ int bp_network_beamform_view(bp_network *net, const bp_feature_view *features,
 int bp_network_beamform_differential(bp_network *net, const bp_feature_view *features,
                                      float *beam, void *stream);
 
+/* Device bytes of a network's differential tables, built on the first
+ * differential call (0 before): delays (int32 and/or paired int16) and
+ * weights.  weight_form: 0 not built, 1 float32 weights, 2 used bits with
+ * one weight value (every nonzero weight of the network has the same bits;
+ * 1-4 phases).  Any output pointer may be NULL.  Extension, no reference
+ * counterpart. */
+int bp_network_diff_table_bytes(const bp_network *net, size_t *delay_bytes,
+                                size_t *weight_bytes, int *weight_form);
+
 /* Zero padding (samples) a feature view of this network needs before and
  * after every row so that no LDS tile takes the guarded edge path. */
 int bp_network_padding(const bp_network *net, size_t *pad_before, size_t *pad_after);

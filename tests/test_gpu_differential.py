@@ -178,3 +178,62 @@ def test_delay_table_widths_around_the_int16_limit(gpu_lib, T):
                                                 flavour="fma")
         Fd = torch.from_numpy(np.ascontiguousarray(Fw.transpose(0, 2, 1))).cuda()
         assert bits_equal(net.beamform_differential(Fd).cpu().numpy(), refw)
+
+
+def weights_of_kind(kind, rng, K, S):
+    """Source weights whose nonzero entries share one value (the used-bit
+    table) or not (the float32 table)."""
+    if kind == "ones":
+        return np.ones((K, S), np.float32)
+    if kind == "all_zero":
+        return np.zeros((K, S), np.float32)
+    beta = np.full((K, S), 1.0 if kind == "mask01" else 0.37, np.float32)
+    beta[rng.random((K, S)) < 0.3] = 0.0
+    beta[rng.random((K, S)) < 0.05] = -0.0  # negative zero: unused
+    beta[1, :] = 0.0                        # a source with no used station
+    if kind == "two_values":
+        beta[3, 5] = 2.0
+    return beta
+
+
+FORM_OF_KIND = {"ones": "used_bits", "mask01": "used_bits", "const": "used_bits",
+                "all_zero": "used_bits", "two_values": "float32"}
+
+
+@pytest.mark.parametrize("kind", sorted(FORM_OF_KIND))
+@pytest.mark.parametrize("P", [1, 2, 3, 4])
+def test_uniform_weights_as_used_bits(gpu_lib, kind, P):
+    """Weights whose nonzero entries share one value are read as one bit per
+    (source, station): bit-identical to the oracle through the C ABI and a
+    resident network, equal to the direct strict beams at the centre sample
+    in the 16-bit formats, and the weight table is 1/8 of the float32 one
+    or less (70 stations: three-phase chunks of 5 stations straddle 32-bit
+    words)."""
+    torch = _torch()
+    from beampower_amd import device as bpd
+    S, K, T = 70, 700, 61
+    u, alpha, tau, _ = synthetic.random_case(40 + P, S=S, C=2, P=P, T=T, K=K,
+                                             negative_delays=True)
+    beta = weights_of_kind(kind, np.random.default_rng(P), K, S)
+    F = oracle.prestack(u, alpha, flavour="fma")
+    init = np.full(K, -7.0, np.float32)
+    ref = oracle.beamform_differential(F, tau, beta, init=init, flavour="fma")
+    assert bits_equal(abi_differential(gpu_lib, F, tau, beta, init), ref)
+    net = bpd.Network(tau, beta)
+    Fd = torch.from_numpy(np.ascontiguousarray(F.transpose(0, 2, 1))).cuda()
+    b = net.beamform_differential(Fd, beam=torch.from_numpy(init).cuda())
+    assert bits_equal(b.cpu().numpy(), ref)
+    info = net.diff_tables()
+    assert info["weight_form"] == FORM_OF_KIND[kind]
+    blocks = (K + 63) // 64
+    if info["weight_form"] == "used_bits":
+        assert 0 < info["weight_bytes"] * 8 <= blocks * 64 * 4 * S
+    else:
+        assert info["weight_bytes"] >= blocks * 64 * 4 * S
+    for f in (bpd.BP_BF16, bpd.BP_F16):
+        V = bpd.pack(Fd, f, 3, 4)
+        bv = net.beamform_differential(V, beam=torch.full((K,), -7.0, device="cuda"))
+        direct = torch.full((K, T), -7.0, device="cuda")
+        net.beamform(V, 0, beam=direct)
+        col = direct[:, (T - 1) // 2].contiguous().cpu().numpy()
+        assert bits_equal(bv.cpu().numpy(), col), f

@@ -211,3 +211,46 @@ def test_three_and_four_phases_in_every_tile_class(gpu_lib, P):
         assert bits_equal(m, mo) and np.array_equal(a, ao), (P, oob)
         b = abi_none(gpu_lib, F, tau, beta, oob)
         assert bits_equal(b, oracle.beamform(F, tau, beta, oob)), (P, oob)
+
+
+@pytest.mark.parametrize("zero", ["some_tiles", "every_tile"])
+def test_tiles_whose_sources_use_no_station(gpu_lib, zero):
+    """Whole source tiles with all-zero weights (no source of the tile uses
+    a station): they read no feature, yet still stage a window; its start
+    must stay inside the rows.  Bit-identical to the oracle through the C
+    ABI and a resident network in the caller's order (tiles of consecutive
+    sources), with padded rows (fast staging) and unpadded ones (guarded
+    staging); in bfloat16 storage the rows of those sources are 0."""
+    import torch
+    from beampower_amd import device as bpd
+    from beampower_amd import load_library
+    tile_c = ct.c_int()
+    assert load_library("gpu").bp_kernel_shape(ct.byref(tile_c), None, None) == 0
+    tile = tile_c.value
+    u, alpha, tau, beta = synthetic.random_case(61, S=9, C=2, P=2, T=1300, K=8 * tile,
+                                                tau_range=(0, 60))
+    zero_rows = slice(tile, 4 * tile) if zero == "some_tiles" else slice(None)
+    beta[zero_rows] = 0.0
+    F = oracle.prestack(u, alpha)
+    K, T = tau.shape[0], F.shape[1]
+    net = bpd.Network(tau, beta, order="natural")
+    Fd = torch.from_numpy(np.ascontiguousarray(F.transpose(0, 2, 1))).cuda()
+    for oob in (0, 1):
+        mo, ao = oracle.beamform_max(F, tau, beta, oob)
+        m, a = abi_max(gpu_lib, F, tau, beta, oob)
+        assert bits_equal(m, mo) and np.array_equal(a, ao), oob
+        bo = oracle.beamform(F, tau, beta, oob)
+        assert bits_equal(abi_none(gpu_lib, F, tau, beta, oob), bo), oob
+        for pad in (net.padding(), (0, 0)):
+            V = bpd.pack(Fd, bpd.BP_F32, *pad)
+            m, a = net.beamform_max(V, oob)
+            assert bits_equal(m.cpu().numpy(), mo), (oob, pad)
+            assert np.array_equal(a.cpu().numpy(), ao), (oob, pad)
+            b = net.beamform(V, oob).cpu().numpy()
+            assert bits_equal(b, bo), (oob, pad)
+    for pad in (net.padding(), (0, 0)):
+        V = bpd.pack(Fd, bpd.BP_BF16, *pad)
+        b = net.beamform(V, 1).cpu().numpy()
+        assert not np.any(b[zero_rows]), pad
+        net.beamform_max(V, 1)
+    torch.cuda.synchronize()
