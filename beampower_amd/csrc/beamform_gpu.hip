@@ -1048,9 +1048,10 @@ int build_diff_weights(const float *beta, long long K, int S, int P, DevBuf &out
       const int W = bp::diff_mask_words(S, P);
       const long long nw = (K + bp::kDiffBlock - 1) / bp::kDiffBlock * W * bp::kDiffBlock;
       BP_TRY(out.alloc((size_t)nw * sizeof(unsigned)));
+      // one wave per (two sources, word) item: nw / 128 items, 4 waves a group
       hipLaunchKernelGGL(bp::diff_mask_kernel,
-                         dim3((unsigned)std::min<long long>(65536, (nw + 255) / 256)), dim3(256),
-                         0, st, beta, K, S, W, out.as<unsigned>());
+                         dim3((unsigned)std::min<long long>(65536, (nw / 128 + 3) / 4)),
+                         dim3(256), 0, st, beta, K, S, W, out.as<unsigned>());
       HIP_TRY(hipGetLastError());
       wform = 2;
       return BP_OK;

@@ -1319,24 +1319,27 @@ __global__ __launch_bounds__(256) void diff_weight_bits_kernel(const unsigned *_
 
 // (K, S) row-major weights -> wave-transposed used-bit words
 // (ceil(K/64), W, 64): bit (s % 32) of word s / 32 of source k is
-// beta[k, s] != 0; bits past station S and sources past K are 0.  One
-// thread per output word, the grid striding over them.
+// beta[k, s] != 0; bits past station S and sources past K are 0.  Each
+// wave item is two sources and one word: lanes 0-31 read 32 consecutive
+// weights of the first source, lanes 32-63 of the second (coalesced row
+// pieces), and a ballot forms both words.  Waves stride over the items.
 __global__ __launch_bounds__(256) void diff_mask_kernel(const float *__restrict__ beta,
                                                         long long K, int S, int W,
                                                         unsigned *__restrict__ out) {
-  const long long n = (K + 63) / 64 * W * 64;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x) {
-    const long long row = i / 64, lane = i - row * 64;  // row = b * W + w
-    const long long b = row / W, w = row - b * W;
-    const long long k = b * 64 + lane;
-    unsigned m = 0u;
-    if (k < K) {
-      const int s0 = (int)w * 32, s1 = min(S, s0 + 32);
-      const float *r = beta + k * S;
-      for (int s = s0; s < s1; ++s) m |= (r[s] != 0.0f ? 1u : 0u) << (s - s0);
-    }
-    out[i] = m;
+  const int lane = threadIdx.x & 63;
+  const long long n_src = (K + 63) / 64 * 64;  // every word of the last block is written
+  const long long n_items = n_src / 2 * W;
+  const long long n_waves = ((long long)gridDim.x * blockDim.x) >> 6;
+  for (long long it = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6; it < n_items;
+       it += n_waves) {
+    const long long pair = it / W;
+    const int w = (int)(it - pair * W);
+    const long long k = 2 * pair + (lane >> 5);
+    const int s = w * 32 + (lane & 31);
+    const bool used = k < K && s < S && beta[k * S + s] != 0.0f;
+    const unsigned long long m = __ballot(used);
+    if ((lane & 31) == 0)
+      out[((k >> 6) * W + w) * 64 + (k & 63)] = (unsigned)(m >> (lane & 32));
   }
 }
 
