@@ -10,11 +10,13 @@ namespace bp {
 constexpr int kWave = 64;
 
 // Order-preserving map of a float onto uint32 (larger float -> larger key).
-// Beams reaching the reduction are never -0.0 (an FP32 sum started at +0.0
-// cannot become -0.0 under round-to-nearest), so no zero canonicalisation
-// is needed.
+// -0.0 maps to the key of +0.0: the two compare equal under the
+// reference's '>' (beamform.c:326), so a tie between them must go to the
+// lower source index like any other tie.  (A beam can be -0.0: an
+// underflowing negative product, fmaf(-1e-30f, 1e-30f, +0.0f), gives it.)
 __device__ __forceinline__ uint32_t float_order(float v) {
   uint32_t b = __float_as_uint(v);
+  b = b == 0x80000000u ? 0u : b;
   return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
 }
 
@@ -23,15 +25,26 @@ __device__ __forceinline__ float float_unorder(uint32_t o) {
   return __uint_as_float(b);
 }
 
-// 64-bit reduction key: value in the high word, inverted source index in the
-// low word, so an unsigned max picks the largest value and, among equal
-// values, the lowest source index -- the same winner as the reference's
-// ascending scan with a strict '>' (beamform.c:320-331).  Key 0 means
-// "no candidate" (candidates satisfy v > -FLT_MAX, whose order word is
-// >= 0x00800000).
+// 64-bit reduction key: value order in the high word; in the low word the
+// inverted source index (bits 31..1) and whether the value is -0.0 (bit 0).
+// An unsigned max picks the largest value and, among equal values (+0.0 and
+// -0.0 included), the lowest source index -- the same winner as the
+// reference's ascending scan with a strict '>' (beamform.c:320-331) -- and
+// bit 0 then restores the winner's own zero sign.  Source indices are below
+// 2^31.  Key 0 means "no candidate" (candidates satisfy v > -FLT_MAX, whose
+// order word is >= 0x00800000).
 __device__ __forceinline__ unsigned long long pack_key(float v, int k) {
+  const uint32_t negz = __float_as_uint(v) == 0x80000000u ? 1u : 0u;
   return ((unsigned long long)float_order(v) << 32) |
-         (unsigned long long)(0xffffffffu - (uint32_t)k);
+         (unsigned long long)(((0x7fffffffu - (uint32_t)k) << 1) | negz);
+}
+
+__device__ __forceinline__ float key_value(unsigned long long key) {
+  return (key & 1ull) ? -0.0f : float_unorder((uint32_t)(key >> 32));
+}
+
+__device__ __forceinline__ int key_index(unsigned long long key) {
+  return (int)(0x7fffffffu - ((uint32_t)key >> 1));
 }
 
 __device__ __forceinline__ int uniform(int x) {

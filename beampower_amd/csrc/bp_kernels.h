@@ -180,8 +180,8 @@ __host__ __device__ __forceinline__ int window_slot_bytes(int Lw) { return (Lw +
 // copy e % SPA, e / SPA words past the lane's own word.  The weight word is
 // beta[k,s] * scale[s] as float32 bits, or for bfloat16 features its bf16
 // rounding in the high half.  Sources past K and unused stations get weight
-// 0 and the offset of the all-zero block that follows the phase slots, so
-// the kernel needs no branch for them: fma(0, 0, acc) == acc.  Tiles served
+// -0.0 and the offset of the all-zero block that follows the phase slots, so
+// the kernel needs no branch for them: fma(-0, +0, acc) == acc.  Tiles served
 // by the direct kernel (Lw = 0) get offsets 0.
 __global__ void schedule_kernel(const int *__restrict__ tau, const float *__restrict__ beta,
                                 const int *__restrict__ tile_lo,
@@ -213,7 +213,10 @@ __global__ void schedule_kernel(const int *__restrict__ tau, const float *__rest
     }
     out[p] = off;
   }
-  const float w = b * (scale ? scale[s] : 1.0f);
+  // An unused term gets weight -0.0: fma(-0, +0, acc) is acc bit for bit,
+  // a zero acc keeping its sign, as when the reference skips the station
+  // (beamform.c:61-62); with weight +0.0 an acc of -0.0 would become +0.0.
+  const float w = b != 0.0f ? b * (scale ? scale[s] : 1.0f) : -0.0f;
   out[P] = fmt == BP_BF16 ? (int)(bf16_bits(w) << 16) : __float_as_int(w);
 }
 
@@ -424,8 +427,8 @@ __global__ void finalize_max_kernel(const unsigned long long *__restrict__ keys,
   if (t >= T) return;
   unsigned long long key = keys[t];
   if (key) {
-    beam_max[t] = float_unorder((uint32_t)(key >> 32));
-    beam_argmax[t] = (int)(0xffffffffu - (uint32_t)key);
+    beam_max[t] = key_value(key);
+    beam_argmax[t] = key_index(key);
   } else {
     beam_argmax[t] = 0;
   }
@@ -611,6 +614,42 @@ struct Fmt<BP_BF16> {
   }
 };
 
+// The reference's flexible chain for one (source position k, sample t)
+// (beamform.c:74-105): used stations ascending, phases ascending, terms
+// outside [0, T) skipped, features read from HBM.  The LDS kernel instead
+// accumulates an out-of-range term as fma(w, 0, acc); that equals acc except
+// for the sign of an acc that is exactly zero (w * 0 is a signed zero), and
+// a sign difference of a zero acc disappears at the next nonzero term.  So
+// the two chains differ only in the sign of a beam that ends at zero, and
+// such beams are recomputed here.  The weight and feature are those the LDS
+// kernel applies (station scale; 16-bit samples from parity plane 0).
+template <int FMT, int P, class A>
+__device__ __forceinline__ float flexible_chain(const A &a, int k, int t) {
+  const int S = a.S;
+  const float *beta = a.beta + (size_t)k * S;
+  const int *tau = a.tau + (size_t)k * S * P;
+  float acc = 0.0f;
+  for (int s = 0; s < S; ++s) {
+    const float b0 = beta[s];
+    if (b0 == 0.0f) continue;
+    const float w = b0 * (a.scale ? a.scale[s] : 1.0f);
+    for (int p = 0; p < P; ++p) {
+      const long long tg = (long long)t + tau[s * P + p];
+      if (tg < 0 || tg >= a.T) continue;
+      const size_t row = (size_t)s * P + p;
+      float x;
+      if (FMT == BP_F32) {
+        x = static_cast<const float *>(a.F)[row * a.ldf + tg];
+      } else {
+        const unsigned short h = static_cast<const unsigned short *>(a.F)[row * 2 * a.ldf + tg];
+        x = (float)__builtin_bit_cast(_Float16, h);
+      }
+      acc = fmaf(w, x, acc);
+    }
+  }
+  return acc;
+}
+
 template <int FMT, int P, int KW, int W, int NB, bool NONE>
 struct StackLds {
   static_assert(P >= 1 && P <= 4, "the LDS kernel takes one to four phases");
@@ -621,6 +660,7 @@ struct StackLds {
   static constexpr int NC = SPA;          // shifted copies of a window
   static constexpr int NACC = BP_NACC;    // accumulators per lane and source
   static_assert(NACC == 8 || NACC == 16, "8 or 16 samples per lane");
+  static_assert(KW * NACC <= 64 && KW <= 8, "zero-beam masks: one bit per accumulator");
   static constexpr int R = NACC / SPA;    // 16-byte words per lane per term
   static constexpr int PL = ES == 4 ? 1 : 2;  // feature planes per (s, p) in HBM
   static constexpr int TT = kTimeTile;
@@ -932,7 +972,23 @@ struct StackLds {
       }
     }
 
+    // Flexible mode, time tiles where a term can fall outside [0, T): beams
+    // that end at zero take the reference's chain (flexible_chain).  The
+    // bfloat16 form is held to a tolerance, not to the bits, and skips this.
+    const bool flex_edge =
+        FMT != BP_BF16 && a.oob &&
+        ((long long)t0 + tall[0] < 0 || (long long)t0 + TT - 1 + tall[1] >= a.T);
+
     if (NONE) {
+      // bit kk * NACC + j: the beam of (kbase + kk, sample j) ended at zero
+      unsigned long long zero_beams = 0ull;
+      if (flex_edge) {
+#pragma unroll
+        for (int kk = 0; kk < KW; ++kk)
+#pragma unroll
+          for (int j = 0; j < NACC; ++j)
+            if (acc[kk][j] == 0.0f) zero_beams |= 1ull << (kk * NACC + j);
+      }
 #pragma unroll
       for (int kk = 0; kk < KW; ++kk) {
         const int k = kbase + kk;
@@ -958,6 +1014,18 @@ struct StackLds {
             if (t >= vlo && t < vhi) row[t] = acc[kk][j];
           }
         }
+      }
+      // flexible edge tiles: zero beams again from the reference's chain (the
+      // same lane stored the value above, so this store replaces it)
+      while (zero_beams) {
+        const int i = __builtin_ctzll(zero_beams);
+        zero_beams &= zero_beams - 1;
+        const int kk = i / NACC, j = i - kk * NACC;
+        const int k = kbase + kk, t = sample_of(t0, lane, j);
+        if (k >= a.K || t >= a.T) continue;
+        const int ko = a.perm ? a.perm[k] : k;
+        if (ko < a.k_begin || ko >= a.k_end) continue;
+        a.beam[(size_t)(ko - a.k_begin) * a.T + t] = flexible_chain<FMT, P>(a, k, t);
       }
     } else {
       float bv[NACC];
@@ -1030,6 +1098,35 @@ struct StackLds {
               }
             }
           }
+        }
+      }
+      if (flex_edge) {
+        // Where a lane's maximum is a zero, the winner is its first candidate
+        // (ascending position) whose beam is zero, and the winner's zero sign
+        // comes from the reference's chain.  Nibble j of `win`: 8 | kk of the
+        // winner at sample j.
+        unsigned long long win = 0ull;
+#pragma unroll
+        for (int j = 0; j < NACC; ++j) {
+          if (bv[j] != 0.0f) continue;
+          int w = -1;
+#pragma unroll
+          for (int kk = KW - 1; kk >= 0; --kk) {
+            const int k = kbase + kk;
+            if (k >= a.K) continue;
+            const int ko = source_index(a, k);
+            if (ko >= a.k_begin && ko < a.k_end && acc[kk][j] == 0.0f) w = kk;
+          }
+          if (w >= 0) win |= (unsigned long long)(8 | w) << (4 * j);
+        }
+        while (win) {
+          const int i = __builtin_ctzll(win) / 4;
+          const int kk = (int)((win >> (4 * i)) & 7ull);
+          win &= ~(15ull << (4 * i));
+          const float v = flexible_chain<FMT, P>(a, kbase + kk, sample_of(t0, lane, i));
+#pragma unroll
+          for (int j = 0; j < NACC; ++j)
+            if (j == i) bv[j] = v;
         }
       }
 #pragma unroll
@@ -1142,8 +1239,8 @@ __global__ void finalize_max_batch_kernel(const BatchKeys *w) {
        t += (long long)gridDim.x * blockDim.x) {
     const unsigned long long key = b.keys[t];
     if (key) {
-      b.beam_max[t] = float_unorder((uint32_t)(key >> 32));
-      b.beam_argmax[t] = (int)(0xffffffffu - (uint32_t)key);
+      b.beam_max[t] = key_value(key);
+      b.beam_argmax[t] = key_index(key);
     } else {
       b.beam_argmax[t] = 0;
     }

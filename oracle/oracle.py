@@ -170,6 +170,8 @@ def beamform_differential(F, tau, beta, num_threads=0, flavour="fma", init=None)
 
 
 def prestack_f64(u, alpha, num_threads=0):
+    """FP64 twin of prestack (beamform.c:108-152): (S,C,T), (S,C,P) ->
+    (S,T,P) float64, accumulated in ascending c from 0."""
     u = _c(u, np.float64)
     alpha = _c(alpha, np.float64)
     S, C, T = u.shape
@@ -220,21 +222,6 @@ def beamform_max_f64(F, tau, beta, out_of_bounds=0, t_begin=0, t_end=None, num_t
                                            _ptr(arg, _P_I), _ptr(valid, _P_U8),
                                            _ptr(scratch, _P_I))
     return mx, sec, arg, valid.astype(bool)
-
-
-def max_from_beams(beam, valid):
-    """Reference reduction rule (beamform.c:265-334) applied to given beams:
-    ascending k, strict '>' from -FLT_MAX (in the beams' own dtype)."""
-    floor = -np.finfo(np.float32).max
-    T = beam.shape[1]
-    best = np.full(T, floor, beam.dtype)
-    arg = np.zeros(T, np.int64)
-    for k in range(beam.shape[0]):
-        b = np.where(valid[k], beam[k], -np.inf)
-        upd = b > best
-        best = np.where(upd, b, best)
-        arg = np.where(upd, k, arg)
-    return best, arg, best > floor
 
 
 # ---------------------------------------------------------------------------

@@ -20,7 +20,7 @@ def pytest_configure(config):
 
 
 GOLDEN_CASES = ("random_small", "negative_delays", "ties", "nonfinite", "phases1", "phases3",
-                "plain_mini")
+                "plain_mini", "signed_zero")
 
 
 def load_golden(name):

@@ -83,6 +83,30 @@ def differential_case(seed, T, K=80, nonfinite=False):
     return dict(u=u, alpha=alpha, tau=tau, beta=beta), out
 
 
+def signed_zero_case(seed, S=5, C=2, P=2, T=64, K=40, tau_max=20):
+    """Beams that end at +0.0 or -0.0.  Features and most weights are tiny
+    (about 1e-30), so their products underflow to a zero whose sign is the
+    product's: fmaf(-1e-30f, 1e-30f, +0.0f) is -0.0.  Unused stations (0 and
+    -0.0 weights) and, in flexible mode, terms outside [0, T) are skipped by
+    the reference (beamform.c:61-62,94-99) and must keep a zero's sign; -0.0
+    and +0.0 tie under its '>' (beamform.c:326), so the lower index wins.  A
+    few sources with ordinary weights give nonzero beams beside them."""
+    rng = np.random.default_rng(seed)
+    u = rng.choice(np.array([-1e-30, 1e-30, -2e-30, 0.0, -0.0], np.float32), (S, C, T),
+                   p=(0.6, 0.1, 0.1, 0.1, 0.1))
+    # mostly negative features: ordinary weights give negative beams, so
+    # the maximum is often one of the zero beams
+    alpha = rng.choice(np.array([1.0, 0.5, 0.0], np.float32), (S, C, P))
+    tau = rng.integers(0, tau_max, (K, S, P)).astype(np.int32)
+    beta = rng.choice(np.array([1e-30, -1e-30, 0.0, -0.0], np.float32), (K, S),
+                      p=(0.6, 0.1, 0.15, 0.15))
+    beta[[0, 1, 5, 17, 30]] = rng.choice(np.array([1.0, 0.0], np.float32), (5, S))
+    beta[11] = 0.0   # a source with no used station
+    tau[23] = tau[8]  # duplicated rows: equal beams at different indices
+    beta[23] = beta[8]
+    return u, alpha, tau, beta
+
+
 def save(name, inputs, outputs, **meta):
     path = os.path.join(HERE, f"{name}.npz")
     np.savez_compressed(path, **inputs, **outputs,
@@ -93,6 +117,14 @@ def save(name, inputs, outputs, **meta):
 def main(only=None):
     if not oracle.ref_available("cmake") or not oracle.ref_available("make"):
         sys.exit("oracle/_ref is not built: run `make -C oracle ref`")
+
+    # 9. beams that end at a signed zero
+    if only in (None, "signed_zero"):
+        u, alpha, tau, beta = signed_zero_case(24)
+        save("signed_zero", dict(u=u, alpha=alpha, tau=tau, beta=beta),
+             run_case(u, alpha, tau, beta), seed=24)
+    if only == "signed_zero":
+        return
 
     # 8. mode='differential', odd and even T, non-finite values
     for name, seed, T, nf in (("differential_odd", 17, 41, False),
@@ -173,5 +205,6 @@ def main(only=None):
 
 
 if __name__ == "__main__":
-    # `make_golden.py differential` writes only the mode='differential' cases
+    # `make_golden.py differential` writes only the mode='differential' cases,
+    # `make_golden.py signed_zero` only the signed-zero case
     main(sys.argv[1] if len(sys.argv) > 1 else None)

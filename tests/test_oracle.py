@@ -170,6 +170,54 @@ def test_closed_form_prestack():
         np.testing.assert_array_equal(F, np.einsum("sct,scp->stp", u, alpha))
 
 
+# ---- FP64 twins (the ground truth of the 16-bit precision reports) ---------
+
+def test_prestack_f64_vs_bruteforce():
+    """oracle.prestack_f64 (beamform.c:108-152 in double) equals the NumPy
+    contraction sum_c alpha[s,c,p] u[s,c,t] to double rounding.  Distinct
+    dimensions (S=5, C=3, P=2, T=37) and non-symmetric values make a wrong
+    index or a transposed operand fail."""
+    rng = np.random.default_rng(91)
+    S, C, P, T = 5, 3, 2, 37
+    u = rng.standard_normal((S, C, T))
+    alpha = rng.uniform(-2, 2, (S, C, P))
+    F = oracle.prestack_f64(u, alpha)
+    assert F.shape == (S, T, P) and F.dtype == np.float64
+    np.testing.assert_allclose(F, bf.prestack(u, alpha), rtol=1e-13, atol=1e-13)
+    # each plausible mistake moves the result away from it
+    mistakes = {
+        "phases_swapped": np.einsum("sct,scp->stp", u, alpha[:, :, ::-1]),
+        "channels_reversed": np.einsum("sct,scp->stp", u[:, ::-1], alpha),
+        "last_channel_dropped": np.einsum("sct,scp->stp", u[:, :-1], alpha[:, :-1]),
+        "stations_shifted": np.einsum("sct,scp->stp", np.roll(u, 1, 0), alpha),
+        "time_shifted": np.einsum("sct,scp->stp", np.roll(u, 1, 2), alpha),
+    }
+    for name, bad in mistakes.items():
+        assert not np.allclose(F, bad, rtol=1e-6, atol=1e-6), name
+
+
+def test_prestack_f64_closed_form_and_fp32_twin():
+    """Exact inputs (small integers, alpha in {0, +-0.5, 1, 2}): the FP64
+    prestack is the exact integer sum, and equals the FP32 oracle bit for
+    bit; on the reference's golden prestacks it is within FP32 rounding."""
+    rng = np.random.default_rng(92)
+    S, C, P, T = 4, 3, 3, 50
+    u = rng.integers(-64, 64, (S, C, T)).astype(np.float64)
+    alpha = rng.choice([0.0, 0.5, -0.5, 1.0, 2.0], (S, C, P))
+    exact = np.einsum("sct,scp->stp", (u * 2).astype(np.int64),
+                      (alpha * 2).astype(np.int64)) / 4.0
+    F = oracle.prestack_f64(u, alpha)
+    assert np.array_equal(F, exact)
+    F32 = oracle.prestack(u.astype(np.float32), alpha.astype(np.float32))
+    assert np.array_equal(F32.astype(np.float64), F)
+    for case in GOLDEN_CASES:
+        g = load_golden(case)
+        F64 = oracle.prestack_f64(g["u"], g["alpha"])
+        gold = g["F_cmake"].astype(np.float64)
+        fin = np.isfinite(gold)
+        np.testing.assert_allclose(F64[fin], gold[fin], rtol=2e-6, atol=1e-6)
+
+
 def test_delay_bounds_vs_numpy():
     u, alpha, tau, beta = synthetic.random_case(21, S=6, C=1, P=3, T=10, K=40,
                                                 negative_delays=True)
@@ -270,3 +318,52 @@ def test_differential_oracle_matches_reference_build_live(seed, T):
         F = oracle.prestack(u, alpha, flavour=fl)
         br = oracle.ref_beamform_differential(F, tau, beta, num_threads=1, flavour=ref_flavour)
         assert bits_equal(oracle.beamform_differential(F, tau, beta, flavour=fl), br)
+
+
+# ---- the Makefile build's tolerance contract (tests/contract.py) -----------
+
+@pytest.mark.parametrize("case", GOLDEN_CASES)
+def test_fma_chain_meets_makefile_contract(case):
+    """The FMA chain (the pip/CMake build, which the GPU engine reproduces
+    bit for bit) against the Makefile build's golden outputs: beams within
+    1e-5 of the term scale, argmax equal where the top-two gap decides."""
+    import contract
+    g = load_golden(case)
+    Fm = g["F_make"]
+    F = oracle.prestack(g["u"], g["alpha"], flavour="fma")
+    for oob in (0, 1):
+        if f"none_{oob}_make" in g:
+            contract.check_beams(g[f"none_{oob}_make"], Fm, g["tau"], g["beta"], oob)
+            contract.check_beams(oracle.beamform(F, g["tau"], g["beta"], oob), Fm,
+                                 g["tau"], g["beta"], oob)
+        if f"max_{oob}_make" in g:
+            m, a = oracle.beamform_max(F, g["tau"], g["beta"], oob)
+            contract.check_max(m, a, Fm, g["tau"], g["beta"], oob)
+            # the checker reproduces the golden Makefile maxima themselves
+            contract.check_max(g[f"max_{oob}_make"], g[f"argmax_{oob}_make"], Fm, g["tau"],
+                               g["beta"], oob)
+
+
+def test_makefile_contract_catches_a_planted_error():
+    """A last-bit change passes; an error of 1e-4 of the term scale in one
+    beam, or a wrong argmax at a decided sample, fails."""
+    import contract
+    g = load_golden("random_small")
+    Fm, tau, beta = g["F_make"], g["tau"], g["beta"]
+    b = oracle.beamform(Fm, tau, beta, 1, flavour="nofma")
+    contract.check_beams(b, Fm, tau, beta, 1)
+    contract.check_beams(np.where(b != 0, np.nextafter(b, np.float32(np.inf)), b), Fm, tau,
+                         beta, 1)
+    scale = contract.term_scale(Fm, tau, beta, 1)
+    k, t = np.unravel_index(np.argmax(scale), scale.shape)
+    bad = b.copy()
+    bad[k, t] += np.float32(1e-4 * scale[k, t])
+    with pytest.raises(AssertionError):
+        contract.check_beams(bad, Fm, tau, beta, 1)
+    m, a = g["max_1_make"], g["argmax_1_make"]
+    frac = contract.check_max(m, a, Fm, tau, beta, 1)
+    assert frac > 0.5
+    a_bad = a.copy()
+    a_bad[np.argmax(m)] = (a[np.argmax(m)] + 1) % tau.shape[0]
+    with pytest.raises(AssertionError):
+        contract.check_max(m, a_bad, Fm, tau, beta, 1)
