@@ -86,6 +86,13 @@ def beamform(
     n_stations, _, n_samples = waveform_features.shape
     n_sources, _, n_phases = time_delays.shape
 
+    if mode in ["normal", "direct"] and reduce not in ["none", "None", None, "max"]:
+        # the reference's direct branch has no case for this reduce: after a
+        # prestack whose result goes unused it falls through to its final
+        # else (beampower.py:225-227), the mode message and 1
+        print(f"Mode should either be 'direct' or 'differential', not {mode}.")
+        return 1
+
     if mode in ["normal", "direct", "differential"]:
         F = prestack_traces(waveform_features, weights_phases, num_threads=num_threads,
                             device=device)
@@ -112,7 +119,6 @@ def beamform(
             if on_gpu:
                 check(status, "beamform_max", lib)
             return beam_max, beam_argmax
-        return None
 
     if mode == "differential":
         beam = np.zeros(n_sources, dtype=np.float32)

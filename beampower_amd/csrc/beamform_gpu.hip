@@ -243,12 +243,22 @@ size_t parity_stride(size_t T) { return (T + 1 + 7) / 8 * 8; }
 
 }  // namespace
 
+// The LDS kernel's schedule for one feature format and one set of station
+// scales: it encodes the copy layout of the format and the weights
+// beta * scale.  A schedule is built once and never rewritten, so a batch
+// or a call that holds one keeps its weights whatever scale sets the
+// network is used with afterwards, and building a new one cannot race with
+// kernels still reading another on some stream.
+struct Schedule {
+  std::vector<float> scales;  // host copy of the station scales (1 for BP_F32)
+  DevBuf sched, scale;
+};
+
 // Source tiles are split in three classes by their largest per-(s,p) delay
 // spread: class 0 uses the LDS kernel with the small window Lw[0], class 1
 // the LDS kernel with Lw[1] <= kLargeLw, class 2 the direct kernel.  The
-// LDS kernel's schedule depends on the feature format (copy layout, weight
-// encoding, per-station scale) and is built per format on first use, and
-// again when the scales change.
+// LDS kernel's schedules are built per format and scale set on first use
+// (see Schedule); the network keeps the most recently used ones.
 struct bp_network {
   int K = 0, S = 0, P = 0;
   int n_tiles = 0;
@@ -257,7 +267,8 @@ struct bp_network {
   int pad_lo = 0, pad_hi = 0;  // feature row padding that makes every LDS tile interior
   std::vector<int> tiles[3];  // ascending tile ids per class (host copy)
   DevBuf tau, beta, lohi, tile_lo, tile_all, tile_lw, dtiles[3];
-  DevBuf sched[3], scale[3];
+  // per format, most recently used first (at most kSchedulesPerFormat)
+  std::vector<std::shared_ptr<Schedule>> scheds[3];
   // differential mode: wave-transposed tables in the caller's source order,
   // each built on first use and kept: weights (float32, or used bits with
   // the one weight value diff_cw), int32 delays, paired int16 delays
@@ -270,25 +281,30 @@ struct bp_network {
   bool permuted = false;
   DevBuf perm;
   std::vector<int> order, inv;
-  bool sched_ok[3] = {false, false, false};
-  std::vector<float> sched_scale[3];
-  mutable std::mutex mu;  // schedule and table (re)builds
+  mutable std::mutex mu;  // schedule and table builds
 };
 
 namespace {
 
-bp::StackArgs make_args(const bp_network *net, const void *F, int fmt, int T, int oob) {
+// Schedules of a network per format: a fifth scale set replaces the least
+// recently used one.  Dropping a schedule that nothing else holds frees its
+// device memory, and hipFree waits for the device to be idle first (it
+// performs an implicit hipDeviceSynchronize), so no kernel still reads it.
+constexpr size_t kSchedulesPerFormat = 4;
+
+bp::StackArgs make_args(const bp_network *net, const void *F, int fmt, int T, int oob,
+                        const Schedule &sc) {
   bp::StackArgs a{};
   a.F = F;
   a.ldf = fmt == BP_F32 ? (long long)T : (long long)parity_stride(T);
-  a.scale = fmt == BP_F32 ? nullptr : net->scale[fmt].as<float>();
+  a.scale = fmt == BP_F32 ? nullptr : sc.scale.as<float>();
   a.tau = net->tau.as<int>();
   a.beta = net->beta.as<float>();
   a.src_lohi = net->lohi.as<int>();
   a.perm = net->permuted ? net->perm.as<int>() : nullptr;
   a.tile_lo = net->tile_lo.as<int>();
   a.tile_all = net->tile_all.as<int>();
-  a.sched = net->sched[fmt].as<int>();
+  a.sched = sc.sched.as<int>();
   a.T = T;
   a.K = net->K;
   a.S = net->S;
@@ -300,8 +316,9 @@ bp::StackArgs make_args(const bp_network *net, const void *F, int fmt, int T, in
 
 // Arguments for a feature view: rows of row_stride elements, sample 0 at
 // element pad_before, readable zeros around [0, T).
-bp::StackArgs make_args_view(const bp_network *net, const bp_feature_view &v, int oob) {
-  bp::StackArgs a = make_args(net, nullptr, v.fmt, (int)v.n_samples, oob);
+bp::StackArgs make_args_view(const bp_network *net, const bp_feature_view &v, int oob,
+                             const Schedule &sc) {
+  bp::StackArgs a = make_args(net, nullptr, v.fmt, (int)v.n_samples, oob, sc);
   const size_t es = v.fmt == BP_F32 ? 4 : 2;
   a.F = static_cast<const char *>(v.data) + v.pad_before * es;
   a.ldf = (long long)v.row_stride;
@@ -332,30 +349,42 @@ bool has_lds_tiles(const bp_network *net) {
   return !net->tiles[0].empty() || !net->tiles[1].empty();
 }
 
-// Build (or rebuild after a scale change) the schedule of format `fmt`.
-// scales: host pointer to n_stations floats, or NULL for 1.
-int ensure_schedule(bp_network *net, int fmt, const float *scales, hipStream_t st) {
+// The schedule of format `fmt` for the station scales `scales` (host
+// pointer to n_stations floats, or NULL for 1): found among the network's
+// recent ones or built now (synchronously on `st`, so that it is complete
+// before any stream uses it).
+int ensure_schedule(bp_network *net, int fmt, const float *scales, hipStream_t st,
+                    std::shared_ptr<Schedule> &out) {
   std::lock_guard<std::mutex> lock(net->mu);
   std::vector<float> want(net->S, 1.0f);
   if (scales) want.assign(scales, scales + net->S);
-  if (net->sched_ok[fmt] && net->sched_scale[fmt] == want) return BP_OK;
+  std::vector<std::shared_ptr<Schedule>> &list = net->scheds[fmt];
+  for (size_t i = 0; i < list.size(); ++i)
+    if (list[i]->scales == want) {
+      std::rotate(list.begin(), list.begin() + i, list.begin() + i + 1);
+      out = list[0];
+      return BP_OK;
+    }
+  auto sc = std::make_shared<Schedule>();
+  sc->scales = want;
   const long long n = (long long)net->n_tiles * net->S * kTileSources;
-  if (!net->scale[fmt].p) BP_TRY(net->scale[fmt].alloc((size_t)net->S * sizeof(float)));
-  HIP_TRY(hipMemcpyAsync(net->scale[fmt].p, want.data(), want.size() * sizeof(float),
+  BP_TRY(sc->scale.alloc((size_t)net->S * sizeof(float)));
+  HIP_TRY(hipMemcpyAsync(sc->scale.p, want.data(), want.size() * sizeof(float),
                          hipMemcpyHostToDevice, st));
   if (has_lds_tiles(net)) {
-    if (!net->sched[fmt].p) BP_TRY(net->sched[fmt].alloc((size_t)n * (net->P + 1) * sizeof(int)));
+    BP_TRY(sc->sched.alloc((size_t)n * (net->P + 1) * sizeof(int)));
     hipLaunchKernelGGL(bp::schedule_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
                        net->tau.as<int>(), net->beta.as<float>(), net->tile_lo.as<int>(),
                        net->tile_lw.as<int>(),
-                       fmt == BP_F32 ? (const float *)nullptr : net->scale[fmt].as<float>(),
-                       net->K, net->S, net->P, kTileSources, fmt, n, net->sched[fmt].as<int>());
+                       fmt == BP_F32 ? (const float *)nullptr : sc->scale.as<float>(), net->K,
+                       net->S, net->P, kTileSources, fmt, n, sc->sched.as<int>());
     HIP_TRY(hipGetLastError());
   }
   // the host vector `want` is read by the copy above: finish before returning
   HIP_TRY(hipStreamSynchronize(st));
-  net->sched_scale[fmt] = want;
-  net->sched_ok[fmt] = true;
+  if (list.size() >= kSchedulesPerFormat) list.pop_back();
+  list.insert(list.begin(), sc);
+  out = std::move(sc);
   return BP_OK;
 }
 
@@ -454,12 +483,13 @@ int network_max(bp_network *net, const bp_feature_view &v, int oob, float *bmax,
                 unsigned long long *keys, hipStream_t st) {
   const int T = (int)v.n_samples;
   const int fmt = v.fmt;
-  BP_TRY(ensure_schedule(net, fmt, fmt == BP_F32 ? nullptr : v.station_scale, st));
+  std::shared_ptr<Schedule> sc;
+  BP_TRY(ensure_schedule(net, fmt, fmt == BP_F32 ? nullptr : v.station_scale, st, sc));
   const unsigned nb = (unsigned)((T + 255) / 256);
   hipLaunchKernelGGL(bp::fill_u64_kernel, dim3(nb), dim3(256), 0, st, keys, (long long)T,
                      0ull);
   HIP_TRY(hipGetLastError());
-  bp::StackArgs a = make_args_view(net, v, oob);
+  bp::StackArgs a = make_args_view(net, v, oob, *sc);
   a.keys = keys;
   if (fmt == BP_F32)
     BP_TRY((launch_stack<BP_F32, false>(net, a, 0, net->K, st)));
@@ -680,7 +710,8 @@ int network_create(const int *tau, const float *beta, size_t K, size_t S, size_t
       net = std::move(alt);
     }
   }
-  BP_TRY(ensure_schedule(net.get(), BP_F32, nullptr, st));
+  std::shared_ptr<Schedule> sc;
+  BP_TRY(ensure_schedule(net.get(), BP_F32, nullptr, st, sc));
   out = std::move(net);
   return BP_OK;
 }
@@ -797,6 +828,9 @@ struct bp_batch {
   std::vector<bp::StackArgs> large[2];
   std::vector<bp::StackArgs> direct;  // per window with direct-kernel tiles
   std::vector<int> direct_n;
+  // the windows' schedules: held, so that later calls of their networks
+  // with other station scales leave this batch's weights as they were
+  std::vector<std::shared_ptr<Schedule>> scheds;
   DevBuf dkeys;
   hipStream_t cap = nullptr;  // private stream for graph capture
   hipGraph_t graph = nullptr;
@@ -1268,7 +1302,8 @@ int bp_network_prepare(bp_network *net, int fmt, const float *station_scale) try
   BP_TRY(check_device());
   Stream st;
   BP_TRY(st.create());
-  return ensure_schedule(net, fmt, station_scale, st.s);
+  std::shared_ptr<Schedule> sc;
+  return ensure_schedule(net, fmt, station_scale, st.s, sc);
 } BP_CATCH
 
 int bp_network_padding(const bp_network *net, size_t *pad_before, size_t *pad_after) try {
@@ -1364,10 +1399,12 @@ int bp_network_beamform(bp_network *net, const float *features, size_t n_samples
                 source_end, net->K);
   BP_TRY(check_shapes(n_samples, net->K, net->S, net->P));
   if (n_samples == 0) return BP_OK;
-  bp::StackArgs a = make_args(net, features, BP_F32, (int)n_samples, out_of_bounds);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  std::shared_ptr<Schedule> sc;
+  BP_TRY(ensure_schedule(net, BP_F32, nullptr, st, sc));
+  bp::StackArgs a = make_args(net, features, BP_F32, (int)n_samples, out_of_bounds, *sc);
   a.beam = beam;
-  return launch_stack<BP_F32, true>(net, a, (int)source_begin, (int)source_end,
-                                    static_cast<hipStream_t>(stream));
+  return launch_stack<BP_F32, true>(net, a, (int)source_begin, (int)source_end, st);
 } BP_CATCH
 
 int bp_network_beamform_view(bp_network *net, const bp_feature_view *features,
@@ -1384,8 +1421,9 @@ int bp_network_beamform_view(bp_network *net, const bp_feature_view *features,
   if (features->n_samples == 0) return BP_OK;
   hipStream_t st = static_cast<hipStream_t>(stream);
   const int fmt = features->fmt;
-  BP_TRY(ensure_schedule(net, fmt, fmt == BP_F32 ? nullptr : features->station_scale, st));
-  bp::StackArgs a = make_args_view(net, *features, out_of_bounds);
+  std::shared_ptr<Schedule> sc;
+  BP_TRY(ensure_schedule(net, fmt, fmt == BP_F32 ? nullptr : features->station_scale, st, sc));
+  bp::StackArgs a = make_args_view(net, *features, out_of_bounds, *sc);
   a.beam = beam;
   if (fmt == BP_F32)
     return launch_stack<BP_F32, true>(net, a, (int)source_begin, (int)source_end, st);
@@ -1414,7 +1452,8 @@ int bp_network_beamform_differential(bp_network *net, const bp_feature_view *fea
   hipStream_t st = static_cast<hipStream_t>(stream);
   const int fmt = features->fmt;
   BP_TRY(check_differential(fmt, net->S, net->P, (long long)features->row_stride));
-  if (fmt != BP_F32) BP_TRY(ensure_schedule(net, fmt, features->station_scale, st));
+  std::shared_ptr<Schedule> sc;
+  BP_TRY(ensure_schedule(net, fmt, fmt == BP_F32 ? nullptr : features->station_scale, st, sc));
   // The tables are in the caller's source order even when the stack plan
   // orders sources by similarity: a wave's 64 lanes are 64 sources, and on
   // a grid 64 consecutive indices gather from fewer cache lines than a
@@ -1462,7 +1501,7 @@ int bp_network_beamform_differential(bp_network *net, const bp_feature_view *fea
       tau_ok = true;
     }
   }
-  bp::StackArgs a = make_args_view(net, *features, 0);
+  bp::StackArgs a = make_args_view(net, *features, 0, *sc);
   a.perm = nullptr;  // tables in the caller's order
   const DevBuf &tauT = pairs ? net->diff_tau_pairs : net->diff_tau;
   return launch_differential(fmt, a, tauT.as<int>(), net->diff_beta.as<float>(), net->diff_wform,
@@ -1732,7 +1771,9 @@ int beamform(float *waveform_features, int *time_delays, float *weights_sources,
   hipLaunchKernelGGL(bp::fill_f32_kernel, dim3(fill_blocks), dim3(256), 0, st.s,
                      dbeam.as<float>(), n, 0.0f);
   HIP_TRY(hipGetLastError());
-  bp::StackArgs a = make_args_view(net.get(), v, out_of_bounds);
+  std::shared_ptr<Schedule> sc;
+  BP_TRY(ensure_schedule(net.get(), BP_F32, nullptr, st.s, sc));
+  bp::StackArgs a = make_args_view(net.get(), v, out_of_bounds, *sc);
   a.beam = dbeam.as<float>();
   BP_TRY((launch_stack<BP_F32, true>(net.get(), a, 0, (int)K, st.s)));
   HIP_TRY(hipMemcpyAsync(beam, dbeam.p, K * T * sizeof(float), hipMemcpyDeviceToHost, st.s));
@@ -1791,7 +1832,6 @@ int bp_batch_create(int n_windows, bp_network *const *networks,
   b->n = n_windows;
   Stream st;
   BP_TRY(st.create());
-  std::map<const bp_network *, std::vector<float>> scales_of;
   std::vector<bp::BatchKeys> kd;
   for (int i = 0; i < n_windows; ++i) {
     bp_network *net = networks[i];
@@ -1808,19 +1848,14 @@ int bp_batch_create(int n_windows, bp_network *const *networks,
     const int oob = out_of_bounds[i];
     if (oob != 0 && oob != 1)
       return fail(BP_EINVAL, "window %d: out_of_bounds must be 0 or 1, not %d", i, oob);
-    const float *sc = fv.station_scale;
-    std::vector<float> want(net->S, 1.0f);
-    if (sc) want.assign(sc, sc + net->S);
-    auto it = scales_of.find(net);
-    if (it != scales_of.end() && it->second != want)
-      return fail(BP_EINVAL, "window %d: a network's windows in one batch must share station scales", i);
-    scales_of[net] = want;
-    BP_TRY(ensure_schedule(net, fmt, sc, st.s));
+    std::shared_ptr<Schedule> sched;
+    BP_TRY(ensure_schedule(net, fmt, fmt == BP_F32 ? nullptr : fv.station_scale, st.s, sched));
+    b->scheds.push_back(sched);
     const int T = (int)fv.n_samples;
     kd.push_back(bp::BatchKeys{keys[i], beam_max[i], beam_argmax[i], T});
     b->max_T = std::max(b->max_T, T);
     if (T == 0) continue;
-    bp::StackArgs a = make_args_view(net, fv, oob);
+    bp::StackArgs a = make_args_view(net, fv, oob, *sched);
     a.keys = keys[i];
     a.k_begin = 0;
     a.k_end = net->K;

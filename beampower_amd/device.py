@@ -22,11 +22,35 @@ def _torch():
     return torch
 
 
-def _stream_ptr(stream):
+def _stream_ptr(stream, *tensors):
+    """HIP stream handle of a library call on `stream` (None = the current
+    stream).  When the call runs on another stream than the current one:
+    that stream first waits for the work already on the current stream
+    (the wrapper's contiguous copies and zero fills, and the caller's own
+    preparation of the inputs), and the tensors the call reads or writes are
+    marked as used on it (record_stream), so that torch's caching allocator
+    does not hand their memory to other work while the call's kernels may
+    still run, even if the caller drops them on return."""
     torch = _torch()
     if stream is None:
         stream = torch.cuda.current_stream()
+    elif stream != torch.cuda.current_stream():
+        stream.wait_stream(torch.cuda.current_stream())
+        for t in tensors:
+            if t is not None:
+                t.record_stream(stream)
     return ct.c_void_p(stream.cuda_stream)
+
+
+def _zeros(shape, dtype, device, stream):
+    """A zero-filled default output, filled on the stream the library call
+    runs on (a fill on the current stream would not be ordered before the
+    call's kernels on another stream)."""
+    torch = _torch()
+    if stream is None:
+        return torch.zeros(shape, dtype=dtype, device=device)
+    with torch.cuda.stream(stream):
+        return torch.zeros(shape, dtype=dtype, device=device)
 
 
 # Source order of a network's plan (bp_network_create_ordered).
@@ -116,7 +140,7 @@ class Network:
                 raise ValueError("features do not match the network")
             T, dev = features.n_samples, features.data.device
             if beam_max is None:
-                beam_max = torch.zeros(T, dtype=torch.float32, device=dev)
+                beam_max = _zeros(T, torch.float32, dev, stream)
             if beam_argmax is None:
                 beam_argmax = torch.empty(T, dtype=torch.int32, device=dev)
             if keys is None:
@@ -124,7 +148,8 @@ class Network:
             view = features.c_view()
             check(self.lib.bp_network_beamform_max_view(
                 self._h, ct.byref(view), int(out_of_bounds), beam_max.data_ptr(),
-                beam_argmax.data_ptr(), keys.data_ptr(), _stream_ptr(stream)),
+                beam_argmax.data_ptr(), keys.data_ptr(),
+                _stream_ptr(stream, features.data, beam_max, beam_argmax, keys)),
                 "bp_network_beamform_max_view", self.lib)
             return beam_max, beam_argmax
         fmt, scale = BP_F32, None
@@ -132,18 +157,19 @@ class Network:
             fmt, T, dev = features.fmt, features.n_samples, features.data.device
             S, P = features.data.shape[:2]
             scale = features.weight_scale.ctypes.data
-            ptr = features.data.data_ptr()
+            data = features.data
+            ptr = data.data_ptr()
         else:
             S, P, T = features.shape
             if features.dtype != torch.float32:
                 raise ValueError("features must be float32 (n_stations, n_phases, n_samples)")
-            features = features.contiguous()
-            dev, ptr = features.device, features.data_ptr()
+            data = features.contiguous()
+            dev, ptr = data.device, data.data_ptr()
         if (S, P) != (self.n_stations, self.n_phases):
             raise ValueError(f"features are for {(S, P)} stations/phases, network for "
                              f"{(self.n_stations, self.n_phases)}")
         if beam_max is None:
-            beam_max = torch.zeros(T, dtype=torch.float32, device=dev)
+            beam_max = _zeros(T, torch.float32, dev, stream)
         if beam_argmax is None:
             beam_argmax = torch.empty(T, dtype=torch.int32, device=dev)
         if keys is None:
@@ -151,7 +177,8 @@ class Network:
         check(self.lib.bp_network_beamform_max(
             self._h, ptr, fmt, scale, T, int(out_of_bounds),
             beam_max.data_ptr(), beam_argmax.data_ptr(), keys.data_ptr(),
-            _stream_ptr(stream)), "bp_network_beamform_max", self.lib)
+            _stream_ptr(stream, data, beam_max, beam_argmax, keys)), "bp_network_beamform_max",
+            self.lib)
         return beam_max, beam_argmax
 
     def beamform(self, features, out_of_bounds=0, source_begin=0, source_end=None,
@@ -165,20 +192,22 @@ class Network:
             source_end = self.n_sources if source_end is None else int(source_end)
             n = source_end - int(source_begin)
             if beam is None:
-                beam = torch.zeros((n, v.n_samples), dtype=torch.float32, device=v.data.device)
+                beam = _zeros((n, v.n_samples), torch.float32, v.data.device, stream)
             cv = v.c_view()
             check(self.lib.bp_network_beamform_view(
                 self._h, ct.byref(cv), int(out_of_bounds), int(source_begin), source_end,
-                beam.data_ptr(), _stream_ptr(stream)), "bp_network_beamform_view", self.lib)
+                beam.data_ptr(), _stream_ptr(stream, v.data, beam)), "bp_network_beamform_view",
+                self.lib)
             return beam
         S, P, T = features.shape
         source_end = self.n_sources if source_end is None else int(source_end)
         n = source_end - int(source_begin)
         if beam is None:
-            beam = torch.zeros((n, T), dtype=torch.float32, device=features.device)
+            beam = _zeros((n, T), torch.float32, features.device, stream)
+        data = features.contiguous()
         check(self.lib.bp_network_beamform(
-            self._h, features.contiguous().data_ptr(), T, int(out_of_bounds),
-            int(source_begin), source_end, beam.data_ptr(), _stream_ptr(stream)),
+            self._h, data.data_ptr(), T, int(out_of_bounds),
+            int(source_begin), source_end, beam.data_ptr(), _stream_ptr(stream, data, beam)),
             "bp_network_beamform", self.lib)
         return beam
 
@@ -191,10 +220,10 @@ class Network:
         torch = _torch()
         v = FeatureView.of(features)
         if beam is None:
-            beam = torch.zeros(self.n_sources, dtype=torch.float32, device=v.data.device)
+            beam = _zeros(self.n_sources, torch.float32, v.data.device, stream)
         cv = v.c_view()
         check(self.lib.bp_network_beamform_differential(self._h, ct.byref(cv), beam.data_ptr(),
-                                                        _stream_ptr(stream)),
+                                                        _stream_ptr(stream, v.data, beam)),
               "bp_network_beamform_differential", self.lib)
         return beam
 
@@ -255,9 +284,11 @@ def pack(F_spt, fmt=BP_F32, pad_before=0, pad_after=0, stream=None):
     dtype = torch.float32 if fmt == BP_F32 else torch.int16
     data = torch.empty(rows * row_stride, dtype=dtype, device=F_spt.device)
     scale = None if fmt == BP_F32 else np.empty(S, np.float32)
-    check(lib.bp_pack_features(F_spt.contiguous().data_ptr(), S, P, T, int(fmt), pad_before,
+    src = F_spt.contiguous()
+    check(lib.bp_pack_features(src.data_ptr(), S, P, T, int(fmt), pad_before,
                                row_stride, None if scale is None else scale.ctypes.data,
-                               data.data_ptr(), _stream_ptr(stream)), "bp_pack_features", lib)
+                               data.data_ptr(), _stream_ptr(stream, src, data)),
+          "bp_pack_features", lib)
     return FeatureView(data, fmt, S, P, T, row_stride, pad_before, scale)
 
 
@@ -270,8 +301,9 @@ def quantize(F_spt, fmt=BP_BF16, stream=None):
     ldq = lib.bp_parity_stride(T)
     out = torch.empty((S, P, 2, ldq), dtype=torch.int16, device=F_spt.device)
     wscale = np.empty(S, np.float32)
-    check(lib.bp_quantize_planar(F_spt.contiguous().data_ptr(), S, P, T, int(fmt),
-                                 wscale.ctypes.data, out.data_ptr(), _stream_ptr(stream)),
+    src = F_spt.contiguous()
+    check(lib.bp_quantize_planar(src.data_ptr(), S, P, T, int(fmt),
+                                 wscale.ctypes.data, out.data_ptr(), _stream_ptr(stream, src, out)),
           "bp_quantize_planar", lib)
     return QuantizedFeatures(out, wscale, int(fmt), T)
 
@@ -282,8 +314,9 @@ def interleaved_to_planar(F_stp, stream=None):
     lib = load_library("gpu")
     S, T, P = F_stp.shape
     out = torch.empty((S, P, T), dtype=torch.float32, device=F_stp.device)
-    check(lib.bp_interleaved_to_planar(F_stp.contiguous().data_ptr(), S, T, P,
-                                       out.data_ptr(), _stream_ptr(stream)),
+    src = F_stp.contiguous()
+    check(lib.bp_interleaved_to_planar(src.data_ptr(), S, T, P,
+                                       out.data_ptr(), _stream_ptr(stream, src, out)),
           "bp_interleaved_to_planar", lib)
     return out
 
@@ -295,8 +328,9 @@ def prestack_planar(u, weights_phases, stream=None):
     S, C, T = u.shape
     P = weights_phases.shape[2]
     out = torch.empty((S, P, T), dtype=torch.float32, device=u.device)
-    check(lib.bp_prestack_planar(u.contiguous().data_ptr(), weights_phases.contiguous().data_ptr(),
-                                 S, C, T, P, out.data_ptr(), _stream_ptr(stream)),
+    uc, wc = u.contiguous(), weights_phases.contiguous()
+    check(lib.bp_prestack_planar(uc.data_ptr(), wc.data_ptr(),
+                                 S, C, T, P, out.data_ptr(), _stream_ptr(stream, uc, wc, out)),
           "bp_prestack_planar", lib)
     return out
 
@@ -333,6 +367,8 @@ class Batch:
             self._keys.append(torch.empty(max(T, 1), dtype=torch.int64, device=dev))
         if len({v.fmt for v in views}) != 1:
             raise ValueError("one feature format per batch")
+        # the outputs' zero fill is on the current stream; launches may use any
+        torch.cuda.current_stream().synchronize()
         self._views = views
         vp = ct.c_void_p
         arr = lambda xs: (vp * n)(*xs)
@@ -357,12 +393,16 @@ class Batch:
     def __del__(self):
         self.close()
 
+    def _tensors(self):
+        return ([v.data for v in self._views] + self.beam_max + self.beam_argmax +
+                self._keys)
+
     def launch(self, stream=None):
-        check(self.lib.bp_batch_launch(self._h, _stream_ptr(stream)), "bp_batch_launch",
-              self.lib)
+        check(self.lib.bp_batch_launch(self._h, _stream_ptr(stream, *self._tensors())),
+              "bp_batch_launch", self.lib)
 
     def graph_launch(self, stream=None):
-        check(self.lib.bp_batch_graph_launch(self._h, _stream_ptr(stream)),
+        check(self.lib.bp_batch_graph_launch(self._h, _stream_ptr(stream, *self._tensors())),
               "bp_batch_graph_launch", self.lib)
 
     def config(self):
@@ -388,7 +428,8 @@ class Batch:
         tried = (BatchConfig * 64)()
         n, best = ct.c_int(), BatchConfig()
         check(self.lib.bp_batch_tune(self._h, int(reps), tried, 64, ct.byref(n), ct.byref(best),
-                                     _stream_ptr(stream)), "bp_batch_tune", self.lib)
+                                     _stream_ptr(stream, *self._tensors())), "bp_batch_tune",
+              self.lib)
         conv = lambda c: dict(time_group=c.time_group, stage_buffers=c.stage_buffers,
                               blocks_per_cu=c.blocks_per_cu, ms=c.ms)
         return conv(best), [conv(tried[i]) for i in range(n.value)]

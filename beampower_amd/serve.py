@@ -17,8 +17,10 @@ from .device import BP_BF16, BP_F32
 
 
 class Ticket:
-    """One submitted window: its id, host features and host outputs (kept
-    alive until the request completes)."""
+    """One submitted window: its id, host features and host outputs.  The
+    server's native threads read the features and write the outputs, so the
+    Server holds every ticket until its request has finished (wait() got
+    its final status) or the server is closed."""
 
     def __init__(self, request_id, network, features, beam_max, beam_argmax):
         self.id, self.network = request_id, network
@@ -38,6 +40,7 @@ class Server:
         self._h = h
         self.fmt = int(fmt)
         self.networks = []
+        self._pending = {}  # request id -> Ticket, until its final status
 
     def _check(self, status, what):
         if status != 0:
@@ -45,9 +48,13 @@ class Server:
             raise RuntimeError(f"{what} failed with status {status}: {msg}")
 
     def close(self):
+        """Stop the server: bp_server_destroy joins the native threads (the
+        batches in flight complete, requests not yet loaded are dropped), and
+        only then are the pending requests' buffers released."""
         if getattr(self, "_h", None):
             self.lib.bp_server_destroy(self._h)
             self._h = None
+        self._pending = {}
 
     def __del__(self):
         self.close()
@@ -83,7 +90,9 @@ class Server:
                                               int(out_of_bounds), bmax.ctypes.data,
                                               barg.ctypes.data, ct.byref(rid)),
                     "bp_server_submit")
-        return Ticket(rid.value, network, u, bmax, barg)
+        ticket = Ticket(rid.value, network, u, bmax, barg)
+        self._pending[ticket.id] = ticket
+        return ticket
 
     def wait(self, ticket, timeout=None):
         """Block until the request is done; returns (beam_max, beam_argmax),
@@ -93,6 +102,7 @@ class Server:
                                      ct.byref(info))
         if st == 1:
             raise TimeoutError(f"request {ticket.id} not done after {timeout} s")
+        self._pending.pop(ticket.id, None)  # final status: the threads are done with it
         self._check(st, "request")
         ticket.info = {f: getattr(info, f) for f, _ in RequestInfo._fields_}
         return ticket.beam_max, ticket.beam_argmax

@@ -148,7 +148,10 @@ int bp_network_classes(const bp_network *net, int *tiles_per_class, int *window_
  * keys: DEVICE scratch of n_samples uint64 (caller-provided workspace).
  * stream: hipStream_t (NULL = default stream).  Asynchronous, except that
  *   the first call with a format, or with new station_scale values, builds
- *   the network's schedule for it synchronously. */
+ *   the network's schedule for it synchronously.  A schedule, once built,
+ *   is never rewritten; a network keeps those of its four most recently
+ *   used scale sets per format, and dropping an older one frees device
+ *   memory, which waits for the device to be idle (hipFree). */
 int bp_network_beamform_max(bp_network *net, const void *features, int fmt,
                             const float *station_scale, size_t n_samples,
                             int out_of_bounds, float *beam_max, int *beam_argmax,
@@ -252,8 +255,10 @@ int bp_prestack_planar(const float *u_sct, const float *weights_phases,
  * feature format and with one number of phases.  The LDS-class tiles of
  * every window run in one persistent launch per window class; results are
  * bit-identical to bp_network_beamform_max_view on each window alone.
- * Windows of one network must share station scales.  Creation builds
- * missing schedules synchronously
+ * Windows of one network may have different station scales.  Creation
+ * builds missing schedules synchronously, and the batch holds the schedules
+ * of its windows: later calls or batches of the same networks with other
+ * station scales do not change its results
  * and uploads the window table; the networks, features and outputs must
  * outlive the batch.  bp_batch_launch enqueues the batch on `stream`
  * (no allocation, no host synchronisation); bp_batch_graph_launch captures

@@ -104,16 +104,20 @@ def test_batch_matches_windows_alone(fmt_name):
     assert_batch_equals(batch, ref, f"tuned {best}")
 
 
-def test_batch_rejects_f16_windows_with_different_scales_on_one_network():
-    _torch()
+def test_batch_f16_windows_with_different_scales_on_one_network():
+    """Two FP16 windows of one network with their own station scales in one
+    batch: each window uses its own schedule, so each equals its result
+    alone."""
+    torch = _torch()
     from beampower_amd import device as bpd
     wins = make_windows(bpd.BP_F16)
     shared = [w for w in wins if w[0] is wins[0][0]]
     assert len(shared) == 2
-    if np.array_equal(shared[0][1].weight_scale, shared[1][1].weight_scale):
-        pytest.skip("the two windows happen to share scales")
-    with pytest.raises(RuntimeError, match="share station scales"):
-        bpd.Batch(wins)
+    ref = alone(wins)
+    batch = bpd.Batch(wins)
+    batch.launch()
+    torch.cuda.synchronize()
+    assert_batch_equals(batch, ref, "f16, one network with two scale sets")
 
 
 def test_batch_f16_without_shared_networks():
@@ -157,3 +161,54 @@ def test_padded_views_match_the_plain_layout(fmt_name):
     batch.graph_launch()
     torch.cuda.synchronize()
     assert_batch_equals(batch, ref, f"padded {fmt_name}")
+
+
+def test_fp16_station_scales_per_batch_and_call():
+    """FP16 features of one network quantized with different per-station
+    scales (the weights then differ by those scales): a batch keeps its own
+    schedule, so relaunching and replaying it after batches and calls with
+    other scales -- more scale sets than the network keeps -- still gives
+    each window's result alone; a batch may mix windows of one network with
+    different scales."""
+    torch = _torch()
+    from beampower_amd import device as bpd
+    w = synthetic.batched_c4_window(300, minutes=(1.0, 1.5), stations=(12, 16),
+                                    sources=(2000, 3000))
+    net = bpd.Network(w.time_delays, w.weights_sources)
+    F = _planar(torch, bpd, w.waveform_features, w.weights_phases)
+    factors = (1.0, 37.0, 1e-3, 5e3, 0.3, 1e4, 3e-5)
+    Q = [bpd.quantize(F * f, bpd.BP_F16) for f in factors]
+    scales = {tuple(np.asarray(q.weight_scale).tolist()) for q in Q}
+    assert len(scales) == len(factors), "every factor must give its own scale set"
+
+    def alone_of(q, oob):
+        m, a = net.beamform_max(q, oob)
+        return m.cpu().numpy(), a.cpu().numpy()
+
+    ref_a = [alone_of(Q[0], 0), alone_of(Q[0], 1)]
+    batch_a = bpd.Batch([(net, Q[0], 0), (net, Q[0], 1)])
+    batch_a.launch()
+    assert_batch_equals(batch_a, ref_a, "first launch")
+    ref_b = [alone_of(Q[1], 0)]
+    batch_b = bpd.Batch([(net, Q[1], 0)])
+    batch_b.launch()
+    assert_batch_equals(batch_b, ref_b, "second batch")
+    for q in Q[2:]:  # more scale sets than the network keeps
+        net.beamform_max(q, 0)
+    torch.cuda.synchronize()
+    for what in ("relaunch", "graph", "graph again"):
+        for b in (batch_a.beam_max + batch_b.beam_max):
+            b.zero_()
+        if what == "relaunch":
+            batch_a.launch()
+            batch_b.launch()
+        else:
+            batch_a.graph_launch()
+            batch_b.graph_launch()
+        torch.cuda.synchronize()
+        assert_batch_equals(batch_a, ref_a, what)
+        assert_batch_equals(batch_b, ref_b, what)
+    mixed = bpd.Batch([(net, Q[0], 0), (net, Q[1], 0), (net, Q[3], 1)])
+    mixed.launch()
+    assert_batch_equals(mixed, [ref_a[0], ref_b[0], alone_of(Q[3], 1)], "mixed scales")
+    torch.cuda.synchronize()

@@ -69,3 +69,34 @@ def test_concurrent_calls_match_sequential(gpu_lib):
             assert bits_equal(out[0], oracle.beamform(F, tau, beta, oob))
         else:
             assert bits_equal(out[0], oracle.beamform_differential(F, tau, beta))
+
+
+def test_device_api_on_a_side_stream(gpu_lib):
+    """Device-API calls on a stream other than the current one, with the
+    wrapper's default outputs and temporaries (zero-filled on that stream,
+    marked as used there), give the current-stream results while other work
+    allocates and fills memory on the current stream."""
+    import torch
+    from beampower_amd import device as bpd
+    u, alpha, tau, beta = synthetic.random_case(90, S=10, C=3, P=2, T=6000, K=2000,
+                                                tau_range=(0, 300))
+    net = bpd.Network(tau, beta)
+    F = bpd.prestack_planar(torch.from_numpy(u).cuda(), torch.from_numpy(alpha).cuda())
+    m0, a0 = net.beamform_max(F, 1)
+    b0 = net.beamform(F, 1, 0, 300)
+    torch.cuda.synchronize()
+    side = torch.cuda.Stream()
+    outs, junk = [], []
+    for rep in range(3):
+        Ft = F.transpose(1, 2).contiguous().transpose(1, 2)  # a non-contiguous view
+        m, a = net.beamform_max(Ft, 1, stream=side)
+        b = net.beamform(Ft, 1, 0, 300, stream=side)
+        del Ft
+        junk += [torch.full((1 << 20,), 7.0, device="cuda") for _ in range(4)]
+        outs.append((m, a, b))
+    side.synchronize()
+    torch.cuda.synchronize()
+    for m, a, b in outs:
+        assert bits_equal(m.cpu().numpy(), m0.cpu().numpy())
+        assert np.array_equal(a.cpu().numpy(), a0.cpu().numpy())
+        assert bits_equal(b.cpu().numpy(), b0.cpu().numpy())

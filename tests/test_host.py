@@ -63,6 +63,16 @@ def test_bad_mode_returns_one(capsys):
     assert bp.beamform(w[0], w[2], w[1], w[3], device="gpu", mode="sideways") == 1
 
 
+def test_unknown_reduce_in_direct_mode_returns_one(capsys):
+    """The reference's direct branch has no case for another reduce and
+    falls through to its final else (beampower.py:225-227)."""
+    w = synthetic.random_case(0, S=2, C=3, P=2, T=16, K=3)
+    for mode in ("direct", "normal"):
+        assert bp.beamform(w[0], w[2], w[1], w[3], device="gpu", mode=mode,
+                           reduce="mean") == 1
+        assert f"not {mode}." in capsys.readouterr().out
+
+
 def test_differential_c_abi_checks_arguments_without_a_device():
     """beamform_differential (the CPU library's signature) validates before
     touching a device; shapes with nothing to compute leave `beam` as is."""
