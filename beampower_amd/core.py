@@ -59,6 +59,7 @@ GPU_SYMBOLS = (
     "bp_parity_stride", "bp_quantize_planar",
     "bp_profiling", "bp_profiling_read",
     "bp_network_beamform_max_view", "bp_network_beamform_view", "bp_network_padding",
+    "bp_network_beamform_max_noalloc",
     "bp_network_beamform_differential", "bp_network_diff_table_bytes",
     "bp_pack_features",
     "bp_batch_create", "bp_batch_launch", "bp_batch_graph_launch", "bp_batch_tune",

@@ -429,7 +429,7 @@ void launch_lds(int P, const bp::StackArgs &a, long long blocks, hipStream_t st)
 // tiles overlapping the range.
 template <int FMT, bool NONE>
 int launch_stack(const bp_network *net, bp::StackArgs a, int k_begin, int k_end,
-                 hipStream_t st) {
+                 hipStream_t st, bool profile = true) {
   a.k_begin = k_begin;
   a.k_end = k_end;
   if (k_end <= k_begin || a.T <= 0) return BP_OK;
@@ -447,7 +447,7 @@ int launch_stack(const bp_network *net, bp::StackArgs a, int k_begin, int k_end,
     tile1 = hi / kTileSources + 1;
   }
   hipEvent_t ev[2];
-  BP_TRY(prof_begin(st, ev));
+  if (profile) BP_TRY(prof_begin(st, ev));
   for (int cls = 0; cls < 3; ++cls) {
     const std::vector<int> &v = net->tiles[cls];
     const auto lo = std::lower_bound(v.begin(), v.end(), tile0);
@@ -475,32 +475,41 @@ int launch_stack(const bp_network *net, bp::StackArgs a, int k_begin, int k_end,
     }
     HIP_TRY(hipGetLastError());
   }
-  BP_TRY(prof_end(st, ev));
+  if (profile) BP_TRY(prof_end(st, ev));
+  return BP_OK;
+}
+
+// reduce='max' with schedule `sc` and key scratch `keys` (n_samples uint64):
+// key fill, stack kernels, decode, all on `st`.  Allocates nothing (the
+// kernel timer creates events when profiling is on and `profile` is set).
+int network_max_with(bp_network *net, const bp_feature_view &v, int oob, float *bmax,
+                     int *barg, unsigned long long *keys, hipStream_t st, const Schedule &sc,
+                     bool profile) {
+  const int T = (int)v.n_samples;
+  const int fmt = v.fmt;
+  const unsigned nb = (unsigned)((T + 255) / 256);
+  hipLaunchKernelGGL(bp::fill_u64_kernel, dim3(nb), dim3(256), 0, st, keys, (long long)T,
+                     0ull);
+  HIP_TRY(hipGetLastError());
+  bp::StackArgs a = make_args_view(net, v, oob, sc);
+  a.keys = keys;
+  if (fmt == BP_F32)
+    BP_TRY((launch_stack<BP_F32, false>(net, a, 0, net->K, st, profile)));
+  else if (fmt == BP_F16)
+    BP_TRY((launch_stack<BP_F16, false>(net, a, 0, net->K, st, profile)));
+  else
+    BP_TRY((launch_stack<BP_BF16, false>(net, a, 0, net->K, st, profile)));
+  hipLaunchKernelGGL(bp::finalize_max_kernel, dim3(nb), dim3(256), 0, st, keys, T, bmax,
+                     barg);
+  HIP_TRY(hipGetLastError());
   return BP_OK;
 }
 
 int network_max(bp_network *net, const bp_feature_view &v, int oob, float *bmax, int *barg,
                 unsigned long long *keys, hipStream_t st) {
-  const int T = (int)v.n_samples;
-  const int fmt = v.fmt;
   std::shared_ptr<Schedule> sc;
-  BP_TRY(ensure_schedule(net, fmt, fmt == BP_F32 ? nullptr : v.station_scale, st, sc));
-  const unsigned nb = (unsigned)((T + 255) / 256);
-  hipLaunchKernelGGL(bp::fill_u64_kernel, dim3(nb), dim3(256), 0, st, keys, (long long)T,
-                     0ull);
-  HIP_TRY(hipGetLastError());
-  bp::StackArgs a = make_args_view(net, v, oob, *sc);
-  a.keys = keys;
-  if (fmt == BP_F32)
-    BP_TRY((launch_stack<BP_F32, false>(net, a, 0, net->K, st)));
-  else if (fmt == BP_F16)
-    BP_TRY((launch_stack<BP_F16, false>(net, a, 0, net->K, st)));
-  else
-    BP_TRY((launch_stack<BP_BF16, false>(net, a, 0, net->K, st)));
-  hipLaunchKernelGGL(bp::finalize_max_kernel, dim3(nb), dim3(256), 0, st, keys, T, bmax,
-                     barg);
-  HIP_TRY(hipGetLastError());
-  return BP_OK;
+  BP_TRY(ensure_schedule(net, v.fmt, v.fmt == BP_F32 ? nullptr : v.station_scale, st, sc));
+  return network_max_with(net, v, oob, bmax, barg, keys, st, *sc, true);
 }
 
 int round_up8(long long v) { return (int)((v + 7) / 8 * 8); }
@@ -1291,6 +1300,81 @@ int bp_network_beamform_max_view(bp_network *net, const bp_feature_view *feature
   if (features->n_samples == 0) return BP_OK;
   return network_max(net, *features, out_of_bounds, beam_max, beam_argmax, keys,
                      static_cast<hipStream_t>(stream));
+} BP_CATCH
+
+// ---- reduce='max' without allocation ---------------------------------------
+
+// Key scratch of bp_network_beamform_max_noalloc: one packed (value, source)
+// key per sample.  A __device__ array: every GPU the library runs on holds
+// its own copy, at its own address, until the process ends.  In host code
+// its name is only a stand-in for the symbol; the device address comes from
+// hipGetSymbolAddress (scratch_for_current_device).
+static __device__ __align__(256) unsigned long long scratch[BP_NOALLOC_MAX_SAMPLES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned long long *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's scratch, looked up on the first
+// call on that device (with that device current) and kept.
+int scratch_for_current_device(unsigned long long **out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(BP_ENODEV, "device %d: the key scratch serves devices 0..%d", dev,
+                kMaxDevices - 1);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = static_cast<unsigned long long *>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return BP_OK;
+}
+
+// An existing schedule of the network for (fmt, scales), or NULL; builds
+// nothing and allocates nothing.
+std::shared_ptr<Schedule> find_schedule(bp_network *net, int fmt, const float *scales) {
+  std::lock_guard<std::mutex> lock(net->mu);
+  for (const std::shared_ptr<Schedule> &sc : net->scheds[fmt]) {
+    bool same = true;
+    for (int s = 0; s < net->S && same; ++s)
+      same = sc->scales[s] == (scales ? scales[s] : 1.0f);
+    if (same) return sc;
+  }
+  return nullptr;
+}
+
+}  // namespace
+
+int bp_network_beamform_max_noalloc(bp_network *net, const bp_feature_view *features,
+                                    int out_of_bounds, float *beam_max, int *beam_argmax,
+                                    void *stream) try {
+  if (!net || !features || !beam_max || !beam_argmax) return fail(BP_EINVAL, "NULL argument");
+  if (out_of_bounds != 0 && out_of_bounds != 1)
+    return fail(BP_EINVAL, "out_of_bounds must be 0 (strict) or 1 (flexible), not %d",
+                out_of_bounds);
+  BP_TRY(check_view(net, *features));
+  if (features->n_samples > (size_t)BP_NOALLOC_MAX_SAMPLES)
+    return fail(BP_EINVAL,
+                "n_samples %zu exceeds the %d samples of the no-allocation key scratch "
+                "(use bp_network_beamform_max_view with a workspace)",
+                features->n_samples, BP_NOALLOC_MAX_SAMPLES);
+  if (features->n_samples == 0) return BP_OK;
+  const int fmt = features->fmt;
+  std::shared_ptr<Schedule> sc =
+      find_schedule(net, fmt, fmt == BP_F32 ? nullptr : features->station_scale);
+  if (!sc)
+    return fail(BP_EINVAL,
+                "no schedule for feature format %d with these station scales: build it first "
+                "(bp_network_prepare)", fmt);
+  unsigned long long *keys = nullptr;
+  BP_TRY(scratch_for_current_device(&keys));
+  return network_max_with(net, *features, out_of_bounds, beam_max, beam_argmax, keys,
+                          static_cast<hipStream_t>(stream), *sc, false);
 } BP_CATCH
 
 int bp_network_prepare(bp_network *net, int fmt, const float *station_scale) try {

@@ -181,6 +181,24 @@ int bp_network_beamform_max_view(bp_network *net, const bp_feature_view *feature
                                  int out_of_bounds, float *beam_max, int *beam_argmax,
                                  unsigned long long *keys, void *stream);
 
+/* reduce='max' on a feature view without allocating anything (no
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  As bp_network_beamform_max_view, but the key scratch
+ * is the library's own: a device array of BP_NOALLOC_MAX_SAMPLES packed
+ * keys (32 MiB) that each GPU holds from the first call on it until the
+ * process ends.  Every call on one GPU shares that array, so calls that
+ * overlap on one GPU (on different streams or from different threads) race
+ * and must not be made.  Fails (BP_EINVAL, nothing launched) when n_samples
+ * exceeds BP_NOALLOC_MAX_SAMPLES or when the network has no schedule yet
+ * for the view's format and station scales (build it with
+ * bp_network_prepare or any allocating call first).  Allocates no host or
+ * device memory and creates no events; asynchronous on `stream`. */
+#define BP_NOALLOC_MAX_SAMPLES 4194304
+int bp_network_beamform_max_noalloc(bp_network *net, const bp_feature_view *features,
+                                    int out_of_bounds, float *beam_max, int *beam_argmax,
+                                    void *stream);
+
 /* reduce='none' on a feature view (any format) for the sources
  * [source_begin, source_end): beam DEVICE (n, n_samples) float32; cells
  * invalid in strict mode are left untouched. */
