@@ -192,16 +192,20 @@ void oracle_beamform(const float *F, const int *tau, const float *beta,
     }
 }
 
-/* beamform.c:230-336: per-t max over sources, lowest index on ties. */
-void oracle_beamform_max(const float *F, const int *tau, const float *beta,
-                         size_t n_samples, size_t n_sources, size_t n_stations,
-                         size_t n_phases, int out_of_bounds, int num_threads,
-                         float *beam_max, int *beam_argmax, int *lohi_scratch)
+/* beamform.c:230-336: per-t max over sources, lowest index on ties, for the
+ * samples [t_begin, t_end) of the window (outputs indexed from t_begin).
+ * Each sample's maximum depends only on the whole window's features and
+ * bounds, so a sub-range gives exactly the full call's values there. */
+void oracle_beamform_max_range(const float *F, const int *tau, const float *beta,
+                               size_t n_samples, size_t n_sources, size_t n_stations,
+                               size_t n_phases, int out_of_bounds, int num_threads,
+                               long long t_begin, long long t_end, float *beam_max,
+                               int *beam_argmax, int *lohi_scratch)
 {
     int nt = nthreads_or_default(num_threads);
     oracle_delay_bounds(tau, beta, n_sources, n_stations, n_phases, nt, lohi_scratch);
 #pragma omp parallel for num_threads(nt) schedule(static)
-    for (long long t = 0; t < (long long)n_samples; t++) {
+    for (long long t = t_begin; t < t_end; t++) {
         float best = -FLT_MAX;
         int best_k = 0;
         for (size_t k = 0; k < n_sources; k++) {
@@ -221,9 +225,19 @@ void oracle_beamform_max(const float *F, const int *tau, const float *beta,
             }
         }
         if (best > -FLT_MAX)
-            beam_max[t] = best;
-        beam_argmax[t] = best_k;
+            beam_max[t - t_begin] = best;
+        beam_argmax[t - t_begin] = best_k;
     }
+}
+
+void oracle_beamform_max(const float *F, const int *tau, const float *beta,
+                         size_t n_samples, size_t n_sources, size_t n_stations,
+                         size_t n_phases, int out_of_bounds, int num_threads,
+                         float *beam_max, int *beam_argmax, int *lohi_scratch)
+{
+    oracle_beamform_max_range(F, tau, beta, n_samples, n_sources, n_stations, n_phases,
+                              out_of_bounds, num_threads, 0, (long long)n_samples, beam_max,
+                              beam_argmax, lohi_scratch);
 }
 
 /* beamform.c:338-394 (mode='differential'): one beam per source at the

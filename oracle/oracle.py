@@ -60,6 +60,9 @@ def _oracle(flavour):
                                         ct.c_int, ct.c_int, _P_F, _P_I]
         lib.oracle_beamform_max.argtypes = [_P_F, _P_I, _P_F, _SZ, _SZ, _SZ, _SZ,
                                             ct.c_int, ct.c_int, _P_F, _P_I, _P_I]
+        lib.oracle_beamform_max_range.argtypes = [_P_F, _P_I, _P_F, _SZ, _SZ, _SZ, _SZ,
+                                                  ct.c_int, ct.c_int, ct.c_longlong,
+                                                  ct.c_longlong, _P_F, _P_I, _P_I]
         lib.oracle_beamform_differential.argtypes = [_P_F, _P_I, _P_F, _SZ, _SZ, _SZ, _SZ,
                                                      ct.c_int, _P_F, _P_I]
         lib.oracle_prestack_f64.argtypes = [_P_D, _P_D, _SZ, _SZ, _SZ, _SZ, ct.c_int, _P_D]
@@ -70,7 +73,8 @@ def _oracle(flavour):
                                                 ct.c_int, ct.c_int, ct.c_longlong,
                                                 ct.c_longlong, _P_D, _P_D, _P_I, _P_U8, _P_I]
         for name in ("oracle_delay_bounds", "oracle_prestack", "oracle_beamform",
-                     "oracle_beamform_max", "oracle_beamform_differential", "oracle_prestack_f64",
+                     "oracle_beamform_max", "oracle_beamform_max_range",
+                     "oracle_beamform_differential", "oracle_prestack_f64",
                      "oracle_beamform_f64",
                      "oracle_beamform_max_f64"):
             getattr(lib, name).restype = None
@@ -133,21 +137,28 @@ def beamform(F, tau, beta, out_of_bounds=0, num_threads=0, flavour="fma", init=N
     return beam
 
 
-def beamform_max(F, tau, beta, out_of_bounds=0, num_threads=0, flavour="fma", init=None):
-    """(beam_max (T,), beam_argmax (T,)) from prestacked (S,T,P) features
-    (beamform.c:230-336).  beam_max keeps ``init`` where no source is valid."""
+def beamform_max(F, tau, beta, out_of_bounds=0, num_threads=0, flavour="fma", init=None,
+                 t_begin=0, t_end=None):
+    """(beam_max, beam_argmax) from prestacked (S,T,P) features
+    (beamform.c:230-336) for the samples [t_begin, t_end) of the window
+    (default all T).  beam_max keeps ``init`` where no source is valid."""
     F = _c(F, np.float32)
     tau = _c(tau, np.int32)
     beta = _c(beta, np.float32)
     S, T, P = F.shape
     K = tau.shape[0]
-    bmax = np.zeros(T, np.float32) if init is None else _c(init, np.float32).copy()
-    barg = np.zeros(T, np.int32)
+    t_end = T if t_end is None else int(t_end)
+    t_begin = int(t_begin)
+    if not 0 <= t_begin <= t_end <= T:
+        raise ValueError(f"sample range [{t_begin}, {t_end}) outside [0, {T}]")
+    W = t_end - t_begin
+    bmax = np.zeros(W, np.float32) if init is None else _c(init, np.float32).copy()
+    barg = np.zeros(W, np.int32)
     scratch = np.empty(2 * K, np.int32)
-    _oracle(flavour).oracle_beamform_max(_ptr(F, _P_F), _ptr(tau, _P_I), _ptr(beta, _P_F),
-                                         T, K, S, P, int(out_of_bounds), _nt(num_threads),
-                                         _ptr(bmax, _P_F), _ptr(barg, _P_I),
-                                         _ptr(scratch, _P_I))
+    _oracle(flavour).oracle_beamform_max_range(_ptr(F, _P_F), _ptr(tau, _P_I), _ptr(beta, _P_F),
+                                               T, K, S, P, int(out_of_bounds), _nt(num_threads),
+                                               t_begin, t_end, _ptr(bmax, _P_F), _ptr(barg, _P_I),
+                                               _ptr(scratch, _P_I))
     return bmax, barg
 
 

@@ -367,3 +367,24 @@ def test_makefile_contract_catches_a_planted_error():
     a_bad[np.argmax(m)] = (a[np.argmax(m)] + 1) % tau.shape[0]
     with pytest.raises(AssertionError):
         contract.check_max(m, a_bad, Fm, tau, beta, 1)
+
+
+@pytest.mark.parametrize("flavour", ["fma", "nofma"])
+def test_max_over_a_sample_range_is_the_full_calls_slice(flavour):
+    """oracle.beamform_max over [t_begin, t_end) equals the full call's
+    values there (validity uses the whole window), for both bounds modes,
+    ranges at both ends, in the middle and empty."""
+    for case in ("random_small", "negative_delays", "ties", "nonfinite", "signed_zero"):
+        g = load_golden(case)
+        F = oracle.prestack(g["u"], g["alpha"], flavour=flavour)
+        T = F.shape[1]
+        for oob in (0, 1):
+            if case == "negative_delays" and oob == 0:
+                continue
+            m, a = oracle.beamform_max(F, g["tau"], g["beta"], oob, flavour=flavour)
+            for t0, t1 in ((0, T), (0, 7), (T - 9, T), (T // 3, T // 2), (5, 5)):
+                mr, ar = oracle.beamform_max(F, g["tau"], g["beta"], oob, flavour=flavour,
+                                             t_begin=t0, t_end=t1)
+                assert bits_equal(mr, m[t0:t1]) and np.array_equal(ar, a[t0:t1]), (case, t0)
+    with pytest.raises(ValueError):
+        oracle.beamform_max(F, g["tau"], g["beta"], 0, t_begin=3, t_end=T + 1)
