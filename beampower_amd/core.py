@@ -64,7 +64,7 @@ GPU_SYMBOLS = (
     "bp_pack_features",
     "bp_batch_create", "bp_batch_launch", "bp_batch_graph_launch", "bp_batch_tune",
     "bp_batch_get_config", "bp_batch_set_config", "bp_batch_destroy",
-    "bp_network_prepare", "bp_detect_threshold", "bp_detect_peaks",
+    "bp_network_prepare", "bp_detect_threshold", "bp_detect_threshold_type", "bp_detect_peaks",
     "bp_server_create", "bp_server_add_network", "bp_server_submit", "bp_server_wait",
     "bp_server_stats", "bp_server_destroy", "bp_server_last_error",
 )
@@ -131,6 +131,8 @@ def _declare_gpu_extension(lib):
     lib.bp_network_beamform_max_view.argtypes = [vp, ct.POINTER(FeatureViewC), ct.c_int, vp,
                                                  vp, vp, vp]
     lib.bp_network_padding.argtypes = [vp, ct.POINTER(_SZ), ct.POINTER(_SZ)]
+    lib.bp_network_beamform_max_noalloc.argtypes = [vp, ct.POINTER(FeatureViewC), ct.c_int, vp,
+                                                    vp, vp]
     lib.bp_network_beamform_view.argtypes = [vp, ct.POINTER(FeatureViewC), ct.c_int, _SZ, _SZ,
                                              vp, vp]
     lib.bp_network_beamform_differential.argtypes = [vp, ct.POINTER(FeatureViewC), vp, vp]
@@ -147,6 +149,8 @@ def _declare_gpu_extension(lib):
     lib.bp_batch_destroy.argtypes = [vp]
     lib.bp_network_prepare.argtypes = [vp, ct.c_int, vp]
     lib.bp_detect_threshold.argtypes = [vp, _SZ, _SZ, ct.c_double, ct.c_double, vp, vp]
+    lib.bp_detect_threshold_type.argtypes = [vp, _SZ, _SZ, ct.c_double, ct.c_double, ct.c_int,
+                                             vp, vp]
     lib.bp_detect_peaks.argtypes = [vp, vp, _SZ, _SZ, vp, _SZ, ct.POINTER(_SZ), vp]
     lib.bp_server_create.argtypes = [ct.POINTER(ServerConfig), ct.POINTER(vp)]
     lib.bp_server_add_network.argtypes = [vp, vp, vp, vp, _SZ, _SZ, _SZ, _SZ, _P_I]

@@ -1651,9 +1651,13 @@ int beamform_differential(float *waveform_features, int *time_delays, float *wei
 
 // ---- detection consumer (notebooks/3_beampower.ipynb:497-626) ------------
 
-int bp_detect_threshold(const float *beam_max, size_t n_samples, size_t window, double overlap,
-                        double num_dev, float *threshold, void *stream) try {
+int bp_detect_threshold_type(const float *beam_max, size_t n_samples, size_t window,
+                             double overlap, double num_dev, int threshold_type,
+                             float *threshold, void *stream) try {
   if (!beam_max || !threshold) return fail(BP_EINVAL, "NULL argument");
+  if (threshold_type != BP_THRESHOLD_RMS && threshold_type != BP_THRESHOLD_MAD)
+    return fail(BP_EINVAL, "threshold_type must be BP_THRESHOLD_RMS or BP_THRESHOLD_MAD, not %d",
+                threshold_type);
   const long long n = (long long)n_samples, W = (long long)window;
   const long long shift = (long long)((1.0 - overlap) * (double)window);
   if (!size_ok(n_samples) || W < 1 || n <= W || shift < 1)
@@ -1661,24 +1665,45 @@ int bp_detect_threshold(const float *beam_max, size_t n_samples, size_t window, 
                 n_samples, window);
   hipStream_t st = static_cast<hipStream_t>(stream);
   const long long nw = (n - W) / shift + 1;
+  // per window: center and deviation (rms: mean and population standard
+  // deviation in double; mad: median and median absolute deviation, float)
+  std::vector<float> center((size_t)nw), deviation((size_t)nw);
   DevBuf stats, dthr;
-  BP_TRY(stats.alloc((size_t)nw * 2 * sizeof(double)));
+  if (threshold_type == BP_THRESHOLD_RMS) {
+    BP_TRY(stats.alloc((size_t)nw * 2 * sizeof(double)));
+    hipLaunchKernelGGL(bp::window_stats_kernel, dim3((unsigned)nw), dim3(256), 0, st, beam_max,
+                       W, shift, stats.as<double>());
+    HIP_TRY(hipGetLastError());
+    std::vector<double> h((size_t)nw * 2);
+    HIP_TRY(hipMemcpyAsync(h.data(), stats.p, h.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    for (long long j = 0; j < nw; ++j) {
+      center[j] = (float)h[2 * j];
+      deviation[j] = (float)h[2 * j + 1];
+    }
+  } else {
+    BP_TRY(stats.alloc((size_t)nw * 2 * sizeof(float)));
+    hipLaunchKernelGGL(bp::window_median_kernel, dim3((unsigned)nw), dim3(256), 0, st,
+                       beam_max, W, shift, stats.as<float>());
+    HIP_TRY(hipGetLastError());
+    std::vector<float> h((size_t)nw * 2);
+    HIP_TRY(hipMemcpyAsync(h.data(), stats.p, h.size() * sizeof(float), hipMemcpyDeviceToHost,
+                           st));
+    HIP_TRY(hipStreamSynchronize(st));
+    for (long long j = 0; j < nw; ++j) {
+      center[j] = h[2 * j];
+      deviation[j] = h[2 * j + 1];
+    }
+  }
   BP_TRY(dthr.alloc((size_t)nw * sizeof(float)));
-  hipLaunchKernelGGL(bp::window_stats_kernel, dim3((unsigned)nw), dim3(256), 0, st, beam_max,
-                     W, shift, stats.as<double>());
-  HIP_TRY(hipGetLastError());
-  std::vector<double> h((size_t)nw * 2);
-  HIP_TRY(hipMemcpyAsync(h.data(), stats.p, h.size() * sizeof(double), hipMemcpyDeviceToHost,
-                         st));
-  HIP_TRY(hipStreamSynchronize(st));
   // center + num_dev * deviation in float32, then the one-step maximum with
   // the previous and (after that) the next window (:556-560)
   std::vector<float> thr((size_t)nw);
   const float nd = (float)num_dev;
   for (long long j = 0; j < nw; ++j) {
-    const float c = (float)h[2 * j], d = (float)h[2 * j + 1];
-    const float dev = nd * d;
-    thr[j] = c + dev;
+    const float dev = nd * deviation[j];
+    thr[j] = center[j] + dev;
   }
   std::vector<float> a = thr;
   for (long long j = 1; j < nw; ++j) thr[j] = std::max(a[j - 1], a[j]);
@@ -1690,6 +1715,12 @@ int bp_detect_threshold(const float *beam_max, size_t n_samples, size_t window, 
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(st));  // thr and dthr are released on return
   return BP_OK;
+} BP_CATCH
+
+int bp_detect_threshold(const float *beam_max, size_t n_samples, size_t window, double overlap,
+                        double num_dev, float *threshold, void *stream) try {
+  return bp_detect_threshold_type(beam_max, n_samples, window, overlap, num_dev,
+                                  BP_THRESHOLD_RMS, threshold, stream);
 } BP_CATCH
 
 int bp_detect_peaks(const float *beam_max, const float *threshold, size_t n_samples,

@@ -331,6 +331,88 @@ __global__ __launch_bounds__(256) void window_stats_kernel(const float *__restri
   }
 }
 
+// "mad" flavour: per window j (samples [j*shift, j*shift + window)), the
+// median of the non-zero samples and the median of their absolute
+// deviations from it, as numpy's median computes them in float32: the middle
+// value of n, or for even n the float32 mean of the two middle values
+// ((a + b) / 2).  stats[2j] = center, stats[2j+1] = deviation (0 when the
+// window holds no non-zero sample).  Each median is an exact radix
+// selection over order-preserving keys: four passes of 8-bit digits, a
+// 256-bin histogram in LDS per pass.
+struct MedianScratch {
+  unsigned hist[256];
+  unsigned digit, rank;
+  unsigned count;
+};
+
+// Key of sample v for the selection: its value (mode 0) or |v - center|
+// (mode 1); exact zeros take no part (they return false).
+__device__ __forceinline__ bool median_key(float v, int mode, float center, unsigned &key) {
+  if (v == 0.0f) return false;
+  key = float_order(mode ? fabsf(v - center) : v);
+  return true;
+}
+
+// The rank-th smallest key (0-based) among the window's selected samples.
+__device__ unsigned select_rank(const float *__restrict__ w, long long n, unsigned rank,
+                                int mode, float center, MedianScratch &sm) {
+  const int tid = threadIdx.x;
+  unsigned prefix = 0u, mask = 0u;
+  for (int shift = 24; shift >= 0; shift -= 8) {
+    for (int b = tid; b < 256; b += blockDim.x) sm.hist[b] = 0u;
+    __syncthreads();
+    for (long long i = tid; i < n; i += blockDim.x) {
+      unsigned key;
+      if (median_key(w[i], mode, center, key) && (key & mask) == prefix)
+        atomicAdd(&sm.hist[(key >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      unsigned r = rank, b = 0;
+      while (b < 255u && sm.hist[b] <= r) r -= sm.hist[b++];
+      sm.digit = b;
+      sm.rank = r;
+    }
+    __syncthreads();
+    prefix |= sm.digit << shift;
+    mask |= 255u << shift;
+    rank = sm.rank;
+    __syncthreads();  // the histogram is cleared next pass
+  }
+  return prefix;
+}
+
+__device__ float median_of(const float *__restrict__ w, long long n, unsigned count, int mode,
+                           float center, MedianScratch &sm) {
+  const float a = float_unorder(select_rank(w, n, (count - 1) / 2, mode, center, sm));
+  if (count & 1u) return a;
+  const float b = float_unorder(select_rank(w, n, count / 2, mode, center, sm));
+  return (a + b) / 2.0f;
+}
+
+__global__ __launch_bounds__(256) void window_median_kernel(const float *__restrict__ x,
+                                                            long long window, long long shift,
+                                                            float *__restrict__ stats) {
+  __shared__ MedianScratch sm;
+  const float *w = x + (long long)blockIdx.x * shift;
+  if (threadIdx.x == 0) sm.count = 0u;
+  __syncthreads();
+  unsigned c = 0;
+  for (long long i = threadIdx.x; i < window; i += blockDim.x) c += w[i] != 0.0f;
+  atomicAdd(&sm.count, c);
+  __syncthreads();
+  const unsigned count = sm.count;
+  float center = 0.0f, deviation = 0.0f;
+  if (count > 0) {
+    center = median_of(w, window, count, 0, 0.0f, sm);
+    deviation = median_of(w, window, count, 1, center, sm);
+  }
+  if (threadIdx.x == 0) {
+    stats[2 * blockIdx.x] = center;
+    stats[2 * blockIdx.x + 1] = deviation;
+  }
+}
+
 // Per-window thresholds -> every sample: sample t takes window t / shift
 // (clamped to the last), samples before window/2 take the value of sample
 // window/2 and samples from n - (window - window/2) on the value of the

@@ -26,17 +26,25 @@ def _stream(stream):
     return ct.c_void_p((stream or torch.cuda.current_stream()).cuda_stream)
 
 
-def time_dependent_threshold(beam_max, window, overlap=0.66, num_dev=8.0, stream=None):
+THRESHOLD_TYPES = {"rms": 0, "mad": 1}
+
+
+def time_dependent_threshold(beam_max, window, overlap=0.66, num_dev=8.0, threshold_type="rms",
+                             stream=None):
     """beam_max: (T,) float32 CUDA tensor -> (T,) float32 CUDA threshold
-    (notebooks/3_beampower.ipynb:497-578, "rms"; exact zeros are left out of
-    the window statistics)."""
+    (notebooks/3_beampower.ipynb:497-578).  threshold_type "rms" (mean and
+    standard deviation per window) or "mad" (median and median absolute
+    deviation); exact zeros are left out of the window statistics."""
     torch = _torch()
     lib = load_library("gpu")
+    kind = THRESHOLD_TYPES.get(str(threshold_type).lower())
+    if kind is None:
+        raise ValueError(f"threshold_type must be 'rms' or 'mad', not {threshold_type!r}")
     x = beam_max.contiguous()
     out = torch.empty_like(x)
-    check(lib.bp_detect_threshold(x.data_ptr(), x.numel(), int(window), float(overlap),
-                                  float(num_dev), out.data_ptr(), _stream(stream)),
-          "bp_detect_threshold", lib)
+    check(lib.bp_detect_threshold_type(x.data_ptr(), x.numel(), int(window), float(overlap),
+                                       float(num_dev), kind, out.data_ptr(), _stream(stream)),
+          "bp_detect_threshold_type", lib)
     return out
 
 
@@ -60,13 +68,13 @@ def find_peaks(beam_max, distance, threshold=None, stream=None):
 
 
 def detect(beam_max, beam_argmax, sampling_rate, window_sec=7200.0, overlap=0.66, num_dev=8.0,
-           min_interval_sec=60.0, stream=None):
+           min_interval_sec=60.0, threshold_type="rms", stream=None):
     """The tutorial's detector (notebooks/3_beampower.ipynb:582-626): returns
     dict(peaks, sources, values, threshold) with peaks as sample indices,
     sources = beam_argmax at the peaks, values = beam_max at the peaks and
     the threshold CUDA tensor."""
     window = int(window_sec * sampling_rate)
-    thr = time_dependent_threshold(beam_max, window, overlap, num_dev, stream)
+    thr = time_dependent_threshold(beam_max, window, overlap, num_dev, threshold_type, stream)
     peaks = find_peaks(beam_max, int(min_interval_sec * sampling_rate), thr, stream)
     idx = _torch().from_numpy(peaks).to(beam_max.device)
     return dict(peaks=peaks, sources=beam_argmax[idx].cpu().numpy(),

@@ -376,6 +376,17 @@ const char *bp_server_last_error(void);
  *   n_peaks.  Synchronous. */
 int bp_detect_threshold(const float *beam_max, size_t n_samples, size_t window, double overlap,
                         double num_dev, float *threshold, void *stream);
+/* The threshold in either of the tutorial's flavours
+ * (notebooks/3_beampower.ipynb:497-578, threshold_type "rms" or "mad"):
+ * BP_THRESHOLD_RMS is bp_detect_threshold; BP_THRESHOLD_MAD takes each
+ * window's median as center and the median of the absolute deviations from
+ * it as deviation (numpy's median in float32: the mean of the two middle
+ * values for an even count), exact zeros again left out.  Synchronous. */
+#define BP_THRESHOLD_RMS 0
+#define BP_THRESHOLD_MAD 1
+int bp_detect_threshold_type(const float *beam_max, size_t n_samples, size_t window,
+                             double overlap, double num_dev, int threshold_type,
+                             float *threshold, void *stream);
 int bp_detect_peaks(const float *beam_max, const float *threshold, size_t n_samples,
                     size_t distance, int *peaks, size_t capacity, size_t *n_peaks,
                     void *stream);

@@ -6,8 +6,9 @@ detection consumer (bp_detect_threshold / bp_detect_peaks).
 Reference: notebooks/3_beampower.ipynb
   * time_dependent_threshold (:497-578): sliding windows of `window` samples
     every int((1 - overlap) * window) samples, center + num_dev * deviation
-    per window ("rms": mean and population standard deviation), one-step
-    max with each neighbour, nearest-window expansion to every sample;
+    per window ("rms": mean and population standard deviation; "mad": median
+    and median absolute deviation), one-step max with each neighbour,
+    nearest-window expansion to every sample;
   * detection (:618-626): scipy.signal.find_peaks(beam_max, distance=...)
     then peaks above the threshold.
 
@@ -25,11 +26,17 @@ agree.
 import numpy as np
 
 
-def time_dependent_threshold(series, window, overlap=0.66, num_dev=8.0, zeros="exclude"):
-    """float32 threshold of len(series) (notebooks/3_beampower.ipynb:497-578,
-    rms flavour).  zeros: "exclude" (drop exact zeros from every window's
+def time_dependent_threshold(series, window, overlap=0.66, num_dev=8.0, zeros="exclude",
+                             threshold_type="rms"):
+    """float32 threshold of len(series) (notebooks/3_beampower.ipynb:497-578).
+    threshold_type: "rms" (mean and population standard deviation per
+    window) or "mad" (median and median absolute deviation, numpy's float32
+    median).  zeros: "exclude" (drop exact zeros from every window's
     statistics) or a numpy Generator (the reference's noise fill, seeded)."""
     x = np.asarray(series, np.float32).copy()
+    kind = threshold_type.lower()
+    if kind not in ("rms", "mad"):
+        raise ValueError(f"threshold_type must be 'rms' or 'mad', not {threshold_type!r}")
     n = len(x)
     half = window // 2
     shift = int((1.0 - overlap) * window)
@@ -39,14 +46,29 @@ def time_dependent_threshold(series, window, overlap=0.66, num_dev=8.0, zeros="e
     starts = np.arange(0, n - window + 1, shift)
     if isinstance(zeros, np.random.Generator):
         fill = zeros.normal(size=int(z.sum())).astype(np.float32)
-        x[z] = fill * np.std(x[~z]) + x[~z].mean()
+        if kind == "rms":
+            x[z] = fill * np.std(x[~z]) + x[~z].mean()
+        else:
+            c0 = np.median(x[~z])
+            x[z] = fill * np.median(np.abs(x[~z] - c0)) + c0
         win = np.lib.stride_tricks.sliding_window_view(x, window)[::shift, :]
-        center = np.mean(win, axis=-1)
-        deviation = np.std(win, axis=-1)
+        if kind == "rms":
+            center = np.mean(win, axis=-1)
+            deviation = np.std(win, axis=-1)
+        else:
+            center = np.median(win, axis=-1)
+            deviation = np.median(np.abs(win - center[:, np.newaxis]), axis=-1)
     else:
         center = np.empty(len(starts), np.float32)
         deviation = np.empty(len(starts), np.float32)
         for j, s in enumerate(starts):
+            if kind == "mad":
+                w = x[s:s + window]
+                w = w[w != 0.0]
+                c = np.median(w) if w.size else np.float32(0.0)
+                center[j] = c
+                deviation[j] = np.median(np.abs(w - c)) if w.size else 0.0
+                continue
             w = x[s:s + window].astype(np.float64)
             w = w[w != 0.0]
             m = w.mean() if w.size else 0.0

@@ -63,6 +63,42 @@ def test_threshold_reference_fill_agrees_on_zero_free_series():
     assert np.allclose(a, b, rtol=2e-6)
 
 
+def test_mad_threshold_closed_forms():
+    """"mad": constant series -> deviation 0; the alternating 1, 3 series
+    has median 2 (mean of the two middle values) and every |x - 2| = 1;
+    1..100 repeated (window 100) has median 50.5 and median absolute
+    deviation 25 (the mean of 24.5 and 25.5); exact zeros are left out."""
+    thr = do.time_dependent_threshold(np.full(1000, 3.5, np.float32), 200, num_dev=8,
+                                      threshold_type="mad")
+    assert np.all(thr == np.float32(3.5))
+    x = np.tile(np.array([1.0, 3.0], np.float32), 600)
+    thr = do.time_dependent_threshold(x, 100, overlap=0.5, num_dev=8, threshold_type="MAD")
+    assert np.all(thr == np.float32(2 + 8 * 1.0))
+    x = np.tile(np.arange(1, 101, dtype=np.float32), 12)
+    thr = do.time_dependent_threshold(x, 100, overlap=0.5, num_dev=2, threshold_type="mad")
+    assert np.all(thr == np.float32(50.5 + 2 * 25.0))
+    x = np.full(2000, 2.0, np.float32)
+    x[::3] = 0.0
+    thr = do.time_dependent_threshold(x, 300, num_dev=8, threshold_type="mad")
+    assert np.all(thr == np.float32(2.0))
+
+
+def test_mad_threshold_is_the_tutorials_on_zero_free_series():
+    """On a series without zeros the tutorial's own steps (noise fill of no
+    sample, np.median over sliding_window_view, :541-553) give the same
+    bits as the oracle's per-window exclusion rule."""
+    rng = np.random.default_rng(3)
+    x = np.abs(rng.standard_normal(6000)).astype(np.float32) + 0.1
+    a = do.time_dependent_threshold(x, 700, num_dev=8, threshold_type="mad")
+    b = do.time_dependent_threshold(x, 700, num_dev=8, zeros=np.random.default_rng(1),
+                                    threshold_type="mad")
+    assert np.array_equal(a, b)
+    # and the mad threshold differs from the rms one (a wrong branch fails)
+    assert not np.allclose(a, do.time_dependent_threshold(x, 700, num_dev=8), rtol=1e-3)
+    with pytest.raises(ValueError):
+        do.time_dependent_threshold(x, 700, threshold_type="std")
+
+
 @pytest.mark.parametrize("seed,distance", [(0, 1), (1, 5), (2, 60), (3, 250), (4, 1500)])
 def test_find_peaks_matches_scipy(seed, distance):
     rng = np.random.default_rng(seed)

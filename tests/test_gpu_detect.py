@@ -49,6 +49,23 @@ def test_threshold_matches_oracle(seed, n, window, overlap):
     assert np.allclose(thr.cpu().numpy(), ref, rtol=1e-6, atol=0)
 
 
+@pytest.mark.parametrize("seed,n,window,overlap", [(0, 20000, 3000, 0.66), (1, 100000, 18000, 0.66),
+                                                   (2, 7000, 1000, 0.5), (7, 9001, 1000, 0.66)])
+def test_mad_threshold_matches_oracle(seed, n, window, overlap):
+    """"mad": medians are exact selections and the arithmetic is the same
+    float32 steps, so the threshold is bit-identical to the oracle's; the
+    series has ties (plateaus) and a zero tail."""
+    torch = _torch()
+    from beampower_amd import detect
+    x = beam_like(seed, n)
+    if seed == 7:
+        x[::2] = np.round(x[::2])  # many equal values
+    thr = detect.time_dependent_threshold(torch.from_numpy(x).cuda(), window, overlap, 8.0,
+                                          threshold_type="mad")
+    ref = do.time_dependent_threshold(x, window, overlap, 8.0, threshold_type="mad")
+    assert np.array_equal(thr.cpu().numpy(), ref)
+
+
 @pytest.mark.parametrize("seed,distance", [(3, 1), (4, 7), (5, 60), (6, 1500)])
 def test_peaks_match_scipy(seed, distance):
     torch = _torch()

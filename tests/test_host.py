@@ -34,6 +34,14 @@ def test_library_exports_every_declared_symbol():
     assert lib.bp_version().decode().count(".") == 2
 
 
+def test_every_declared_symbol_has_argument_types():
+    """ctypes converts Python floats and structs only for declared argument
+    types; an undeclared entry point would fail at its first call."""
+    lib = bp.load_library("gpu")
+    for name in core.GPU_SYMBOLS:
+        assert getattr(lib, name).argtypes is not None, name
+
+
 def test_library_is_gfx950_only():
     out = os.popen(f"/opt/rocm/llvm/bin/clang-offload-bundler --list --type=o "
                    f"--input={core.LIBRARIES['gpu']['filepath_library']} 2>/dev/null").read()
@@ -198,3 +206,20 @@ def test_tuner_candidates_and_build_flags():
     assert all(f in bp_build.command("x.so", flags) for f in flags)
     with pytest.raises(ValueError):
         bp_build.shape_flags({"BP_ANYTHING": 1})
+
+
+def test_reference_import_name():
+    """`import beampower` exposes the reference's names and module layout
+    (beampower/__init__.py:9-14, beampower/beampower.py, beampower/core.py),
+    served by this engine."""
+    import beampower
+    import beampower_amd
+    assert beampower.__all__ == ["load_library", "beamform"]
+    assert beampower.beamform is beampower_amd.beamform
+    assert beampower.beampower.beamform is beampower_amd.beampower.beamform
+    assert beampower.beampower.prestack_traces is beampower_amd.beampower.prestack_traces
+    assert beampower.load_library is beampower_amd.load_library
+    assert beampower.core.load_library("gpu") is beampower_amd.load_library("gpu")
+    assert beampower.core.LIBRARIES is beampower_amd.core.LIBRARIES
+    w = synthetic.random_case(0, S=2, C=3, P=2, T=16, K=3)
+    assert beampower.beamform(w[0], w[2], w[1], w[3], device="gpu", mode="sideways") == 1
