@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "beamform_gpu.h"
+#include "bp_internal.h"
 #include "bp_kernels.h"
 #include "bp_order.h"
 
@@ -121,23 +122,32 @@ constexpr long long kMaxLaunchBlocks = ((0xffffffffLL / (64 * kW)) / 8) * 8;
 template <int FMT, int P, int NB, bool NONE>
 using Lds = bp::StackLds<FMT, P, kKW, kW, NB, NONE>;
 
-// Device buffer owned by RAII.
+// Device buffer owned by RAII, or borrowed from memory that someone else
+// owns (borrow(): never freed here).
 struct DevBuf {
   void *p = nullptr;
   size_t bytes = 0;  // as allocated
+  bool own = true;
   DevBuf() = default;
   DevBuf(const DevBuf &) = delete;
   DevBuf &operator=(const DevBuf &) = delete;
   ~DevBuf() {
-    if (p) (void)hipFree(p);
+    if (p && own) (void)hipFree(p);
+  }
+  void borrow(void *ptr, size_t n) {
+    if (p && own) (void)hipFree(p);
+    p = ptr;
+    bytes = n;
+    own = false;
   }
   int alloc(size_t bytes) {
     if (bytes == 0) bytes = 16;
     if (p) {  // a buffer that is allocated again releases the old memory
-      (void)hipFree(p);
+      if (own) (void)hipFree(p);
       p = nullptr;
       this->bytes = 0;
     }
+    own = true;
     hipError_t e = hipMalloc(&p, bytes);
     if (e != hipSuccess) {
       p = nullptr;
@@ -817,6 +827,7 @@ struct FusedSet {
   std::vector<bp::BatchWindow> wins;
   DevBuf dwins;
   DevBuf dsplit;            // 9 item indices: XCD x walks [split[x], split[x+1])
+  long long split_h[9] = {};  // host copy (the source of an asynchronous upload)
   long long n_items = 0;
   long long max_range = 0;  // largest XCD range (items)
   int max_lw = 0;
@@ -841,10 +852,36 @@ struct bp_batch {
   // with other station scales leave this batch's weights as they were
   std::vector<std::shared_ptr<Schedule>> scheds;
   DevBuf dkeys;
+  std::vector<bp::BatchKeys> keys_h;  // host copy of dkeys
+  // Where the device tables live: hipMalloc'd (ws == NULL), or carved from a
+  // caller's workspace and uploaded asynchronously on `upload` (a batch of
+  // the serving loop: no allocation, no synchronisation, no hipFree).
+  char *ws = nullptr;
+  size_t ws_bytes = 0, ws_used = 0;
+  hipStream_t upload = nullptr;
   hipStream_t cap = nullptr;  // private stream for graph capture
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   ~bp_batch() { drop_graph(); if (cap) (void)hipStreamDestroy(cap); }
+  // device memory of `bytes` for a table
+  int table(DevBuf &d, size_t bytes) {
+    if (!ws) return d.alloc(bytes);
+    const size_t off = (ws_used + 255) / 256 * 256;
+    if (off + bytes > ws_bytes)
+      return fail(BP_EINVAL, "batch tables need more than the %zu-byte workspace", ws_bytes);
+    d.borrow(ws + off, bytes);
+    ws_used = off + bytes;
+    return BP_OK;
+  }
+  // host -> device copy of a table (asynchronous on `upload` when placed)
+  int put(DevBuf &d, const void *src, size_t bytes) {
+    if (upload) {
+      HIP_TRY(hipMemcpyAsync(d.p, src, bytes, hipMemcpyHostToDevice, upload));
+    } else {
+      HIP_TRY(hipMemcpy(d.p, src, bytes, hipMemcpyHostToDevice));
+    }
+    return BP_OK;
+  }
   void drop_graph() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -879,16 +916,16 @@ int class_nb(const bp_batch *b, int c) {
 // equal shares of the cost (items x stations): windows differ 8x in
 // stations, and equal item counts made the fused launch 23 % slower than
 // the windows one by one.
-int finish_fused(FusedSet &f) {
+int finish_fused(bp_batch *b, FusedSet &f) {
   if (f.wins.empty()) return BP_OK;
-  BP_TRY(f.dwins.alloc(f.wins.size() * sizeof(bp::BatchWindow)));
+  BP_TRY(b->table(f.dwins, f.wins.size() * sizeof(bp::BatchWindow)));
   const std::vector<bp::BatchWindow> &ws = f.wins;
   auto items_of = [&](size_t w) {
     return (w + 1 < ws.size() ? ws[w + 1].item_begin : f.n_items) - ws[w].item_begin;
   };
   double total = 0.0;
   for (size_t w = 0; w < ws.size(); ++w) total += (double)items_of(w) * ws[w].a.S;
-  long long split[9];
+  long long *split = f.split_h;
   split[0] = 0;
   split[8] = f.n_items;
   size_t w = 0;
@@ -909,9 +946,8 @@ int finish_fused(FusedSet &f) {
     split[x] = std::max(split[x], split[x - 1]);
   }
   for (int x = 0; x < 8; ++x) f.max_range = std::max(f.max_range, split[x + 1] - split[x]);
-  BP_TRY(f.dsplit.alloc(sizeof(split)));
-  HIP_TRY(hipMemcpy(f.dsplit.p, split, sizeof(split), hipMemcpyHostToDevice));
-  return BP_OK;
+  BP_TRY(b->table(f.dsplit, sizeof(f.split_h)));
+  return b->put(f.dsplit, f.split_h, sizeof(f.split_h));
 }
 
 // One window's tiles of class c in a launch of its own: the kernel a window
@@ -1021,8 +1057,7 @@ int batch_configure(bp_batch *b, const bp_batch_config &c) {
     for (FusedSet *f : {&b->all[k], &b->small[k]}) {
       if (f->wins.empty()) continue;
       for (auto &w : f->wins) w.a.time_group = c.time_group;
-      HIP_TRY(hipMemcpy(f->dwins.p, f->wins.data(), f->wins.size() * sizeof(bp::BatchWindow),
-                        hipMemcpyHostToDevice));
+      BP_TRY(b->put(f->dwins, f->wins.data(), f->wins.size() * sizeof(bp::BatchWindow)));
     }
     for (auto &a : b->large[k]) a.time_group = c.time_group;
   }
@@ -1302,6 +1337,8 @@ int bp_network_beamform_max_view(bp_network *net, const bp_feature_view *feature
                      static_cast<hipStream_t>(stream));
 } BP_CATCH
 
+}  // extern "C"
+
 // ---- reduce='max' without allocation ---------------------------------------
 
 // Key scratch of bp_network_beamform_max_noalloc: one packed (value, source)
@@ -1349,6 +1386,8 @@ std::shared_ptr<Schedule> find_schedule(bp_network *net, int fmt, const float *s
 }
 
 }  // namespace
+
+extern "C" {
 
 int bp_network_beamform_max_noalloc(bp_network *net, const bp_feature_view *features,
                                     int out_of_bounds, float *beam_max, int *beam_argmax,
@@ -1930,10 +1969,16 @@ int prestack_waveform_features(float *waveform_features, float *weights_phases,
   return BP_OK;
 } BP_CATCH
 
-int bp_batch_create(int n_windows, bp_network *const *networks,
-                    const bp_feature_view *features, const int *out_of_bounds,
-                    float *const *beam_max, int *const *beam_argmax,
-                    unsigned long long *const *keys, bp_batch **out) try {
+}  // extern "C"
+
+namespace {
+
+// Shared by bp_batch_create (pl == NULL: tables hipMalloc'd, synchronous
+// uploads) and bp_batch_create_placed.
+int batch_create(int n_windows, bp_network *const *networks, const bp_feature_view *features,
+                 const int *out_of_bounds, float *const *beam_max, int *const *beam_argmax,
+                 unsigned long long *const *keys, const bp_batch_placement *pl,
+                 bp_batch **out) {
   if (!out) return fail(BP_EINVAL, "out is NULL");
   *out = nullptr;
   if (n_windows <= 0 || !networks || !features || !out_of_bounds || !beam_max ||
@@ -1941,13 +1986,25 @@ int bp_batch_create(int n_windows, bp_network *const *networks,
     return fail(BP_EINVAL, "a batch needs n_windows > 0 and non-NULL arrays");
   if (n_windows > 65535) return fail(BP_EINVAL, "too many windows (%d)", n_windows);
   const int fmt = features[0].fmt;
-  BP_TRY(check_device());
   auto b = std::make_unique<bp_batch>();
   b->fmt = fmt;
   b->n = n_windows;
-  Stream st;
-  BP_TRY(st.create());
-  std::vector<bp::BatchKeys> kd;
+  Stream own;
+  hipStream_t st = nullptr;
+  if (pl) {
+    if (!pl->ws || pl->ws_bytes < bp_batch_table_bytes(n_windows))
+      return fail(BP_EINVAL, "batch table workspace of %zu bytes, %zu needed", pl->ws_bytes,
+                  bp_batch_table_bytes(n_windows));
+    b->ws = static_cast<char *>(pl->ws);
+    b->ws_bytes = pl->ws_bytes;
+    b->upload = pl->stream;
+    st = pl->stream;
+  } else {
+    BP_TRY(check_device());
+    BP_TRY(own.create());
+    st = own.s;
+  }
+  std::vector<bp::BatchKeys> &kd = b->keys_h;
   for (int i = 0; i < n_windows; ++i) {
     bp_network *net = networks[i];
     if (!net || !beam_max[i] || !beam_argmax[i] || !keys[i])
@@ -1964,7 +2021,15 @@ int bp_batch_create(int n_windows, bp_network *const *networks,
     if (oob != 0 && oob != 1)
       return fail(BP_EINVAL, "window %d: out_of_bounds must be 0 or 1, not %d", i, oob);
     std::shared_ptr<Schedule> sched;
-    BP_TRY(ensure_schedule(net, fmt, fmt == BP_F32 ? nullptr : fv.station_scale, st.s, sched));
+    const float *sc = fmt == BP_F32 ? nullptr : fv.station_scale;
+    if (pl) {
+      sched = find_schedule(net, fmt, sc);
+      if (!sched)
+        return fail(BP_EINVAL, "window %d: its network has no schedule for format %d and these "
+                    "station scales (bp_network_prepare)", i, fmt);
+    } else {
+      BP_TRY(ensure_schedule(net, fmt, sc, st, sched));
+    }
     b->scheds.push_back(sched);
     const int T = (int)fv.n_samples;
     kd.push_back(bp::BatchKeys{keys[i], beam_max[i], beam_argmax[i], T});
@@ -1999,12 +2064,11 @@ int bp_batch_create(int n_windows, bp_network *const *networks,
     }
   }
   for (int c = 0; c < 2; ++c) {
-    BP_TRY(finish_fused(b->all[c]));
-    BP_TRY(finish_fused(b->small[c]));
+    BP_TRY(finish_fused(b.get(), b->all[c]));
+    BP_TRY(finish_fused(b.get(), b->small[c]));
   }
-  BP_TRY(b->dkeys.alloc(kd.size() * sizeof(bp::BatchKeys)));
-  HIP_TRY(hipMemcpy(b->dkeys.p, kd.data(), kd.size() * sizeof(bp::BatchKeys),
-                    hipMemcpyHostToDevice));
+  BP_TRY(b->table(b->dkeys, kd.size() * sizeof(bp::BatchKeys)));
+  BP_TRY(b->put(b->dkeys, kd.data(), kd.size() * sizeof(bp::BatchKeys)));
   BP_TRY(init_batch_attrs());
   BP_TRY(init_kernel_attrs());
   bp_batch_config cfg = b->cfg;
@@ -2014,9 +2078,39 @@ int bp_batch_create(int n_windows, bp_network *const *networks,
     if (it != g_tune_cache.end()) cfg = it->second;
   }
   BP_TRY(batch_configure(b.get(), cfg));
-  HIP_TRY(hipStreamSynchronize(st.s));
+  if (!pl) HIP_TRY(hipStreamSynchronize(st));
   *out = b.release();
   return BP_OK;
+}
+
+}  // namespace
+
+size_t bp_batch_table_bytes(int n_windows) {
+  const size_t n = (size_t)std::max(0, n_windows);
+  auto up = [](size_t v) { return (v + 255) / 256 * 256; };
+  // four fused sets (two classes, all windows and the small ones): window
+  // descriptors and XCD split each; the keys table
+  return 4 * (up(n * sizeof(bp::BatchWindow)) + up(9 * sizeof(long long))) +
+         up(n * sizeof(bp::BatchKeys)) + 256;
+}
+
+int bp_batch_create_placed(int n_windows, bp_network *const *networks,
+                           const bp_feature_view *features, const int *out_of_bounds,
+                           float *const *beam_max, int *const *beam_argmax,
+                           unsigned long long *const *keys, const bp_batch_placement &pl,
+                           bp_batch **out) try {
+  return batch_create(n_windows, networks, features, out_of_bounds, beam_max, beam_argmax, keys,
+                      &pl, out);
+} BP_CATCH
+
+extern "C" {
+
+int bp_batch_create(int n_windows, bp_network *const *networks,
+                    const bp_feature_view *features, const int *out_of_bounds,
+                    float *const *beam_max, int *const *beam_argmax,
+                    unsigned long long *const *keys, bp_batch **out) try {
+  return batch_create(n_windows, networks, features, out_of_bounds, beam_max, beam_argmax, keys,
+                      nullptr, out);
 } BP_CATCH
 
 int bp_batch_launch(bp_batch *b, void *stream) try {

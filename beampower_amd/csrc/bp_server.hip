@@ -9,12 +9,20 @@
 //              copies the features in, prestacks them (the network's phase
 //              weights) and packs them into rows padded for the network in
 //              the server's storage format;
-//   scheduler  takes every admitted request that is ready, up to max_batch,
-//              into one batched launch (bp_batch_*) on one of `in_flight`
-//              streams, completes finished batches (outputs back to the
-//              caller's host buffers) and returns their memory to the arena.
+//   scheduler  forms a batch whenever one of the `in_flight` slots is free:
+//              ready requests in earliest-deadline order (a request's deadline
+//              is its ready time plus kSlack times its estimated GPU time, so
+//              short windows go ahead of long ones and a long one still gets
+//              its turn), up to max_batch windows and about kBatchSeconds of
+//              estimated GPU work (at least one request); launches it on the
+//              slot's stream (bp_batch_*), completes finished batches
+//              (outputs back to the caller's host buffers) and returns their
+//              memory to the arena.
 // Requests arriving while batches run are admitted as memory frees up and
-// join the next launch; with in_flight = 2 two batches overlap on the device.
+// join the next launch; up to in_flight batches overlap on the device.  A
+// slot's batch tables live in device memory allocated once with the server,
+// so forming and retiring batches allocates and frees no device memory
+// (hipFree would wait for every batch in flight).
 // A request's result does not depend on what it is batched with: the batch
 // runs the same tile code on the same tiles and reduces with an
 // order-independent packed-key maximum.
@@ -35,8 +43,18 @@
 #include <vector>
 
 #include "beamform_gpu.h"
+#include "bp_internal.h"
 
 namespace {
+
+// Scheduling: estimated GPU time of a request = its gather-FMAs
+// (sources x stations x phases x samples) / kFmaRate, the stack kernel's
+// measured rate on these windows (2.6e13/s); a batch takes about
+// kBatchSeconds of such work; a request's deadline is its ready time plus
+// kSlack times its estimated time.
+constexpr double kFmaRate = 2.6e13;
+constexpr double kBatchSeconds = 0.010;
+constexpr double kSlack = 4.0;
 
 double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
@@ -125,6 +143,8 @@ struct Request {
   float *bmax_host = nullptr;
   int *barg_host = nullptr;
   bp_request_info info{};
+  double work = 0.0;      // gather-FMAs
+  double deadline = 0.0;  // scheduling key (steady clock)
   // device memory: temporaries (freed once packed) and what the launch reads
   long long off_tmp = -1, off_keep = -1;
   size_t bytes_tmp = 0, bytes_keep = 0;
@@ -141,6 +161,7 @@ struct Flight {
   bp_batch *batch = nullptr;
   hipEvent_t ev = nullptr;
   std::vector<Request *> reqs;
+  int slot = -1;  // its stream and batch-table memory
 };
 
 thread_local std::string t_err;
@@ -182,13 +203,15 @@ struct bp_server {
   std::deque<Flight> flights;
   std::map<long long, Request *> reqs;
   hipStream_t load_stream = nullptr;
-  std::vector<hipStream_t> compute;
+  std::vector<hipStream_t> compute;  // one stream per slot
+  void *tables = nullptr;            // batch tables, table_bytes per slot
+  size_t table_bytes = 0;
+  std::vector<int> free_slots;
   std::thread loader, scheduler;
   bool stop = false;
   bool broken = false;  // a worker thread failed: submits are refused
   long long next_id = 1;
   long long n_batches = 0, n_batched_windows = 0, n_completed = 0;
-  int launch_index = 0;
 
   // memory a request needs: temporaries (features in, prestacked planar)
   // and what stays until completion (packed rows, outputs, keys)
@@ -206,7 +229,7 @@ struct bp_server {
   void guarded(void (bp_server::*loop)());
   void fail_request(Request *r, int status, const char *msg) noexcept;
   int admit(Request *r);
-  void launch(std::vector<Request *> batch);
+  void launch(std::vector<Request *> batch, int slot);
   void complete(Flight &f);
 };
 
@@ -331,6 +354,7 @@ void bp_server::loader_loop() {
     arena.release(r->off_tmp, r->bytes_tmp);
     r->off_tmp = -1;
     r->info.ready_s = now_s();
+    r->deadline = r->info.ready_s + kSlack * r->work / kFmaRate;
     if (rc != BP_OK) {
       fail_request(r, rc, t_err.c_str());
       cv_done.notify_all();
@@ -349,9 +373,10 @@ void bp_server::loader_loop() {
   }
 }
 
-void bp_server::launch(std::vector<Request *> batch) {
+void bp_server::launch(std::vector<Request *> batch, int slot) {
   const int n = (int)batch.size();
   Flight f;
+  f.slot = slot;
   int rc = BP_OK;
   double t = now_s();
   try {
@@ -371,9 +396,11 @@ void bp_server::launch(std::vector<Request *> batch) {
       keys[i] = r->d_keys;
     }
     f.reqs = batch;
-    hipStream_t st = compute[launch_index++ % compute.size()];
-    rc = bp_batch_create(n, nv.data(), views.data(), oob.data(), bmax.data(), barg.data(),
-                         keys.data(), &f.batch);
+    hipStream_t st = compute[slot];
+    const bp_batch_placement pl{static_cast<char *>(tables) + (size_t)slot * table_bytes,
+                                table_bytes, st};
+    rc = bp_batch_create_placed(n, nv.data(), views.data(), oob.data(), bmax.data(),
+                                barg.data(), keys.data(), pl, &f.batch);
     if (rc != BP_OK) fail(rc, bp_last_error());
     t = now_s();
     if (rc == BP_OK && (rc = bp_batch_launch(f.batch, st)) != BP_OK) fail(rc, bp_last_error());
@@ -401,8 +428,13 @@ void bp_server::launch(std::vector<Request *> batch) {
       r->off_keep = -1;
       r->done = true;
     }
-    if (f.batch) bp_batch_destroy(f.batch);
+    if (f.batch) {
+      // its table uploads may still be queued on the slot's stream
+      (void)hipStreamSynchronize(compute[slot]);
+      bp_batch_destroy(f.batch);
+    }
     if (f.ev) (void)hipEventDestroy(f.ev);
+    free_slots.push_back(slot);
     cv_done.notify_all();
     cv_load.notify_all();
     return;
@@ -426,9 +458,10 @@ void bp_server::complete(Flight &f) {
       rc = fail(BP_EHIP, "result download failed");
   }
   const double t_out = now_s();
-  bp_batch_destroy(f.batch);
+  bp_batch_destroy(f.batch);  // its tables are the slot's: nothing is freed on the device
   (void)hipEventDestroy(f.ev);
   std::lock_guard<std::mutex> lock(mu);
+  free_slots.push_back(f.slot);
   for (Request *r : f.reqs) {
     r->info.gpu_done_s = t;
     r->info.done_s = t_out;
@@ -450,24 +483,46 @@ void bp_server::scheduler_loop() {
   (void)hipSetDevice(device);
   std::unique_lock<std::mutex> lock(mu);
   for (;;) {
-    if (!ready.empty() && (int)flights.size() < cfg.in_flight) {
+    if (!ready.empty() && !free_slots.empty()) {
+      // earliest deadline first, up to max_batch windows and about
+      // kBatchSeconds of estimated work (the first request always goes)
+      std::stable_sort(ready.begin(), ready.end(),
+                       [](const Request *a, const Request *b) { return a->deadline < b->deadline; });
+      const double cap = kBatchSeconds * kFmaRate;
       std::vector<Request *> batch;
       // reserved before any request leaves `ready`, so a failed allocation
       // leaves them all there
       batch.reserve(std::min<size_t>(ready.size(), (size_t)cfg.max_batch));
-      while (!ready.empty() && (int)batch.size() < cfg.max_batch) {
+      double work = 0.0;
+      while (!ready.empty() && (int)batch.size() < cfg.max_batch &&
+             (batch.empty() || work + ready.front()->work <= cap)) {
+        work += ready.front()->work;
         batch.push_back(ready.front());
         ready.pop_front();
       }
+      const int slot = free_slots.back();
+      free_slots.pop_back();
       lock.unlock();
-      launch(std::move(batch));
+      launch(std::move(batch), slot);
       lock.lock();
       continue;
     }
     if (!flights.empty()) {
-      // the oldest batch: complete it when done; new work may arrive meanwhile
-      if (hipEventQuery(flights.front().ev) == hipSuccess ||
-          (int)flights.size() >= cfg.in_flight || stop) {
+      // complete the batches that are done (in any order); if every slot
+      // is taken, or the server stops, wait for the oldest
+      bool done_any = false;
+      for (size_t i = 0; i < flights.size(); ++i) {
+        if (hipEventQuery(flights[i].ev) != hipSuccess) continue;
+        Flight f = std::move(flights[i]);
+        flights.erase(flights.begin() + (long)i);
+        lock.unlock();
+        complete(f);
+        lock.lock();
+        done_any = true;
+        break;
+      }
+      if (done_any) continue;
+      if (stop) {
         Flight f = std::move(flights.front());
         flights.pop_front();
         lock.unlock();
@@ -504,6 +559,9 @@ int bp_server_create(const bp_server_config *cfg, bp_server **out) try {
     s->compute.reserve((size_t)cfg->in_flight);
     e = hipGetDevice(&s->device);
     if (e == hipSuccess) e = s->arena.init(cfg->hbm_budget);
+    s->table_bytes = bp_batch_table_bytes(cfg->max_batch);
+    if (e == hipSuccess) e = hipMalloc(&s->tables, s->table_bytes * (size_t)cfg->in_flight);
+    for (int i = cfg->in_flight - 1; i >= 0; --i) s->free_slots.push_back(i);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->load_stream, hipStreamNonBlocking);
     for (int i = 0; e == hipSuccess && i < cfg->in_flight; ++i) {
       hipStream_t st = nullptr;
@@ -583,6 +641,10 @@ int bp_server_submit(bp_server *s, int network_id, const float *waveform_feature
   r->u_host = waveform_features;
   r->T = n_samples;
   r->oob = out_of_bounds;
+  {
+    const Network &n = s->nets[network_id];
+    r->work = (double)n.K * (double)n.S * (double)n.P * (double)n_samples;
+  }
   r->bmax_host = beam_max;
   r->barg_host = beam_argmax;
   r->info.submit_s = now_s();
@@ -646,6 +708,7 @@ int bp_server_destroy(bp_server *s) try {
   for (auto &kv : s->reqs) delete kv.second;
   for (hipStream_t st : s->compute) (void)hipStreamDestroy(st);
   if (s->load_stream) (void)hipStreamDestroy(s->load_stream);
+  if (s->tables) (void)hipFree(s->tables);
   for (Network &n : s->nets) {
     if (n.alpha) (void)hipFree(n.alpha);
     bp_network_destroy(n.net);

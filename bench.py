@@ -43,12 +43,34 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_threads():
+def host_cpus():
+    """CPUs this process may run on (its affinity set)."""
     try:
-        n = len(os.sched_getaffinity(0))
+        return len(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(16, n))
+        return os.cpu_count() or 1
+
+
+def cpu_threads():
+    """Threads of the CPU baseline: every CPU of the affinity set, or the
+    host's CPU share when OMP_NUM_THREADS states it (a shared GPU host
+    grants each GPU a share of its cores)."""
+    n = host_cpus()
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    if share.isdigit() and int(share) > 0:
+        n = min(n, int(share))
+    return max(1, n)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(w, budget_s):
@@ -96,6 +118,8 @@ def cpu_baseline(w, budget_s):
         "value": work / dt,
         "unit": "source·samples/s",
         "cores": threads,
+        "host_cpus": host_cpus(),
+        "cpu_model": cpu_model(),
         "kind": kind,
         "sample": (f"beamform_max strict, all {tau.shape[0]} sources, {Tp}-sample slice "
                    f"({W} + {halo} halo) of the same features; {work:.3e} valid "
@@ -128,19 +152,23 @@ def bench_batched(args, log):
     from beampower_amd import device as bpd
     from beampower_amd import synthetic
 
+    import concurrent.futures
     n = args.batch_windows
     log(f"batched form: generating {n} windows")
+    t_gen = time.perf_counter()
     wins, work, fmas = [], 0, 0
-    for i in range(n):
-        w = synthetic.batched_c4_window(1000 + i)
-        F = bpd.prestack_planar(torch.from_numpy(w.waveform_features).cuda(),
-                                torch.from_numpy(w.weights_phases).cuda())
-        net = bpd.Network(w.time_delays, w.weights_sources)
-        K, S, P = w.time_delays.shape
-        work += K * F.shape[2]
-        fmas += K * S * P * F.shape[2]
-        wins.append((net, F, 0))
-        del w
+    with concurrent.futures.ThreadPoolExecutor(max_workers=cpu_threads()) as ex:
+        gen = ex.map(lambda i: synthetic.batched_c4_window(1000 + i), range(n))
+        for w in gen:
+            F = bpd.prestack_planar(torch.from_numpy(w.waveform_features).cuda(),
+                                    torch.from_numpy(w.weights_phases).cuda())
+            net = bpd.Network(w.time_delays, w.weights_sources)
+            K, S, P = w.time_delays.shape
+            work += K * F.shape[2]
+            fmas += K * S * P * F.shape[2]
+            wins.append((net, F, 0))
+            del w
+    log(f"batched form: {n} windows ready in {time.perf_counter() - t_gen:.1f} s")
     planar = wins
     wins = [(net, bpd.pack(F, bpd.BP_F32, *net.padding()), oob) for net, F, oob in planar]
     torch.cuda.synchronize()
@@ -160,7 +188,7 @@ def bench_batched(args, log):
         for net, F, oob in wins:
             net.beamform_max(F, oob)
 
-    reps = max(1, args.steps // 2)
+    reps = 2  # a batch of the full workload runs for seconds
     t_alone = timed(alone, reps)
     ref = []
     for net, F, oob in wins:
@@ -191,7 +219,7 @@ def bench_batched(args, log):
     agree = float(np.mean(np.concatenate(
         [b16.beam_argmax[i].cpu().numpy() == ref[i][1] for i in range(n)])))
     out["bf16"] = {"graph_ms": t16 * 1e3, "value": work / t16,
-                   "speedup_vs_f32": t_graph / t16, "argmax_equal_to_f32": agree}
+                   "speedup_vs_f32": t_graph / t16, "argmax_agreement_vs_f32": agree}
     log(f"batched bf16: {t16 * 1e3:.1f} ms (graph)")
     del b16, q, wins, planar
     torch.cuda.synchronize()
@@ -203,8 +231,9 @@ def bench_serving(args, log):
     """Serving form: four resident networks (20/50/80/120 stations,
     10k/50k/192k/300k sources), window requests of 30 s - 10 min at 25 Hz on
     random networks arriving as a Poisson process at each offered rate,
-    admitted by an HBM budget and continuously batched; BF16 window storage.
-    Latency is submit -> results in host memory."""
+    admitted by an HBM budget and continuously batched; BF16 window storage
+    (the serving default) and FP32 storage beside it, on the same request
+    streams.  Latency is submit -> results in host memory."""
     import numpy as np
     import torch
     from beampower_amd import device as bpd
@@ -213,37 +242,49 @@ def bench_serving(args, log):
 
     log("serving form: building networks")
     nets = synthetic.serving_networks()
-    srv = Server(hbm_budget=args.serve_budget_gb << 30, max_batch=64, fmt=bpd.BP_BF16,
-                 in_flight=2)
-    for n in nets:
-        srv.add_network(*n)
     out = {"networks": [{"stations": t.shape[1], "sources": t.shape[0]} for t, _, _ in nets],
-           "hbm_budget_bytes": args.serve_budget_gb << 30, "storage": "bf16", "max_batch": 64,
-           "in_flight": 2, "rates": []}
-    warm = synthetic.serving_requests(nets, 8, seed=99)
-    poisson_run(srv, warm, rate=1000.0, seed=1)
-    for rate in args.serve_rates:
-        n = int(max(16, min(600, rate * args.serve_seconds)))
-        reqs = synthetic.serving_requests(nets, n, seed=int(rate))
-        tickets, summary = poisson_run(srv, reqs, rate=float(rate), seed=int(rate))
-        out["rates"].append(summary)
-        log(f"serving {rate} req/s: p50 {summary['p50_ms']:.1f} ms, p99 {summary['p99_ms']:.1f} ms, "
-            f"{summary['throughput_req_s']:.1f} req/s, batch {summary['mean_batch_windows']:.1f}")
-    # argmax agreement with the FP32 drop-in on served windows
-    agree, total = 0, 0
-    dnets = [bpd.Network(t, b) for t, b, _ in nets]
-    for t in tickets[:24]:
-        k = t.network
-        F = bpd.prestack_planar(torch.from_numpy(t.features).cuda(),
-                                torch.from_numpy(nets[k][2]).cuda())
-        m, a = dnets[k].beamform_max(bpd.pack(F, bpd.BP_F32, *dnets[k].padding()), 0)
-        a = a.cpu().numpy()
-        agree += int((a == t.beam_argmax).sum())
-        total += a.size
-    out["argmax_agreement_vs_f32"] = agree / max(1, total)
-    out["stats"] = srv.stats()
-    srv.close()
-    del dnets
+           "hbm_budget_bytes": args.serve_budget_gb << 30, "max_batch": 64,
+           "in_flight": args.serve_in_flight}
+    served = {}
+    for name, fmt in (("bf16", bpd.BP_BF16), ("f32", bpd.BP_F32)):
+        srv = Server(hbm_budget=args.serve_budget_gb << 30, max_batch=64, fmt=fmt,
+                     in_flight=args.serve_in_flight)
+        for n in nets:
+            srv.add_network(*n)
+        warm = synthetic.serving_requests(nets, 8, seed=99)
+        poisson_run(srv, warm, rate=1000.0, seed=1)
+        rates = []
+        for rate in args.serve_rates:
+            n = int(max(16, min(600, rate * args.serve_seconds)))
+            reqs = synthetic.serving_requests(nets, n, seed=int(rate))
+            tickets, summary = poisson_run(srv, reqs, rate=float(rate), seed=int(rate))
+            rates.append(summary)
+            log(f"serving {name} {rate} req/s: p50 {summary['p50_ms']:.1f} ms, "
+                f"p99 {summary['p99_ms']:.1f} ms, {summary['throughput_req_s']:.1f} req/s, "
+                f"batch {summary['mean_batch_windows']:.1f}")
+        served[name] = tickets  # the last rate's requests
+        out[name] = {"rates": rates, "stats": srv.stats()}
+        srv.close()
+    if args.serve_rates:
+        top16, top32 = out["bf16"]["rates"][-1], out["f32"]["rates"][-1]
+        out["bf16_over_f32_source_samples_per_s_at_top_rate"] = (
+            top16["source_samples_per_s"] / top32["source_samples_per_s"])
+        # argmax agreement of the BF16-served windows with the FP32 drop-in
+        # (every sample of 24 windows), and FP32 serving against the same
+        agree, total, same32 = 0, 0, 0
+        dnets = [bpd.Network(t, b) for t, b, _ in nets]
+        for t16, t32 in list(zip(served["bf16"], served["f32"]))[:24]:
+            k = t16.network
+            F = bpd.prestack_planar(torch.from_numpy(t16.features).cuda(),
+                                    torch.from_numpy(nets[k][2]).cuda())
+            m, a = dnets[k].beamform_max(bpd.pack(F, bpd.BP_F32, *dnets[k].padding()), 0)
+            a = a.cpu().numpy()
+            agree += int((a == t16.beam_argmax).sum())
+            same32 += int((a == t32.beam_argmax).sum())
+            total += a.size
+        out["bf16_argmax_agreement_vs_f32"] = agree / max(1, total)
+        out["f32_served_argmax_equal_to_dropin"] = same32 / max(1, total)
+        del dnets
     torch.cuda.synchronize()
     torch.cuda.empty_cache()
     return out
@@ -371,7 +412,32 @@ def bench_differential(args, log):
                      "algorithmic_bytes_per_launch": table_bytes + feature_bytes},
     }
     log(f"differential: {ms:.3f} ms per step, {achieved / 1e9:.0f} GB/s")
-    del net, F, beam
+    del net
+    # general weights: two values (1 and 0.5 on alternate station pairs), so
+    # the weight table stays float32 (4 bytes per source and pair)
+    beta2 = np.ascontiguousarray(w.weights_sources * np.where(np.arange(S) % 2, 0.5, 1.0)
+                                 .astype(np.float32)[None, :])
+    net2 = bpd.Network(w.time_delays, beta2, order="natural")
+    for _ in range(max(1, args.warmup)):
+        net2.beamform_differential(F, beam=beam)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(n):
+        net2.beamform_differential(F, beam=beam)
+    e1.record()
+    torch.cuda.synchronize()
+    ms2 = e0.elapsed_time(e1) / n
+    tables2 = net2.diff_tables()
+    table2 = K * S * (4 if pairs else P * 4) + K * S * 4 + K * 4
+    achieved2 = (table2 + feature_bytes) / (ms2 * 1e-3)
+    out["float32_weights"] = {
+        "value": K / (ms2 * 1e-3), "unit": "sources/s", "ms_per_step": ms2,
+        "weight_table": tables2["weight_form"],
+        "roofline": {"bound": "hbm", "achieved": achieved2 / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved2 / HBM_PEAK,
+                     "algorithmic_bytes_per_launch": table2 + feature_bytes}}
+    log(f"differential, float32 weights: {ms2:.3f} ms per step, {achieved2 / 1e9:.0f} GB/s")
+    del net2, F, beam
     torch.cuda.synchronize()
     torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
@@ -415,11 +481,14 @@ def main():
     ap.add_argument("--oob", choices=["strict", "flexible"], default="strict")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--batch-windows", type=int, default=32,
+    ap.add_argument("--batch-windows", type=int, default=256,
                     help="windows of the batched form (256 = the full batched workload; 0 = skip)")
     ap.add_argument("--tune", action="store_true", help="autotune the batched form first")
-    ap.add_argument("--serve-rates", type=float, nargs="*", default=[5.0, 50.0, 200.0],
+    ap.add_argument("--serve-rates", type=float, nargs="*",
+                    default=[5.0, 50.0, 100.0, 125.0, 200.0],
                     help="offered request rates of the serving form (none = skip)")
+    ap.add_argument("--serve-in-flight", type=int, default=2,
+                    help="batches of the serving form on the device at once (1-4)")
     ap.add_argument("--serve-seconds", type=float, default=3.0,
                     help="arrival span per offered rate")
     ap.add_argument("--serve-budget-gb", type=int, default=16)
@@ -474,18 +543,16 @@ def main():
     torch.cuda.synchronize()
 
     log(f"timing {args.steps} steps")
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        net.beamform_max(F, oob, bmax, barg, keys)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    ms_per_step = dt * 1e3 / args.steps
-    value = K * T / (dt / args.steps)
-
+    # the stack kernels are bracketed with HIP events on the launch stream
+    # inside the same timed loop, so kernel_ms <= ms_per_step
     with bpd.KernelTimer() as kt:
+        t0 = time.perf_counter()
         for _ in range(args.steps):
             net.beamform_max(F, oob, bmax, barg, keys)
         torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+    ms_per_step = dt * 1e3 / args.steps
+    value = K * T / (dt / args.steps)
     kernel_ms = kt.total_ms / max(1, kt.launches)
     log(f"step {ms_per_step:.2f} ms, stack kernel {kernel_ms:.2f} ms")
 
@@ -526,8 +593,8 @@ def main():
         decides = gaps > 4 * unit[name] + 1e-6
         precision[name] = {"value": K * T / dtq, "ms_per_step": dtq * 1e3,
                            "speedup_vs_f32": (dt / args.steps) / dtq,
-                           "argmax_equal_to_f32": float(np.mean(aq == a32)),
-                           "argmax_equal_to_f32_where_gap_decides": float(
+                           "argmax_agreement_vs_f32": float(np.mean(aq == a32)),
+                           "argmax_agreement_vs_f32_where_gap_decides": float(
                                np.mean(aq[gap_idx][decides] == a32[gap_idx][decides])),
                            "decidable_fraction": float(np.mean(decides)),
                            "gap_sample": f"{len(gaps)} samples in 4 seeded 1-minute slices"}

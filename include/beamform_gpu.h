@@ -321,9 +321,15 @@ int bp_network_prepare(bp_network *net, int fmt, const float *station_scale);
  * whatever other requests are ready, and its beam_max / beam_argmax (HOST,
  * n_samples; beam_max 0 where no source is valid) written back.  Requests
  * are admitted in arrival order while their device memory fits
- * `hbm_budget`; requests arriving while batches run join the next launch;
- * up to `in_flight` batches run at once on separate streams.  A result does
- * not depend on the requests it is batched with.  The caller keeps the
+ * `hbm_budget`.  Whenever one of the `in_flight` batch slots (each its own
+ * stream) is free, the ready requests are taken in earliest-deadline order
+ * (deadline = ready time + 4 x the request's estimated GPU time, from its
+ * sources x stations x phases x samples), up to max_batch windows and about
+ * 10 ms of estimated work per batch (at least one request), so short
+ * windows do not wait behind long ones.  Batch tables live in memory
+ * allocated with the server: forming and retiring batches allocates and
+ * frees no device memory.  A result does not depend on the requests it is
+ * batched with.  The caller keeps the
  * feature and output buffers alive until bp_server_wait returns for the
  * request.  Thread-safe; bp_server_last_error() has this thread's message. */
 typedef struct bp_server bp_server;

@@ -69,7 +69,9 @@ def test_served_results_are_the_same_alone_and_in_the_mix(fmt_name):
     nets = networks()
     reqs = requests(nets, 36, seed=5)
     ref = alone_reference(nets, reqs, fmt)
-    srv = Server(hbm_budget=256 << 20, max_batch=8, fmt=fmt, in_flight=2)
+    # one batch slot and a burst of arrivals: while a batch runs, the next
+    # requests are admitted and wait together
+    srv = Server(hbm_budget=256 << 20, max_batch=8, fmt=fmt, in_flight=1)
     try:
         ids = [srv.add_network(*n) for n in nets]
         assert ids == [0, 1, 2]
@@ -81,8 +83,8 @@ def test_served_results_are_the_same_alone_and_in_the_mix(fmt_name):
             assert t.info["batch_windows"] == 1
             alone.append((m.copy(), a.copy()))
         # in the mix: Poisson arrivals much faster than service
-        tickets, summary = poisson_run(srv, reqs, rate=2000.0, seed=3)
-        assert summary["mean_batch_windows"] > 1.0, summary
+        tickets, summary = poisson_run(srv, reqs, rate=1e6, seed=3)
+        assert max(t.info["batch_windows"] for t in tickets) > 1, summary
         for i, t in enumerate(tickets):
             assert bits_equal(t.beam_max, ref[i][0]), ("mix vs device API", i)
             assert np.array_equal(t.beam_argmax, ref[i][1]), ("mix vs device API", i)
