@@ -59,7 +59,7 @@ GPU_SYMBOLS = (
     "bp_parity_stride", "bp_quantize_planar",
     "bp_profiling", "bp_profiling_read",
     "bp_network_beamform_max_view", "bp_network_beamform_view", "bp_network_padding",
-    "bp_network_beamform_max_noalloc",
+    "bp_network_beamform_max_noalloc", "bp_plan",
     "bp_network_beamform_differential", "bp_network_diff_table_bytes",
     "bp_pack_features",
     "bp_batch_create", "bp_batch_launch", "bp_batch_graph_launch", "bp_batch_tune",
@@ -90,6 +90,26 @@ class ServerCounters(ct.Structure):
                 ("batches", ct.c_longlong), ("batched_windows", ct.c_longlong),
                 ("arena_bytes", ct.c_size_t), ("arena_used", ct.c_size_t),
                 ("arena_peak", ct.c_size_t)]
+
+
+class PlanRequest(ct.Structure):
+    """bp_plan_request of include/beamform_gpu.h."""
+    _fields_ = [("n_sources", ct.c_size_t), ("n_stations", ct.c_size_t),
+                ("n_phases", ct.c_size_t), ("n_samples", ct.c_size_t), ("fmt", ct.c_int),
+                ("reduce", ct.c_int), ("hbm_budget", ct.c_size_t),
+                ("time_delays", ct.c_void_p), ("weights_sources", ct.c_void_p),
+                ("order", ct.c_int)]
+
+
+class PlanResult(ct.Structure):
+    """bp_plan_result of include/beamform_gpu.h."""
+    _fields_ = [("source_tile", ct.c_int), ("time_tile", ct.c_int),
+                ("lds_window_limit", ct.c_int), ("tiles_per_class", ct.c_int * 3),
+                ("window_per_class", ct.c_int * 2), ("similarity_order", ct.c_int),
+                ("pad_before", ct.c_longlong), ("pad_after", ct.c_longlong),
+                ("lds_bytes", ct.c_int * 2), ("workgroups_per_cu", ct.c_int * 2),
+                ("network_bytes", ct.c_size_t), ("feature_bytes", ct.c_size_t),
+                ("output_bytes", ct.c_size_t), ("call_bytes", ct.c_size_t), ("fits", ct.c_int)]
 
 
 class FeatureViewC(ct.Structure):
@@ -131,6 +151,7 @@ def _declare_gpu_extension(lib):
     lib.bp_network_beamform_max_view.argtypes = [vp, ct.POINTER(FeatureViewC), ct.c_int, vp,
                                                  vp, vp, vp]
     lib.bp_network_padding.argtypes = [vp, ct.POINTER(_SZ), ct.POINTER(_SZ)]
+    lib.bp_plan.argtypes = [ct.POINTER(PlanRequest), ct.POINTER(PlanResult)]
     lib.bp_network_beamform_max_noalloc.argtypes = [vp, ct.POINTER(FeatureViewC), ct.c_int, vp,
                                                     vp, vp]
     lib.bp_network_beamform_view.argtypes = [vp, ct.POINTER(FeatureViewC), ct.c_int, _SZ, _SZ,

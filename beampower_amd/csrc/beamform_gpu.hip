@@ -682,6 +682,114 @@ bool order_pays(const bp_network *net, size_t n_samples) {
   return saving > cost;
 }
 
+// ---- host-only planning (bp_plan) -------------------------------------------
+
+// Host twin of tile_bounds_kernel's per-tile summary (bp_kernels.h): for
+// source tile j, tile_all[8j..8j+4] = least delay, greatest delay, greatest
+// per-(s,p) least delay, largest per-(s,p) spread over the tile's users of
+// each (s,p), and 1 if a used weight is infinite or NaN; a tile using no
+// station gets 0, 0, 0, 0.  Rows are in position order.
+std::vector<int> host_tile_all(const int *tau, const float *beta, size_t K, size_t S, size_t P,
+                               int n_tiles) {
+  std::vector<int> all(8 * (size_t)n_tiles, 0);
+  const size_t SP = S * P;
+  for (int j = 0; j < n_tiles; ++j) {
+    const size_t k0 = (size_t)j * kTileSources, k1 = std::min(K, k0 + kTileSources);
+    int t_lo = INT_MAX, t_hi = INT_MIN, t_maxlo = INT_MIN, t_spread = 0, nonfinite = 0;
+    for (size_t sp = 0; sp < SP; ++sp) {
+      const size_t st = sp / P;
+      int lo = INT_MAX, hi = INT_MIN;
+      for (size_t k = k0; k < k1; ++k) {
+        const float b = beta[k * S + st];
+        if (b == 0.0f) continue;
+        if (!std::isfinite(b)) nonfinite = 1;
+        const int d = tau[k * SP + sp];
+        lo = std::min(lo, d);
+        hi = std::max(hi, d);
+      }
+      if (lo <= hi) {
+        const long long spread = (long long)hi - (long long)lo;
+        t_lo = std::min(t_lo, lo);
+        t_hi = std::max(t_hi, hi);
+        t_maxlo = std::max(t_maxlo, lo);
+        t_spread = std::max(t_spread, spread > INT_MAX ? INT_MAX : (int)spread);
+      }
+    }
+    int *o = all.data() + 8 * (size_t)j;
+    const bool unused = t_lo == INT_MAX;
+    o[0] = unused ? 0 : t_lo;
+    o[1] = unused ? 0 : t_hi;
+    o[2] = unused ? 0 : t_maxlo;
+    o[3] = t_spread;
+    o[4] = nonfinite;
+  }
+  return all;
+}
+
+// The plan bp_network_create_ordered would make for these host moveouts
+// and weights, in a host-only network object (no device buffers): tiles,
+// windows, padding and the source-order decision, by the same
+// classify_tiles and plan_cost.
+int host_network_plan(const int *tau, const float *beta, size_t K, size_t S, size_t P,
+                      int order, std::unique_ptr<bp_network> &out) {
+  auto make = [&]() {
+    auto n = std::make_unique<bp_network>();
+    n->K = (int)K;
+    n->S = (int)S;
+    n->P = (int)P;
+    n->n_tiles = (int)((K + kTileSources - 1) / kTileSources);
+    return n;
+  };
+  auto net = make();
+  classify_tiles(net.get(), host_tile_all(tau, beta, K, S, P, net->n_tiles));
+  const bool try_order =
+      order == BP_ORDER_SIMILARITY || (order == BP_ORDER_AUTO && K > (size_t)kTileSources);
+  if (try_order) {
+    std::vector<int> ord = bp::similarity_order(tau, K, S * P, kTileSources);
+    std::vector<int> tau_p(K * S * P);
+    std::vector<float> beta_p(K * S);
+    for (size_t i = 0; i < K; ++i) {
+      std::memcpy(&tau_p[i * S * P], tau + (size_t)ord[i] * S * P, S * P * sizeof(int));
+      std::memcpy(&beta_p[i * S], beta + (size_t)ord[i] * S, S * sizeof(float));
+    }
+    auto alt = make();
+    classify_tiles(alt.get(), host_tile_all(tau_p.data(), beta_p.data(), K, S, P, alt->n_tiles));
+    if (order == BP_ORDER_SIMILARITY || plan_cost(alt.get()) < plan_cost(net.get())) {
+      alt->permuted = true;
+      net = std::move(alt);
+    }
+  }
+  out = std::move(net);
+  return BP_OK;
+}
+
+// LDS bytes per work-group of the stack kernel for a window of Lw samples
+// (the staging buffers launch_lds_p would pick).
+template <int FMT, int P, bool NONE>
+int lds_bytes_of(int Lw) {
+  if (BP_STAGE_BUFFERS == 3 && Lds<FMT, P, 3, NONE>::lds_bytes(Lw) <= kLdsPerCU / 2)
+    return Lds<FMT, P, 3, NONE>::lds_bytes(Lw);
+  return Lds<FMT, P, 2, NONE>::lds_bytes(Lw);
+}
+
+template <int FMT, bool NONE>
+int lds_bytes_of(int P, int Lw) {
+  switch (P) {
+    case 1: return lds_bytes_of<FMT, 1, NONE>(Lw);
+    case 2: return lds_bytes_of<FMT, 2, NONE>(Lw);
+    case 3: return lds_bytes_of<FMT, 3, NONE>(Lw);
+    case 4: return lds_bytes_of<FMT, 4, NONE>(Lw);
+    default: return 0;
+  }
+}
+
+int lds_bytes_of(int fmt, int P, bool none, int Lw) {
+  if (Lw <= 0) return 0;
+  if (fmt == BP_F32) return none ? lds_bytes_of<BP_F32, true>(P, Lw) : lds_bytes_of<BP_F32, false>(P, Lw);
+  if (fmt == BP_F16) return none ? lds_bytes_of<BP_F16, true>(P, Lw) : lds_bytes_of<BP_F16, false>(P, Lw);
+  return none ? lds_bytes_of<BP_BF16, true>(P, Lw) : lds_bytes_of<BP_BF16, false>(P, Lw);
+}
+
 // n_samples: the samples of the one call a kOrderAutoWide plan is made for.
 int network_create(const int *tau, const float *beta, size_t K, size_t S, size_t P,
                    int order, hipStream_t st, std::unique_ptr<bp_network> &out,
@@ -1427,6 +1535,69 @@ int bp_network_prepare(bp_network *net, int fmt, const float *station_scale) try
   BP_TRY(st.create());
   std::shared_ptr<Schedule> sc;
   return ensure_schedule(net, fmt, station_scale, st.s, sc);
+} BP_CATCH
+
+int bp_plan(const bp_plan_request *req, bp_plan_result *res) try {
+  if (!req || !res) return fail(BP_EINVAL, "NULL argument");
+  *res = bp_plan_result{};
+  const size_t K = req->n_sources, S = req->n_stations, P = req->n_phases, T = req->n_samples;
+  const int fmt = req->fmt;
+  if (fmt != BP_F32 && fmt != BP_F16 && fmt != BP_BF16)
+    return fail(BP_EINVAL, "unknown feature format %d", fmt);
+  if (req->reduce != 0 && req->reduce != 1)
+    return fail(BP_EINVAL, "reduce must be 0 ('max') or 1 ('none'), not %d", req->reduce);
+  if (req->order != BP_ORDER_NATURAL && req->order != BP_ORDER_SIMILARITY &&
+      req->order != BP_ORDER_AUTO)
+    return fail(BP_EINVAL, "unknown source order %d", req->order);
+  BP_TRY(check_shapes(T, K, S, P));
+  if ((req->time_delays == nullptr) != (req->weights_sources == nullptr))
+    return fail(BP_EINVAL, "time_delays and weights_sources come together");
+  const bool none = req->reduce == 1;
+  res->source_tile = kTileSources;
+  res->time_tile = kTT;
+  res->lds_window_limit = lds_window_limit((int)P);
+  for (int c = 0; c < 3; ++c) res->tiles_per_class[c] = -1;
+  res->window_per_class[0] = res->window_per_class[1] = -1;
+  res->similarity_order = -1;
+  res->pad_before = res->pad_after = -1;
+  const int n_tiles = (int)((K + kTileSources - 1) / kTileSources);
+  bool lds_tiles = res->lds_window_limit > 0 && K > 0;
+  long long pad_lo = 0, pad_hi = 0;
+  bool permuted = false;
+  if (req->time_delays && K > 0 && S > 0 && P > 0) {
+    std::unique_ptr<bp_network> net;
+    BP_TRY(host_network_plan(req->time_delays, req->weights_sources, K, S, P, req->order, net));
+    for (int c = 0; c < 3; ++c) res->tiles_per_class[c] = (int)net->tiles[c].size();
+    res->window_per_class[0] = net->Lw[0];
+    res->window_per_class[1] = net->Lw[1];
+    res->similarity_order = net->permuted ? 1 : 0;
+    res->pad_before = pad_lo = net->pad_lo;
+    res->pad_after = pad_hi = net->pad_hi;
+    permuted = net->permuted;
+    lds_tiles = has_lds_tiles(net.get());
+    const int wave_slots = 4 * BP_WAVES_PER_EU / kW;  // work-groups per CU by waves
+    for (int c = 0; c < 2; ++c) {
+      res->lds_bytes[c] = lds_bytes_of(fmt, (int)P, none, net->Lw[c]);
+      res->workgroups_per_cu[c] =
+          res->lds_bytes[c] > 0 ? std::min(kLdsPerCU / res->lds_bytes[c], wave_slots) : 0;
+    }
+  }
+  // resident network: moveouts, weights, per-source bounds, tile tables,
+  // source order, one schedule of the format with its station scales
+  const size_t sched = lds_tiles ? (size_t)n_tiles * S * kTileSources * (P + 1) * 4 : 0;
+  res->network_bytes = K * S * P * 4 + K * S * 4 + 2 * K * 4 + (size_t)n_tiles * S * P * 4 +
+                       (size_t)n_tiles * (8 + 1 + 1) * 4 + (permuted ? K * 4 : 0) + sched +
+                       S * 4;
+  // the window in rows padded for the network (16-bit: two parity planes)
+  const size_t rows = fmt == BP_F32 ? S * P : 2 * S * P, es = fmt == BP_F32 ? 4 : 2;
+  size_t stride = (size_t)pad_lo + T + (size_t)pad_hi;
+  if (fmt != BP_F32) stride += stride & 1;
+  res->feature_bytes = rows * stride * es;
+  res->output_bytes = none ? K * T * 4 : T * (4 + 4 + 8);
+  // a host-pointer call also uploads the interleaved (S, T, P) features
+  res->call_bytes = res->network_bytes + res->feature_bytes + S * T * P * 4 + res->output_bytes;
+  res->fits = req->hbm_budget == 0 || res->call_bytes <= req->hbm_budget;
+  return BP_OK;
 } BP_CATCH
 
 int bp_network_padding(const bp_network *net, size_t *pad_before, size_t *pad_after) try {

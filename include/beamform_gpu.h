@@ -222,6 +222,54 @@ int bp_network_beamform_differential(bp_network *net, const bp_feature_view *fea
 int bp_network_diff_table_bytes(const bp_network *net, size_t *delay_bytes,
                                 size_t *weight_bytes, int *weight_form);
 
+/* Host-only plan of a call (no counterpart in the reference; no device is
+ * touched): what the engine would allocate and how it would tile a window,
+ * so that admission and kernel-fit decisions can be made before any
+ * upload.  Request: the shape, the feature storage format, the reduction
+ * (0 'max', 1 'none'), an HBM budget in bytes (0: not checked) and
+ * optionally the moveouts and weights (HOST, the layouts of beamform_max)
+ * with a source order (BP_ORDER_*, as bp_network_create_ordered).  With
+ * them the plan is the one bp_network_create_ordered would make: tile
+ * classes, LDS windows, row padding, source order; without them those
+ * fields are -1 and the sizes assume every tile in the LDS kernel.
+ * Result:
+ *   source_tile, time_tile      kernel shape of this build
+ *   lds_window_limit            largest LDS window (samples) for n_phases,
+ *                               0 when the LDS kernel does not take n_phases
+ *   lds_bytes[2], workgroups_per_cu[2]  per LDS class (small, large window)
+ *   network_bytes               resident moveouts, weights, bounds, tile
+ *                               tables and one schedule of the format
+ *   feature_bytes               the window as the engine stores it (rows
+ *                               padded for the network, in the format)
+ *   output_bytes                'max': beam_max, argmax and key scratch;
+ *                               'none': the (n_sources, n_samples) beams
+ *   call_bytes                  what a host-pointer call (beamform /
+ *                               beamform_max) allocates in all: the above
+ *                               plus the interleaved upload of the features
+ *   fits                        call_bytes <= hbm_budget (1 without budget) */
+typedef struct {
+  size_t n_sources, n_stations, n_phases, n_samples;
+  int fmt;
+  int reduce;
+  size_t hbm_budget;
+  const int *time_delays;
+  const float *weights_sources;
+  int order;
+} bp_plan_request;
+typedef struct {
+  int source_tile, time_tile;
+  int lds_window_limit;
+  int tiles_per_class[3];
+  int window_per_class[2];
+  int similarity_order;
+  long long pad_before, pad_after;
+  int lds_bytes[2];
+  int workgroups_per_cu[2];
+  size_t network_bytes, feature_bytes, output_bytes, call_bytes;
+  int fits;
+} bp_plan_result;
+int bp_plan(const bp_plan_request *request, bp_plan_result *result);
+
 /* Zero padding (samples) a feature view of this network needs before and
  * after every row so that no LDS tile takes the guarded edge path. */
 int bp_network_padding(const bp_network *net, size_t *pad_before, size_t *pad_after);
