@@ -2,6 +2,8 @@
 
     python -m beampower_amd.tune [--stations 50] [--grid 80 80 30] [--box-km 100]
                                  [--samples 90000] [--fmt f32|bf16|f16] [--reps 6] [--apply]
+                                 [--set shapes|tiles] [--build-dir DIR]
+                                 [--build-only | --prebuilt]
 
 The shift-and-stack kernel keeps a tile's accumulators in registers, so its
 source tile (sources per wave x waves per work-group), its time tile (64 x
@@ -17,6 +19,8 @@ row-padded features with HIP events, the builds interleaved over `reps`
 rounds.  All builds must give bit-identical maxima and argmax.  --apply
 writes the fastest shape to beampower_amd/lib/build_flags.json, which
 `python -m beampower_amd.build` (and __graft_entry__.build()) then use.
+--build-only builds the candidates into --build-dir on any host (hipcc
+cross-compiles), and --prebuilt times those builds on a GPU host.
 
 The run-time configuration of a batch (block order, staging buffers, grid)
 is tuned separately by bp_batch_tune.
@@ -52,6 +56,27 @@ CANDIDATES = (
     {"BP_KW": 6, "BP_WAVES_PER_EU": 6, "BP_PIPE_DEPTH": 1, "BP_SMALL_SHARE_PCT": 90},
     {"BP_KW": 6, "BP_WAVES_PER_EU": 6, "BP_PIPE_DEPTH": 1, "BP_SMALL_SHARE_PCT": 100},
 )
+# Source tiles (sources per wave x waves per work-group) and time tiles
+# (64 x samples per lane) toward the batched workload's ranges, as far as
+# they stay register-resident and fit a work-group: a lane holds
+# KW x NACC <= 64 accumulators (48-64 VGPRs of the 80 or 128 a wave has at 6
+# or 4 waves per SIMD), a work-group holds at most 16 waves (1,024 threads),
+# and the LDS window of one station (two staging buffers) fits a CU's
+# 160 KiB for windows up to 2,048 samples.  So source tiles of 32-128 and
+# time tiles of 512 or 1,024 samples; wider tiles would spill accumulators
+# or exceed a work-group.
+TILE_CANDIDATES = (
+    {"BP_KW": 6, "BP_W": 8, "BP_WAVES_PER_EU": 6, "BP_PIPE_DEPTH": 1},      # 48 x 512
+    {"BP_KW": 6, "BP_W": 12, "BP_WAVES_PER_EU": 6, "BP_PIPE_DEPTH": 1},     # 72 x 512
+    {"BP_KW": 8, "BP_W": 8, "BP_WAVES_PER_EU": 4, "BP_PIPE_DEPTH": 1},      # 64 x 512
+    {"BP_KW": 8, "BP_W": 12, "BP_WAVES_PER_EU": 4, "BP_PIPE_DEPTH": 1},     # 96 x 512
+    {"BP_KW": 8, "BP_W": 16, "BP_WAVES_PER_EU": 4, "BP_PIPE_DEPTH": 1},     # 128 x 512
+    {"BP_KW": 4, "BP_W": 16, "BP_WAVES_PER_EU": 6, "BP_PIPE_DEPTH": 1},     # 64 x 512
+    {"BP_KW": 4, "BP_W": 8, "BP_NACC": 16, "BP_WAVES_PER_EU": 4, "BP_PIPE_DEPTH": 1},   # 32 x 1024
+    {"BP_KW": 3, "BP_W": 16, "BP_NACC": 16, "BP_WAVES_PER_EU": 4, "BP_PIPE_DEPTH": 1},  # 48 x 1024
+    {"BP_KW": 4, "BP_W": 16, "BP_NACC": 16, "BP_WAVES_PER_EU": 4, "BP_PIPE_DEPTH": 1},  # 64 x 1024
+)
+CANDIDATE_SETS = {"shapes": CANDIDATES, "tiles": TILE_CANDIDATES}
 FORMATS = {"f32": 0, "bf16": 1, "f16": 2}
 
 
@@ -59,10 +84,15 @@ def name_of(macros):
     return ",".join(f"{k[3:].lower()}={v}" for k, v in sorted(macros.items()))
 
 
-def build_all(candidates, out_dir, jobs=4, log=print):
-    """Build one library per candidate into out_dir; returns the paths."""
+def build_all(candidates, out_dir, jobs=4, log=print, prebuilt=False):
+    """Build one library per candidate into out_dir (or, with prebuilt, use
+    the libraries already there); returns the paths."""
     def one(i):
-        path = os.path.join(out_dir, f"beamform_gpu_{i}.so")
+        path = os.path.join(out_dir, f"beamform_gpu_{name_of(candidates[i])}.so")
+        if prebuilt:
+            if not os.path.exists(path):
+                raise RuntimeError(f"{path} is not built (run with --build-only first)")
+            return path
         res = subprocess.run(bp_build.command(path, bp_build.shape_flags(candidates[i])),
                              capture_output=True, text=True)
         if res.returncode != 0:
@@ -161,13 +191,29 @@ def main(argv=None):
     ap.add_argument("--fmt", choices=sorted(FORMATS), default="f32")
     ap.add_argument("--reps", type=int, default=6)
     ap.add_argument("--jobs", type=int, default=4, help="parallel builds")
+    ap.add_argument("--set", choices=sorted(CANDIDATE_SETS), default="shapes",
+                    help="candidate set: kernel shapes around the default, or source/time tiles")
+    ap.add_argument("--build-dir", default=None,
+                    help="keep the candidate builds in this directory")
+    ap.add_argument("--build-only", action="store_true",
+                    help="build the candidates into --build-dir and stop (no GPU needed)")
+    ap.add_argument("--prebuilt", action="store_true",
+                    help="time the builds already in --build-dir instead of building")
     ap.add_argument("--apply", action="store_true", help="record the fastest shape for build()")
     args = ap.parse_args(argv)
+    cands_m = CANDIDATE_SETS[args.set]
+    log = lambda m: print(f"[tune] {m}", flush=True)
+    if args.build_only or args.prebuilt:
+        if not args.build_dir:
+            sys.exit("--build-only and --prebuilt need --build-dir")
+        os.makedirs(args.build_dir, exist_ok=True)
+    if args.build_only:
+        build_all(cands_m, args.build_dir, args.jobs, log)
+        return None
     import torch
     from . import synthetic
     if not torch.cuda.is_available():
         sys.exit("the tuner times builds on the GPU: no GPU is visible")
-    log = lambda m: print(f"[tune] {m}", flush=True)
     _, _, tau = synthetic.make_network(2, args.stations, args.box_km, tuple(args.grid))
     beta = np.ones(tau.shape[:2], np.float32)
     rng = np.random.default_rng(2)
@@ -177,10 +223,11 @@ def main(argv=None):
     F = (u[:, :, None, :] * alpha[:, :, :, None]).sum(1).contiguous()  # (S, P, T)
     del u
     log(f"workload: {tau.shape[0]} sources, {S} stations, {args.samples} samples, {args.fmt}")
-    with tempfile.TemporaryDirectory(prefix="bp_tune_") as d:
-        paths = build_all(CANDIDATES, d, args.jobs, log)
+    with tempfile.TemporaryDirectory(prefix="bp_tune_") as tmp:
+        d = args.build_dir or tmp
+        paths = build_all(cands_m, d, args.jobs, log, prebuilt=args.prebuilt)
         cands = []
-        for p, m in zip(paths, CANDIDATES):
+        for p, m in zip(paths, cands_m):
             cands.append(Candidate(p, tau, beta, F, FORMATS[args.fmt]))
             log(f"planned {name_of(m)}")
         ms = time_all(cands, args.reps)
@@ -190,7 +237,7 @@ def main(argv=None):
         for c in cands:
             c.close()
     work = tau.shape[0] * args.samples
-    rows = sorted(zip(ms, CANDIDATES, same), key=lambda r: r[0])
+    rows = sorted(zip(ms, cands_m, same), key=lambda r: r[0])
     for t, m, ok in rows:
         print(f"{name_of(m):48s} {t:9.3f} ms  {work / (t * 1e-3):.3e} source·samples/s"
               f"{'' if ok else '  RESULTS DIFFER'}", flush=True)

@@ -197,10 +197,15 @@ def test_tuner_candidates_and_build_flags():
         assert f"#ifndef {m}\n#define {m} " in src, m  # every tunable macro has a default
     defaults = {m: int(v) for m, v in re.findall(r"#define (BP_[A-Z_]+) (\d+)\b", src)
                 if m in bp_build.SHAPE_MACROS}
-    first = tune.CANDIDATES[0]
-    assert all(defaults[m] == v for m, v in first.items())
-    keys = [tuple(sorted(c.items())) for c in tune.CANDIDATES]
-    assert len(set(keys)) == len(keys)
+    for cands in tune.CANDIDATE_SETS.values():
+        first = cands[0]
+        assert all(defaults[m] == v for m, v in first.items())
+        keys = [tuple(sorted(c.items())) for c in cands]
+        assert len(set(keys)) == len(keys)
+        for c in cands:  # register-resident accumulators, at most 16 waves per work-group
+            kw, nacc, w = c.get("BP_KW"), c.get("BP_NACC", defaults["BP_NACC"]), c.get("BP_W", 8)
+            assert kw * nacc <= 64 and w <= 16, c
+    assert tune.name_of({"BP_KW": 6, "BP_W": 12}) == "kw=6,w=12"
     flags = bp_build.shape_flags({"BP_KW": 5, "BP_PIPE_DEPTH": 2})
     assert flags == ["-DBP_KW=5", "-DBP_PIPE_DEPTH=2"]
     assert all(f in bp_build.command("x.so", flags) for f in flags)
