@@ -1550,6 +1550,18 @@ __device__ __forceinline__ float feature_row(const StackArgs &a, unsigned row, u
   return __uint_as_float((unsigned)h << 16);
 }
 
+// Buffer descriptor over [base, base + bytes), base wave-uniform and bytes
+// below 2^32: raw loads take a 32-bit per-lane offset and a scalar offset
+// (no 64-bit address arithmetic), and a load past the range returns zero.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t range_rsrc(const void *base,
+                                                             unsigned long long bytes) {
+  const unsigned long long b = (unsigned long long)base;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+  return __builtin_amdgcn_make_buffer_rsrc((void *)(((unsigned long long)hi << 32) | lo),
+                                           (short)0, (int)(unsigned)bytes, 0x00020000);
+}
+
 // PT: the phase count when it is 1..4 (a chunk of CS = 16 / PT stations is
 // loaded at once), 0 for any other count (station by station).  The
 // per-source bounds of the reference (beamform.c:10-47) are taken in the
@@ -1585,8 +1597,6 @@ __global__ __launch_bounds__(kDiffSources) void differential_kernel(
   // 32-bit offsets inside the wave's block: 64 * S_pad * P < 2^30 (checked by the host)
   const int *tb = tauT + blk * S_pad * (PAIRS ? 1 : P) * kDiffBlock + lane;
   const float *wb = betaT + blk * S_pad * kDiffBlock + lane;
-  const unsigned *mb = reinterpret_cast<const unsigned *>(betaT) +
-                       blk * diff_mask_words(S, P) * kDiffBlock + lane;
   float acc = 0.0f;
   bool used = false, inside = true;
 
@@ -1624,13 +1634,31 @@ __global__ __launch_bounds__(kDiffSources) void differential_kernel(
     int d[NE];
     float b[CS];
   };
+  // Every load of a chunk is a buffer load: the wave's block of a table, or
+  // the features, under a wave-uniform descriptor, with the lane's 32-bit
+  // offset and a scalar offset per entry (a 256-byte table line per
+  // station or (station, phase)).  The host keeps the features under 2^30
+  // elements (2^32 bytes, 4 * ldf bytes per row in every format).
+  const unsigned lane4 = 4u * (unsigned)lane;
+  constexpr int DE = PAIRS ? 1 : (PT > 0 ? PT : 1);  // delay words per station
+  const int MW = diff_mask_words(S, P);
+  const __amdgpu_buffer_rsrc_t rs_tau =
+      range_rsrc(tauT + blk * S_pad * DE * kDiffBlock, (unsigned long long)S_pad * DE * 256u);
+  const __amdgpu_buffer_rsrc_t rs_w =
+      MASK ? range_rsrc(betaT + blk * MW * kDiffBlock, (unsigned long long)MW * 256u)
+           : range_rsrc(betaT + blk * S_pad * kDiffBlock, (unsigned long long)S_pad * 256u);
+  const unsigned row_bytes = 4u * (unsigned)a.ldf;
+  const __amdgpu_buffer_rsrc_t rs_f =
+      range_rsrc(a.F, (unsigned long long)S * P * row_bytes);
   auto load = [&](int s0, Chunk &ch) {
     // used bits of the chunk's stations: one word, or two when the chunk
     // straddles a word boundary (chunks start at multiples of CS)
     unsigned m0 = 0u, m1 = 0u;
     if (MASK) {
-      m0 = mb[(unsigned)(s0 >> 5) * kDiffBlock];
-      if (32 % CS != 0) m1 = mb[(unsigned)((s0 >> 5) + 1) * kDiffBlock];
+      m0 = __builtin_amdgcn_raw_buffer_load_b32(rs_w, lane4, (unsigned)(s0 >> 5) * 256u, 0);
+      if (32 % CS != 0)
+        m1 = __builtin_amdgcn_raw_buffer_load_b32(rs_w, lane4, (unsigned)((s0 >> 5) + 1) * 256u,
+                                                  0);
     }
 #pragma unroll
     for (int c = 0; c < CS; ++c) {
@@ -1639,48 +1667,73 @@ __global__ __launch_bounds__(kDiffSources) void differential_kernel(
         const unsigned m = (32 % CS == 0 || (sc >> 5) == (s0 >> 5)) ? m0 : m1;
         ch.b[c] = ((m >> (sc & 31)) & 1u) ? cw : 0.0f;
       } else {
-        ch.b[c] = wb[(unsigned)(s0 + c) * kDiffBlock];
+        ch.b[c] = __uint_as_float(
+            __builtin_amdgcn_raw_buffer_load_b32(rs_w, lane4, (unsigned)(s0 + c) * 256u, 0));
       }
       if (PAIRS) {
-        const unsigned w = (unsigned)tb[(unsigned)(s0 + c) * kDiffBlock];
+        const unsigned w =
+            __builtin_amdgcn_raw_buffer_load_b32(rs_tau, lane4, (unsigned)(s0 + c) * 256u, 0);
         ch.d[c * PT] = (int)(short)(w & 0xffffu);
         ch.d[c * PT + 1] = (int)(short)(w >> 16);
       } else {
 #pragma unroll
         for (int p = 0; p < PT; ++p)
-          ch.d[c * PT + p] = tb[(unsigned)((s0 + c) * PT + p) * kDiffBlock];
+          ch.d[c * PT + p] = (int)__builtin_amdgcn_raw_buffer_load_b32(
+              rs_tau, lane4, (unsigned)((s0 + c) * PT + p) * 256u, 0);
       }
     }
   };
+  // 32-bit sample arithmetic: t < 2^30, and t + d computed modulo 2^32 lies
+  // in [0, T) exactly when the true sum does (a wrapped negative sum is at
+  // least 2^31 > T).
+  //
+  // A live entry (used station, delay in range) loads its feature at byte
+  // offset row * row_bytes + t_g * ESZ from the feature descriptor; every
+  // other entry loads at offset 0xffffffff, past the descriptor's range
+  // (below 2^32 bytes), which reads no memory and returns +0.  The chain
+  // then takes every entry as fma(w', x, acc) with w' = -0.0 for an unused
+  // station: fma(-0, +0, acc) is acc exactly, signed zeros and NaN
+  // included.  A used entry out of range would add w * (+0), but it marks
+  // the source out of range and its beam is never written; every written
+  // beam is the reference's chain over its used stations.  The entry tests
+  // are thus made once, at the gathers, and the chain is one FMA an entry.
+  const unsigned tu = (unsigned)t, Tu = (unsigned)T;
+  constexpr unsigned ESZ = FMT == BP_F32 ? 4u : 2u;
   auto gather = [&](int s0, const Chunk &ch, float (&x)[NE]) {
-    // entries not used or out of range read element 0 of row 0
+    // row offsets as a running scalar sum (an opaque stride keeps the
+    // compiler from hoisting one constant per entry into scalar registers)
+    unsigned rb = row_bytes;
+    asm volatile("" : "+s"(rb));
+    unsigned roff = (unsigned)(s0 * PT) * rb;
 #pragma unroll
     for (int c = 0; c < CS; ++c) {
+      const bool u = ch.b[c] != 0.0f;
+      used |= u;
 #pragma unroll
       for (int p = 0; p < PT; ++p) {
-        const long long tg = t + ch.d[c * PT + p];
-        const bool live = ch.b[c] != 0.0f && tg >= 0 && tg < T;
-        x[c * PT + p] = feature_row<FMT>(a, live ? (unsigned)((s0 + c) * PT + p) : 0u,
-                                         live ? (unsigned)tg : 0u);
+        const unsigned tg = tu + (unsigned)ch.d[c * PT + p];
+        const bool in = tg < Tu;
+        inside &= !u || in;
+        const unsigned off = (u && in) ? roff + tg * ESZ : 0xffffffffu;
+        if (FMT == BP_F32) {
+          x[c * PT + p] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs_f, off, 0, 0));
+        } else {
+          const unsigned short h = __builtin_amdgcn_raw_buffer_load_b16(rs_f, off, 0, 0);
+          x[c * PT + p] = FMT == BP_F16 ? (float)__builtin_bit_cast(_Float16, h)
+                                        : __uint_as_float((unsigned)h << 16);
+        }
+        roff += rb;
       }
     }
   };
   auto chain = [&](int s0, const Chunk &ch, const float (&x)[NE]) {
 #pragma unroll
     for (int c = 0; c < CS; ++c) {
-      const bool u = ch.b[c] != 0.0f;
       const float sc =
           (FMT == BP_F32 || !a.scale) ? 1.0f : a.scale[s0 + c < S ? s0 + c : S - 1];
-      const float wt = applied_weight<FMT>(ch.b[c], sc);
-      used |= u;
+      const float wt = ch.b[c] != 0.0f ? applied_weight<FMT>(ch.b[c], sc) : -0.0f;
 #pragma unroll
-      for (int p = 0; p < PT; ++p) {
-        const long long tg = t + ch.d[c * PT + p];
-        const bool in = tg >= 0 && tg < T;
-        inside &= !u || in;
-        const float y = fmaf(wt, x[c * PT + p], acc);
-        acc = (u && in) ? y : acc;
-      }
+      for (int p = 0; p < PT; ++p) acc = fmaf(wt, x[c * PT + p], acc);
     }
   };
   Chunk c0, c1;

@@ -237,3 +237,36 @@ def test_uniform_weights_as_used_bits(gpu_lib, kind, P):
         net.beamform(V, 0, beam=direct)
         col = direct[:, (T - 1) // 2].contiguous().cpu().numpy()
         assert bits_equal(bv.cpu().numpy(), col), f
+
+
+def test_unused_entries_contribute_nothing(gpu_lib):
+    """Unused stations whose features are NaN or infinite and whose delays
+    point anywhere in the int32 range add nothing, signed zeros included:
+    beams whose every used product underflows end at -0.0, and sources
+    with one used delay out of range keep the caller's value."""
+    rng = np.random.default_rng(77)
+    S, P, T, K = 13, 2, 301, 1100
+    F = np.full((S, T, P), 1e-30, np.float32)
+    F[:, ::7, :] = 0.5
+    unused_rows = rng.random(S) < 0.4
+    F[unused_rows] = np.nan
+    F[unused_rows, ::3] = np.inf
+    tau = rng.integers(-T // 2 + 1, T // 2, size=(K, S, P)).astype(np.int32)
+    beta = np.where(rng.random((K, S)) < 0.7, 1.0, 0.0).astype(np.float32)
+    beta[:, unused_rows] = np.where(rng.random((K, unused_rows.sum())) < 0.5, 0.0, -0.0)
+    tau[:, unused_rows] = rng.integers(-2**31, 2**31 - 1, size=(K, unused_rows.sum(), P),
+                                       dtype=np.int64).astype(np.int32)
+    tiny = rng.random(K) < 0.3  # products of -1e-30 and 1e-30 underflow to -0.0
+    beta[tiny] = np.where(beta[tiny] != 0, -1e-30, beta[tiny]).astype(np.float32)
+    tau[tiny] = (tau[tiny] // 7) * 7 + 1  # off the 0.5 samples (t = 150 = 7 * 21 + 3)
+    oob = rng.random(K) < 0.1
+    for k in np.flatnonzero(oob):
+        s = int(rng.choice(np.flatnonzero(~unused_rows)))
+        beta[k, s] = 1.0
+        tau[k, s, 1] = T  # out of range at the centre sample
+    init = np.full(K, 7.0, np.float32)
+    ref = oracle.beamform_differential(F, tau, beta, init=init)
+    assert np.signbit(ref[tiny & ~oob & (ref == 0)]).any()  # -0.0 beams present
+    assert np.isfinite(ref).all()
+    got = abi_differential(gpu_lib, F, tau, beta, init)
+    assert bits_equal(got, ref)
