@@ -2328,7 +2328,7 @@ int bp_batch_tune(bp_batch *b, int reps, bp_batch_config *tried, int max_tried, 
   std::vector<bp_batch_config> cands;
   for (int tg : {1, 2, 4, 8})
     for (int nb : {2, 3})
-      for (int bpc : {-1, 0, 1, 2}) cands.push_back(bp_batch_config{tg, nb, bpc, 0.0f});
+      for (int bpc : {-1, 0, 1, 2, 3}) cands.push_back(bp_batch_config{tg, nb, bpc, 0.0f});
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));

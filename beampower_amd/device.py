@@ -416,7 +416,7 @@ class Batch:
         """Launch configuration (bp_batch_config; the defaults are the
         library's): time tiles per block-order group, staging buffers, and
         -1 = windows that fill the GPU alone in launches of their own, the
-        others fused; 0 = all fused, one work-group per item; 1, 2 = all
+        others fused; 0 = all fused, one work-group per item; 1-3 = all
         fused, a persistent grid of that many work-groups per CU."""
         c = BatchConfig(time_group, stage_buffers, blocks_per_cu, 0.0)
         check(self.lib.bp_batch_set_config(self._h, ct.byref(c)), "bp_batch_set_config",

@@ -227,6 +227,71 @@ def bench_batched(args, log):
     return out
 
 
+def bench_batched_mix(args, log):
+    """The batched form on a serving-like mix: requests of the four serving
+    networks (20/50/80/120 stations, 10k-300k sources, 30 s - 10 min
+    windows, networks shared by several windows), FP32 features, in one
+    batch under each grid policy -- large windows in launches of their own
+    (the default), everything fused with one work-group per work item, and
+    one persistent launch of 1, 2 or 3 work-groups per CU -- against the
+    windows launched one by one; all bit-identical."""
+    import numpy as np
+    import torch
+    from beampower_amd import device as bpd
+    from beampower_amd import synthetic
+
+    nets_h = synthetic.serving_networks()
+    reqs = synthetic.serving_requests(nets_h, args.mix_windows, seed=77)
+    nets = [bpd.Network(t, b) for t, b, _ in nets_h]
+    alphas = [torch.from_numpy(a).cuda() for _, _, a in nets_h]
+    packed = {}
+    wins, work = [], 0
+    for k, u in reqs:
+        key = (k, id(u))
+        if key not in packed:
+            F = bpd.prestack_planar(torch.from_numpy(u).cuda(), alphas[k])
+            packed[key] = bpd.pack(F, bpd.BP_F32, *nets[k].padding())
+            del F
+        wins.append((nets[k], packed[key], 0))
+        work += nets_h[k][0].shape[0] * u.shape[2]
+    torch.cuda.synchronize()
+
+    def timed(fn, reps=3):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    def alone():
+        for net, F, oob in wins:
+            net.beamform_max(F, oob)
+
+    ref = [tuple(x.cpu().numpy() for x in net.beamform_max(F, oob)) for net, F, oob in wins]
+    out = {"windows": len(wins), "source_samples": work,
+           "windows_one_by_one_ms": timed(alone) * 1e3, "grids": []}
+    b = bpd.Batch(wins)
+    for bpc in (-1, 0, 1, 2, 3):
+        b.set_config(blocks_per_cu=bpc)
+        t = timed(b.graph_launch)
+        b.graph_launch()
+        torch.cuda.synchronize()
+        same = all(np.array_equal(b.beam_max[i].cpu().numpy().view(np.uint32),
+                                  ref[i][0].view(np.uint32))
+                   and np.array_equal(b.beam_argmax[i].cpu().numpy(), ref[i][1])
+                   for i in range(len(wins)))
+        out["grids"].append({"blocks_per_cu": bpc, "graph_ms": t * 1e3, "value": work / t,
+                             "identical_to_windows_alone": bool(same)})
+        log(f"batched mix, blocks_per_cu {bpc}: {t * 1e3:.1f} ms")
+    b.set_config()
+    del b, wins, packed
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    return out
+
+
 def bench_serving(args, log):
     """Serving form: four resident networks (20/50/80/120 stations,
     10k/50k/192k/300k sources), window requests of 30 s - 10 min at 25 Hz on
@@ -484,6 +549,8 @@ def main():
     ap.add_argument("--batch-windows", type=int, default=256,
                     help="windows of the batched form (256 = the full batched workload; 0 = skip)")
     ap.add_argument("--tune", action="store_true", help="autotune the batched form first")
+    ap.add_argument("--mix-windows", type=int, default=48,
+                    help="windows of the batched serving-like mix (0 = skip)")
     ap.add_argument("--serve-rates", type=float, nargs="*",
                     default=[5.0, 50.0, 100.0, 125.0, 200.0],
                     help="offered request rates of the serving form (none = skip)")
@@ -664,6 +731,8 @@ def main():
     del F_plain
     torch.cuda.empty_cache()
     batched = bench_batched(args, log) if args.batch_windows > 0 else None
+    if batched is not None and args.mix_windows > 0:
+        batched["serving_mix"] = bench_batched_mix(args, log)
     serving = bench_serving(args, log) if args.serve_rates else None
     differential = None if args.no_differential else bench_differential(args, log)
 

@@ -318,9 +318,11 @@ int bp_prestack_planar(const float *u_sct, const float *weights_phases,
  * window per call): n_windows windows, each with its own network, feature
  * view, bounds mode and DEVICE outputs (beam_max, beam_argmax and keys
  * scratch of n_samples elements, as for bp_network_beamform_max), all in one
- * feature format and with one number of phases.  The LDS-class tiles of
- * every window run in one persistent launch per window class; results are
- * bit-identical to bp_network_beamform_max_view on each window alone.
+ * feature format and with one number of phases.  By default a window with
+ * at least 4096 work items (it fills the GPU alone) runs its LDS-class
+ * tiles in a launch of its own and the other windows share one fused launch
+ * per window class (see bp_batch_config); results are bit-identical to
+ * bp_network_beamform_max_view on each window alone.
  * Windows of one network may have different station scales.  Creation
  * builds missing schedules synchronously, and the batch holds the schedules
  * of its windows: later calls or batches of the same networks with other
@@ -330,17 +332,18 @@ int bp_prestack_planar(const float *u_sct, const float *weights_phases,
  * (no allocation, no host synchronisation); bp_batch_graph_launch captures
  * that sequence into a hipGraph on first use and replays the graph after.
  * bp_batch_tune times every configuration of the search space (time tiles
- * per block-order group 1/2/4/8, staging buffers 2/3, persistent grid of 1
- * or 2 work-groups per CU or one work-group per item) on this batch, keeps the fastest and remembers it for
- * batches of the same shape in this process. */
+ * per block-order group 1/2/4/8, staging buffers 2/3, and the five grid
+ * policies of blocks_per_cu -1..3) on this batch, keeps the fastest and
+ * remembers it for batches of the same shape in this process. */
 typedef struct bp_batch bp_batch;
 typedef struct {
   int time_group;     /* time tiles per block-order group (default 8; 1 = time-major) */
   int stage_buffers;  /* 2 or 3 (3 only where two work-groups still fit a CU) */
   int blocks_per_cu;  /* -1 (default): windows of >= 4096 work items in launches of their
                          own, the others fused with one work-group per work item; 0: all
-                         fused, one work-group per work item; 1, 2: all fused, a persistent
-                         grid of that many work-groups per CU */
+                         fused, one work-group per work item; 1-4: all fused, one persistent
+                         launch of that many work-groups per CU (fewer where their LDS
+                         does not fit) */
   float ms;           /* measured time of the whole batch (tuning output) */
 } bp_batch_config;
 int bp_batch_create(int n_windows, bp_network *const *networks,

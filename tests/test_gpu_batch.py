@@ -94,7 +94,7 @@ def test_batch_matches_windows_alone(fmt_name):
         torch.cuda.synchronize()
         assert_batch_equals(batch, ref, f"blocks_per_cu {bpc}")
     best, tried = batch.tune(reps=1)
-    assert len(tried) == 32 and all(t["ms"] > 0 for t in tried)
+    assert len(tried) == 40 and all(t["ms"] > 0 for t in tried)
     assert best["ms"] == min(t["ms"] for t in tried)
     assert batch.config()["time_group"] == best["time_group"]
     for m in batch.beam_max:
