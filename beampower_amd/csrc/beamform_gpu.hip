@@ -387,7 +387,7 @@ int ensure_schedule(bp_network *net, int fmt, const float *scales, hipStream_t s
                        net->tau.as<int>(), net->beta.as<float>(), net->tile_lo.as<int>(),
                        net->tile_lw.as<int>(),
                        fmt == BP_F32 ? (const float *)nullptr : sc->scale.as<float>(), net->K,
-                       net->S, net->P, kTileSources, fmt, n, sc->sched.as<int>());
+                       net->S, net->P, kTileSources, kKW, fmt, n, sc->sched.as<int>());
     HIP_TRY(hipGetLastError());
   }
   // the host vector `want` is read by the copy above: finish before returning

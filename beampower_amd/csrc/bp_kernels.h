@@ -172,8 +172,10 @@ constexpr int kTimeTile = 64 * BP_NACC;
 constexpr int kMaxWindow = 2048;  // largest LDS window (samples)
 __host__ __device__ __forceinline__ int window_slot_bytes(int Lw) { return (Lw + 63) / 64 * 1024; }
 
-// Per-tile schedule of the LDS kernel.  For tile j, station s and source
-// slot kk (k = j*KT + kk): P LDS byte offsets, then the weight word.  The
+// Per-tile schedule of the LDS kernel.  For tile j and station s, one row
+// of KW*(P+1) words per group of KW source slots (one wave's sources,
+// kk = g*KW + q, k = j*KT + kk): the KW*P LDS byte offsets in (source,
+// phase) order, then the KW weight words.  The
 // moveout d of phase p lies e = d - lo samples into the window (lo: the
 // staging start, tile_lo[j,s,p] rounded down to even for 16-bit samples),
 // so the SPA samples starting at (lane sample + e) sit 16-byte aligned in
@@ -187,7 +189,7 @@ __global__ void schedule_kernel(const int *__restrict__ tau, const float *__rest
                                 const int *__restrict__ tile_lo,
                                 const int *__restrict__ tile_lw,
                                 const float *__restrict__ scale, int K, int S, int P, int KT,
-                                int fmt, long long n, int *__restrict__ sched) {
+                                int KW, int fmt, long long n, int *__restrict__ sched) {
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= n) return;
   const int kk = (int)(idx % KT);
@@ -198,7 +200,8 @@ __global__ void schedule_kernel(const int *__restrict__ tau, const float *__rest
   const int Lw = tile_lw[tile];
   const int spa = fmt == BP_F32 ? 4 : 8;
   const float b = k < K ? beta[(size_t)k * S + s] : 0.0f;
-  int *out = sched + idx * (P + 1);
+  const int q = kk % KW;
+  int *row = sched + (idx - q) * (P + 1);  // the row of slots idx - q .. idx - q + KW - 1
   for (int p = 0; p < P; ++p) {
     int off = 0;
     if (Lw > 0) {
@@ -211,13 +214,13 @@ __global__ void schedule_kernel(const int *__restrict__ tau, const float *__rest
         off = p * SB + 16 * ((e % spa) * (Lw / spa) + e / spa);
       }
     }
-    out[p] = off;
+    row[q * P + p] = off;
   }
   // An unused term gets weight -0.0: fma(-0, +0, acc) is acc bit for bit,
   // a zero acc keeping its sign, as when the reference skips the station
   // (beamform.c:61-62); with weight +0.0 an acc of -0.0 would become +0.0.
   const float w = b != 0.0f ? b * (scale ? scale[s] : 1.0f) : -0.0f;
-  out[P] = fmt == BP_BF16 ? (int)(bf16_bits(w) << 16) : __float_as_int(w);
+  row[KW * P + q] = fmt == BP_BF16 ? (int)(bf16_bits(w) << 16) : __float_as_int(w);
 }
 
 // ---------------------------------------------------------------------------
@@ -894,18 +897,39 @@ struct StackLds {
   }
 
   // Operands of one station for this wave's KW sources: KW*(P+1)
-  // wave-uniform words (P byte offsets and the weight word per source).
+  // wave-uniform words (the KW*P byte offsets in step order, then the KW
+  // weight words).
   __device__ __forceinline__ static void load_sched(const_int *sg, int (&sv)[KW * (P + 1)]) {
 #pragma unroll
     for (int i = 0; i < KW * (P + 1); ++i) sv[i] = sg[i];
   }
 
-  // One gather step is one (source, phase) term: R ds_read_b128.
-  __device__ __forceinline__ static void gather(const unsigned char *buf, unsigned lb, int off,
-                                                uint4 (&v)[R]) {
+  // One gather step is one (source, phase) term: R ds_read_b128 at LDS
+  // address `addr` (an absolute LDS address, so that no base is added).
+  typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+  typedef const __attribute__((address_space(3))) u32x4 lds_u32x4;
+  __device__ __forceinline__ static void gather(unsigned addr, uint4 (&v)[R]) {
 #pragma unroll
-    for (int r = 0; r < R; ++r)
-      v[r] = *reinterpret_cast<const uint4 *>(buf + (lb + (unsigned)off) + 1024 * r);
+    for (int r = 0; r < R; ++r) {
+      const u32x4 w = *(lds_u32x4 *)(size_t)(addr + 1024u * r);
+      v[r] = make_uint4(w.x, w.y, w.z, w.w);
+    }
+  }
+  __device__ __forceinline__ static unsigned lds_address(const unsigned char *p) {
+    return (unsigned)(size_t)(const __attribute__((address_space(3))) unsigned char *)p;
+  }
+
+  // Lane addresses of two steps in one instruction: a 64-bit add of the
+  // lane's offset, held in both halves of a register pair, and the two
+  // steps' (buffer + schedule) offsets as a scalar pair.  Neither half
+  // carries (LDS offsets are below 2^18), so the halves are the two 32-bit
+  // sums; one v_lshl_add_u64 replaces two v_add_u32.
+  __device__ __forceinline__ static unsigned long long pair_addr(unsigned long long lb2,
+                                                                 unsigned o0, unsigned o1) {
+    const unsigned long long o = ((unsigned long long)o1 << 32) | o0;
+    unsigned long long a2;
+    asm("v_lshl_add_u64 %0, %1, 0, %2" : "=v"(a2) : "v"(lb2), "s"(o));
+    return a2;
   }
 
   // One station: straight-line gathers and FMAs over the KW*P (source,
@@ -915,44 +939,62 @@ struct StackLds {
   // station at once, which would exhaust the registers: the empty asm
   // statements take the step's accumulators (its FMAs complete there) and
   // the lane's LDS offset (later gathers depend on it, so they stay behind).
+  // boff: byte offset of the station's staging buffer in the staging area.
   static constexpr int STEPS = KW * P;
   static constexpr int DEPTH = BP_PIPE_DEPTH < STEPS ? BP_PIPE_DEPTH : STEPS - 1;
 
   __device__ __forceinline__ static void consume_station(const int (&sv)[KW * (P + 1)],
-                                                         const unsigned char *buf, int lane,
+                                                         const unsigned char *stage,
+                                                         unsigned boff, int lane,
                                                          float (&acc)[KW][NACC]) {
-    unsigned lb = 16u * (unsigned)lane;
+    const unsigned lb = 16u * (unsigned)lane;
+    unsigned long long lb2 = ((unsigned long long)lb << 32) | lb;
+    const unsigned base = lds_address(stage) + boff;
     uint4 v[DEPTH + 1][R];
+    unsigned addr[STEPS];
+    // address of step n: computed with step n + 1 for even n
+    auto step_addr = [&](int n) {
+      if (n % 2 == 0 && n + 1 < STEPS) {
+        const unsigned long long a2 =
+            pair_addr(lb2, (unsigned)sv[n] + base, (unsigned)sv[n + 1] + base);
+        addr[n] = (unsigned)a2;
+        addr[n + 1] = (unsigned)(a2 >> 32);
+      } else if (n % 2 == 0) {
+        addr[n] = lb + (unsigned)sv[n] + base;
+      }
+      return addr[n];
+    };
 #pragma unroll
-    for (int i = 0; i < DEPTH; ++i)
-      gather(buf, lb, sv[(i / P) * (P + 1) + (i % P)], v[i % (DEPTH + 1)]);
+    for (int i = 0; i < DEPTH; ++i) gather(step_addr(i), v[i % (DEPTH + 1)]);
 #pragma unroll
     for (int i = 0; i < STEPS; ++i) {
       const int kk = i / P;
       const int n = i + DEPTH;
-      if (n < STEPS) gather(buf, lb, sv[(n / P) * (P + 1) + (n % P)], v[n % (DEPTH + 1)]);
-      const int bw = sv[kk * (P + 1) + P];
+      if (n < STEPS) gather(step_addr(n), v[n % (DEPTH + 1)]);
+      const int bw = sv[STEPS + kk];
 #pragma unroll
       for (int r = 0; r < R; ++r) F::mac(&acc[kk][SPA * r], v[i % (DEPTH + 1)][r], bw);
 #pragma unroll
       for (int j = 0; j < NACC; ++j) asm volatile("" : "+v"(acc[kk][j]));
-      asm volatile("" : "+v"(lb));
+      asm volatile("" : "+v"(lb2));
       __builtin_amdgcn_sched_barrier(0);
     }
   }
 
   // One pipeline step: start a later station's global->LDS copies into
-  // `nxt`, then consume the current station from `cur`.  Both buffers are
-  // restrict-qualified in one scope, so the compiler knows the copies cannot
-  // alias the gathers and does not wait for them before reading `cur`.
+  // `nxt`, then consume the current station from buffer `boff` of the
+  // staging area.  The staging area is restrict-qualified against the copy
+  // target in one scope, so the compiler knows the copies cannot alias the
+  // gathers and does not wait for them before reading.
   __device__ __forceinline__ static void station_step(bool dma, const StationSrc &src,
                                                       const RowPlan &rp, int SB, int wave,
                                                       const int (&sv)[KW * (P + 1)],
-                                                      const unsigned char *__restrict__ cur,
+                                                      const unsigned char *__restrict__ stage,
+                                                      unsigned boff,
                                                       unsigned char *__restrict__ nxt,
                                                       int lane, float (&acc)[KW][NACC]) {
     if (dma) dma_station(src, rp, SB, wave, nxt);
-    consume_station(sv, cur, lane, acc);
+    consume_station(sv, stage, boff, lane, acc);
   }
 
   // Sample of accumulator j of a lane: run j / SPA, position j % SPA.
@@ -1007,7 +1049,7 @@ struct StackLds {
         load_sched(sched_w + (size_t)s * KT * (P + 1), sv);
         stage_station_guarded(a, tlo, s, t0, tid, stage);
         __syncthreads();
-        consume_station(sv, stage, lane, acc);
+        consume_station(sv, stage, 0u, lane, acc);
         __syncthreads();
       }
     } else {
@@ -1035,8 +1077,8 @@ struct StackLds {
       for (int s = 0; s < S; ++s) {
         const int sd = s + NB - 1;
         const int di = ci == 0 ? NB - 1 : ci - 1;  // ring index of station sd
-        station_step(sd < S, src, rp, SB, wave, sv, stage + ci * bbytes, stage + di * bbytes,
-                     lane, acc);
+        station_step(sd < S, src, rp, SB, wave, sv, stage, (unsigned)(ci * bbytes),
+                     stage + di * bbytes, lane, acc);
         if (s + 1 < S) {
           // next station's operands and staging sources: the scalar loads
           // overlap the barrier
