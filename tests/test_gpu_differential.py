@@ -239,11 +239,16 @@ def test_uniform_weights_as_used_bits(gpu_lib, kind, P):
         assert bits_equal(bv.cpu().numpy(), col), f
 
 
-def test_unused_entries_contribute_nothing(gpu_lib):
+@pytest.mark.parametrize("layout", ["scattered", "smooth"])
+def test_unused_entries_contribute_nothing(gpu_lib, layout):
     """Unused stations whose features are NaN or infinite and whose delays
     point anywhere in the int32 range add nothing, signed zeros included:
     beams whose every used product underflows end at -0.0, and sources
-    with one used delay out of range keep the caller's value."""
+    with one used delay out of range keep the caller's value.  "smooth":
+    delays that change by at most one sample per source, so that a wave's
+    64 sources mostly fall in one 64-sample window per station and phase
+    (gathered by one load and a cross-lane permute), with a few stations
+    spread wider (gathered lane by lane), mixed within the chunks."""
     rng = np.random.default_rng(77)
     S, P, T, K = 13, 2, 301, 1100
     F = np.full((S, T, P), 1e-30, np.float32)
@@ -251,7 +256,12 @@ def test_unused_entries_contribute_nothing(gpu_lib):
     unused_rows = rng.random(S) < 0.4
     F[unused_rows] = np.nan
     F[unused_rows, ::3] = np.inf
-    tau = rng.integers(-T // 2 + 1, T // 2, size=(K, S, P)).astype(np.int32)
+    if layout == "scattered":
+        tau = rng.integers(-T // 2 + 1, T // 2, size=(K, S, P)).astype(np.int32)
+    else:
+        slope = rng.choice([0, 1, 1, 3], size=(S, P))  # 3: wider than a window
+        start = rng.integers(-60, 40, size=(S, P))
+        tau = (start[None] + slope[None] * (np.arange(K) % 64)[:, None, None]).astype(np.int32)
     beta = np.where(rng.random((K, S)) < 0.7, 1.0, 0.0).astype(np.float32)
     beta[:, unused_rows] = np.where(rng.random((K, unused_rows.sum())) < 0.5, 0.0, -0.0)
     tau[:, unused_rows] = rng.integers(-2**31, 2**31 - 1, size=(K, unused_rows.sum(), P),
