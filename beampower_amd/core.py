@@ -51,7 +51,7 @@ LIBRARIES = {
 # Symbols the gpu library declares in include/beamform_gpu.h.
 GPU_SYMBOLS = (
     "beamform", "beamform_max", "prestack_waveform_features", "beamform_differential",
-    "bp_last_error", "bp_version",
+    "bp_dropin_cache_clear", "bp_last_error", "bp_version",
     "bp_network_create", "bp_network_destroy", "bp_network_info", "bp_network_classes",
     "bp_network_create_ordered", "bp_network_source_order", "bp_kernel_shape",
     "bp_network_beamform_max", "bp_network_beamform",
@@ -133,6 +133,7 @@ def _declare_gpu_extension(lib):
     lib.bp_version.argtypes = []
     lib.bp_network_create.argtypes = [vp, vp, _SZ, _SZ, _SZ, ct.POINTER(vp)]
     lib.bp_network_destroy.argtypes = [vp]
+    lib.bp_dropin_cache_clear.argtypes = []
     lib.bp_network_create_ordered.argtypes = [vp, vp, _SZ, _SZ, _SZ, ct.c_int,
                                               ct.POINTER(vp)]
     lib.bp_network_source_order.argtypes = [vp, _P_I, vp]

@@ -11,6 +11,7 @@
 #include <tuple>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -19,8 +20,10 @@
 #include <memory>
 #include <mutex>
 #include <new>
+#include <cstdlib>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -863,6 +866,160 @@ void degenerate_max(size_t T, size_t K, int oob, float *bmax, int *barg) {
   }
 }
 
+// ---- plan reuse of the host-pointer entry points ---------------------------
+//
+// A caller of the reference's interface passes the same moveouts and
+// weights on every call of a detection loop (one call per window of data).
+// The network built for them (device tables, plan, schedules) is kept with a
+// host copy of those inputs, and a later call on the same device whose
+// moveouts and weights are equal byte for byte (compared in full, on several
+// host threads) reuses it instead of uploading and planning them again; on
+// its first reuse the network is re-planned once with the similarity source
+// order where that is cheaper for the kernels.  Results do not depend on
+// any of this: the plan is a function of the inputs, and the kernels'
+// results do not depend on the plan's source order.  One network is kept
+// (the last one built); bp_dropin_cache_clear() releases it, and
+// BP_DROPIN_CACHE=0 in the environment turns the reuse off.
+
+// Runs f(begin, end) over [0, n) bytes split across up to 16 host threads
+// (pieces of at least 8 MiB; small inputs stay on the calling thread).
+template <class F>
+void parallel_bytes(size_t n, F f) {
+  const size_t piece = (size_t)8 << 20;
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const size_t nt = std::min<size_t>({(size_t)16, (size_t)hw, (n + piece - 1) / piece});
+  if (nt <= 1) {
+    f((size_t)0, n);
+    return;
+  }
+  const size_t step = (n + nt - 1) / nt;
+  std::vector<std::thread> th;
+  th.reserve(nt - 1);
+  for (size_t i = 1; i < nt; ++i) {
+    const size_t b = std::min(n, i * step), e = std::min(n, b + step);
+    th.emplace_back([=] { f(b, e); });
+  }
+  f((size_t)0, std::min(n, step));
+  for (auto &x : th) x.join();
+}
+
+bool same_bytes(const void *a, const void *b, size_t n) {
+  std::atomic<bool> same{true};
+  parallel_bytes(n, [&](size_t lo, size_t hi) {
+    if (std::memcmp(static_cast<const char *>(a) + lo, static_cast<const char *>(b) + lo,
+                    hi - lo) != 0)
+      same = false;
+  });
+  return same;
+}
+
+void copy_bytes(void *dst, const void *src, size_t n) {
+  parallel_bytes(n, [&](size_t lo, size_t hi) {
+    std::memcpy(static_cast<char *>(dst) + lo, static_cast<const char *>(src) + lo, hi - lo);
+  });
+}
+
+struct DropinCache {
+  std::mutex mu;
+  int device = -1;
+  size_t K = 0, S = 0, P = 0;
+  std::unique_ptr<int[]> tau;  // host copies (not zero-filled first: copied in parallel)
+  std::unique_ptr<float[]> beta;
+  std::shared_ptr<bp_network> net;
+  long long reuses = 0;
+};
+
+// Never destroyed: its device memory must not be freed by a static
+// destructor after the runtime has shut down.
+DropinCache &dropin_cache() {
+  static DropinCache *c = new DropinCache;
+  return *c;
+}
+
+bool dropin_cache_on() {
+  const char *e = std::getenv("BP_DROPIN_CACHE");
+  return !(e && e[0] == '0');
+}
+
+// The network of one host-pointer call.  keep: built by this call, to be
+// kept; replan: reused for the first time, to be re-planned with the
+// similarity order.  Both are done by dropin_finish while the call's
+// kernels run.
+struct DropinUse {
+  std::shared_ptr<bp_network> net;
+  bool keep = false, replan = false;
+  int device = 0;
+};
+
+// The kept network when the inputs match, else a new one (planned for this
+// call's n_samples).
+int dropin_network(const int *tau, const float *beta, size_t K, size_t S, size_t P, size_t T,
+                   hipStream_t st, DropinUse &u) {
+  const bool on = dropin_cache_on();
+  HIP_TRY(hipGetDevice(&u.device));
+  DropinCache &c = dropin_cache();
+  if (on) {
+    std::lock_guard<std::mutex> lock(c.mu);
+    if (c.net && c.device == u.device && c.K == K && c.S == S && c.P == P &&
+        same_bytes(c.tau.get(), tau, K * S * P * sizeof(int)) &&
+        same_bytes(c.beta.get(), beta, K * S * sizeof(float))) {
+      u.net = c.net;
+      u.replan = c.reuses++ == 0 && !c.net->permuted && K > (size_t)kTileSources;
+      return BP_OK;
+    }
+  }
+  std::unique_ptr<bp_network> net;
+  BP_TRY(network_create(tau, beta, K, S, P, kOrderAutoWide, st, net, T));
+  u.net = std::shared_ptr<bp_network>(std::move(net));
+  u.keep = on;
+  return BP_OK;
+}
+
+// After the call's kernels are launched on `st`: keep the new network with
+// host copies of its inputs (published once `st` has finished, so that
+// other threads find its tables complete), or re-plan the reused one on a
+// stream of its own.  The call's inputs are still the caller's arrays.
+// Keeping and re-planning are best effort: when memory for them is short
+// the call still succeeds, and only an error of the call's own stream is
+// returned.
+int dropin_finish(const int *tau, const float *beta, size_t K, size_t S, size_t P,
+                  hipStream_t st, const DropinUse &u) {
+  DropinCache &c = dropin_cache();
+  if (u.keep) {
+    std::unique_ptr<int[]> tau_h(new (std::nothrow) int[K * S * P]);
+    std::unique_ptr<float[]> beta_h(new (std::nothrow) float[K * S]);
+    if (tau_h && beta_h) {
+      copy_bytes(tau_h.get(), tau, K * S * P * sizeof(int));
+      copy_bytes(beta_h.get(), beta, K * S * sizeof(float));
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    if (!tau_h || !beta_h) return BP_OK;
+    std::lock_guard<std::mutex> lock(c.mu);
+    c.tau = std::move(tau_h);
+    c.beta = std::move(beta_h);
+    c.device = u.device;
+    c.K = K;
+    c.S = S;
+    c.P = P;
+    c.reuses = 0;
+    c.net = u.net;
+  } else if (u.replan) {
+    try {
+      Stream rs;
+      std::unique_ptr<bp_network> net;
+      if (rs.create() != BP_OK ||
+          network_create(tau, beta, K, S, P, BP_ORDER_AUTO, rs.s, net) != BP_OK ||
+          hipStreamSynchronize(rs.s) != hipSuccess)
+        return BP_OK;  // the reused network stays
+      std::lock_guard<std::mutex> lock(c.mu);
+      if (c.net == u.net) c.net = std::shared_ptr<bp_network>(std::move(net));
+    } catch (...) {
+      return BP_OK;
+    }
+  }
+  return BP_OK;
+}
+
 // Host (S, T, P) features of the drop-in entry points -> device planar rows
 // padded for the network (no LDS tile takes the guarded edge path).
 int stage_dropin_features(const bp_network *net, const float *host_stp, size_t S, size_t T,
@@ -1376,6 +1533,18 @@ int bp_network_create_ordered(const int *time_delays, const float *weights_sourc
 
 int bp_network_destroy(bp_network *net) try {
   delete net;
+  return BP_OK;
+} BP_CATCH
+
+int bp_dropin_cache_clear(void) try {
+  DropinCache &c = dropin_cache();
+  std::lock_guard<std::mutex> lock(c.mu);
+  c.net.reset();
+  c.tau.reset();
+  c.beta.reset();
+  c.device = -1;
+  c.K = c.S = c.P = 0;
+  c.reuses = 0;
   return BP_OK;
 } BP_CATCH
 
@@ -2040,22 +2209,24 @@ int beamform_max(float *waveform_features, int *time_delays, float *weights_sour
   BP_TRY(check_device());
   Stream st;
   BP_TRY(st.create());
-  std::unique_ptr<bp_network> net;
-  BP_TRY(network_create(time_delays, weights_sources, n_sources, n_stations, n_phases,
-                        kOrderAutoWide, st.s, net, n_samples));
+  DropinUse use;
+  BP_TRY(dropin_network(time_delays, weights_sources, n_sources, n_stations, n_phases, n_samples,
+                        st.s, use));
+  bp_network *net = use.net.get();
   const size_t T = n_samples, nf = n_stations * n_samples * n_phases;
   bp_feature_view v{};
   DevBuf f_stp, f_pad, keys, bmax, barg;
-  BP_TRY(stage_dropin_features(net.get(), waveform_features, n_stations, T, n_phases, f_stp,
-                               f_pad, v, st.s));
+  BP_TRY(stage_dropin_features(net, waveform_features, n_stations, T, n_phases, f_stp, f_pad,
+                               v, st.s));
   BP_TRY(keys.alloc(T * sizeof(unsigned long long)));
   BP_TRY(bmax.alloc(T * sizeof(float)));
   BP_TRY(barg.alloc(T * sizeof(int)));
   // the caller's beam_max is kept where no source is valid (beamform.c:332-333)
   HIP_TRY(hipMemcpyAsync(bmax.p, beam_max, T * sizeof(float), hipMemcpyHostToDevice, st.s));
   (void)nf;
-  BP_TRY(network_max(net.get(), v, out_of_bounds, bmax.as<float>(), barg.as<int>(),
+  BP_TRY(network_max(net, v, out_of_bounds, bmax.as<float>(), barg.as<int>(),
                      keys.as<unsigned long long>(), st.s));
+  BP_TRY(dropin_finish(time_delays, weights_sources, n_sources, n_stations, n_phases, st.s, use));
   HIP_TRY(hipMemcpyAsync(beam_max, bmax.p, T * sizeof(float), hipMemcpyDeviceToHost, st.s));
   HIP_TRY(hipMemcpyAsync(beam_argmax, barg.p, T * sizeof(int), hipMemcpyDeviceToHost, st.s));
   HIP_TRY(hipStreamSynchronize(st.s));
@@ -2082,13 +2253,14 @@ int beamform(float *waveform_features, int *time_delays, float *weights_sources,
   BP_TRY(check_device());
   Stream st;
   BP_TRY(st.create());
-  std::unique_ptr<bp_network> net;
-  BP_TRY(network_create(time_delays, weights_sources, n_sources, n_stations, n_phases,
-                        kOrderAutoWide, st.s, net, n_samples));
+  DropinUse use;
+  BP_TRY(dropin_network(time_delays, weights_sources, n_sources, n_stations, n_phases, n_samples,
+                        st.s, use));
+  bp_network *net = use.net.get();
   bp_feature_view v{};
   DevBuf f_stp, f_pad, dbeam;
-  BP_TRY(stage_dropin_features(net.get(), waveform_features, n_stations, T, n_phases, f_stp,
-                               f_pad, v, st.s));
+  BP_TRY(stage_dropin_features(net, waveform_features, n_stations, T, n_phases, f_stp, f_pad, v,
+                               st.s));
   BP_TRY(dbeam.alloc(K * T * sizeof(float)));
   // invalid strict cells read back as 0, as the wrapper's zero-filled buffer
   const long long n = (long long)(K * T);
@@ -2097,10 +2269,11 @@ int beamform(float *waveform_features, int *time_delays, float *weights_sources,
                      dbeam.as<float>(), n, 0.0f);
   HIP_TRY(hipGetLastError());
   std::shared_ptr<Schedule> sc;
-  BP_TRY(ensure_schedule(net.get(), BP_F32, nullptr, st.s, sc));
-  bp::StackArgs a = make_args_view(net.get(), v, out_of_bounds, *sc);
+  BP_TRY(ensure_schedule(net, BP_F32, nullptr, st.s, sc));
+  bp::StackArgs a = make_args_view(net, v, out_of_bounds, *sc);
   a.beam = dbeam.as<float>();
-  BP_TRY((launch_stack<BP_F32, true>(net.get(), a, 0, (int)K, st.s)));
+  BP_TRY((launch_stack<BP_F32, true>(net, a, 0, (int)K, st.s)));
+  BP_TRY(dropin_finish(time_delays, weights_sources, n_sources, n_stations, n_phases, st.s, use));
   HIP_TRY(hipMemcpyAsync(beam, dbeam.p, K * T * sizeof(float), hipMemcpyDeviceToHost, st.s));
   HIP_TRY(hipStreamSynchronize(st.s));
   return BP_OK;

@@ -681,19 +681,37 @@ def main():
     m_h = np.zeros(T, np.float32)
     a_h = np.zeros(T, np.int32)
     P_F, P_I = ct.POINTER(ct.c_float), ct.POINTER(ct.c_int)
-    host_ms = []
-    for _ in range(3):
+    lib.bp_dropin_cache_clear()
+
+    def host_call():
         t1 = time.perf_counter()
         rc = lib.beamform_max(F_host.ctypes.data_as(P_F), tau_h.ctypes.data_as(P_I),
                               beta_h.ctypes.data_as(P_F), T, K, S, P, oob,
                               m_h.ctypes.data_as(P_F), a_h.ctypes.data_as(P_I))
-        host_ms.append((time.perf_counter() - t1) * 1e3)
         assert rc == 0
-    dropin_host = {"ms": min(host_ms), "value": K * T / (min(host_ms) * 1e-3),
+        return (time.perf_counter() - t1) * 1e3
+
+    # with the plan rebuilt on every call (reuse off), then the same arrays
+    # passed again (the first call builds and keeps the network, the second
+    # re-plans it once, the others reuse it)
+    os.environ["BP_DROPIN_CACHE"] = "0"
+    cold_ms = min(host_call() for _ in range(3))
+    del os.environ["BP_DROPIN_CACHE"]
+    first_ms = [host_call() for _ in range(2)]
+    warm_ms = min(host_call() for _ in range(3))
+    lib.bp_dropin_cache_clear()
+    dropin_host = {"ms": cold_ms, "value": K * T / (cold_ms * 1e-3),
                    "unit": "source·samples/s",
                    "identical_to_resident": bool(np.array_equal(a_h, barg.cpu().numpy())),
-                   "note": "beamform_max with host pointers (PCIe-inclusive, plan built per call)"}
-    log(f"host-pointer drop-in: {min(host_ms):.1f} ms")
+                   "note": "beamform_max with host pointers: uploads and plan on every call",
+                   "repeated_inputs": {
+                       "ms": warm_ms, "value": K * T / (warm_ms * 1e-3),
+                       "first_two_calls_ms": first_ms,
+                       "note": "same time_delays and weights_sources on every call (a detection "
+                               "loop over windows): the kept network is reused, features are "
+                               "uploaded per call"}}
+    log(f"host-pointer drop-in: {cold_ms:.1f} ms planned per call, {warm_ms:.1f} ms reused "
+        f"(first calls {first_ms[0]:.1f}, {first_ms[1]:.1f} ms)")
     del F_host
 
     # known answer on the benchmark output itself

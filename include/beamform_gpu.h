@@ -49,6 +49,17 @@ int beamform_max(float *waveform_features, int *time_delays, float *weights_sour
                  size_t n_phases, int out_of_bounds, float *beam_max,
                  int *beam_argmax);
 
+/* beamform and beamform_max keep the network they build for time_delays
+ * and weights_sources (device tables, plan, schedules) with a host copy of
+ * those two arrays; a later call on the same device whose arrays are equal
+ * byte for byte (compared in full) reuses it instead of uploading and
+ * planning again, and re-plans it once with the similarity source order on
+ * its first reuse.  Results are the same either way.  One network is kept
+ * (the last one built); BP_DROPIN_CACHE=0 in the environment turns the reuse
+ * off.  bp_dropin_cache_clear releases the kept network (calls in progress
+ * keep theirs until they return).  Extension, no reference counterpart. */
+int bp_dropin_cache_clear(void);
+
 /* Replaces the CPU library's prestack_waveform_features (beamform_cpu.h:4,
  * beamform.c:108-152; argtypes core.py:40-49), computed on the GPU.
  * waveform_features: (n_stations, n_channels, n_samples) float32

@@ -1,7 +1,8 @@
 """Re-entrancy of the drop-in C ABI: ctypes releases the GIL during a call,
 so Python threads can be inside the library together.  Every call uses its
-own stream, buffers and network plan; concurrent calls must give the same
-bits as the same calls made one at a time."""
+own stream and buffers, and calls share the kept network only when their
+moveouts and weights are equal; concurrent calls must give the same bits as
+the same calls made one at a time."""
 import ctypes as ct
 from concurrent.futures import ThreadPoolExecutor
 
