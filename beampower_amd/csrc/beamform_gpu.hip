@@ -1020,16 +1020,100 @@ int dropin_finish(const int *tau, const float *beta, size_t K, size_t S, size_t 
   return BP_OK;
 }
 
-// Host (S, T, P) features of the drop-in entry points -> device planar rows
-// padded for the network (no LDS tile takes the guarded edge path).
-int stage_dropin_features(const bp_network *net, const float *host_stp, size_t S, size_t T,
-                          size_t P, DevBuf &f_stp, DevBuf &f_pad, bp_feature_view &v,
-                          hipStream_t st) {
-  const size_t nf = S * T * P;
-  const size_t ld = (size_t)net->pad_lo + T + (size_t)net->pad_hi;
+// Upload of a host-pointer call's features.  Pageable memory goes through
+// the runtime's bounce buffers at about 20 GB/s; here it is copied by
+// several host threads into one of two pinned 8 MiB buffers while the
+// other one's DMA runs (about 50 GB/s over the link).  The pinned pair is
+// allocated once per process; a call that finds it in use by another
+// thread, or on another device, uploads from pageable memory.  The call
+// returns when its last DMA has completed (the buffers are then free).
+struct PinnedStage {
+  std::mutex mu;
+  bool tried = false;  // one attempt to allocate (a failed one is not repeated)
+  int device = -1;
+  void *buf[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+};
+constexpr size_t kPinnedBytes = (size_t)8 << 20;
+
+PinnedStage &pinned_stage() {
+  static PinnedStage *p = new PinnedStage;  // never freed (see dropin_cache)
+  return *p;
+}
+
+int upload_host(void *dst, const void *src, size_t n, hipStream_t st) {
+  PinnedStage &ps = pinned_stage();
+  std::unique_lock<std::mutex> lock(ps.mu, std::try_to_lock);
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (lock.owns_lock() && !ps.tried) {
+    ps.tried = true;
+    if (hipHostMalloc(&ps.buf[0], kPinnedBytes, hipHostMallocDefault) == hipSuccess &&
+        hipHostMalloc(&ps.buf[1], kPinnedBytes, hipHostMallocDefault) == hipSuccess &&
+        hipEventCreateWithFlags(&ps.ev[0], hipEventDisableTiming) == hipSuccess &&
+        hipEventCreateWithFlags(&ps.ev[1], hipEventDisableTiming) == hipSuccess) {
+      ps.device = dev;
+    }
+  }
+  if (!lock.owns_lock() || ps.device != dev || n < 2 * kPinnedBytes) {
+    HIP_TRY(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, st));
+    return BP_OK;
+  }
+  size_t done = 0;
+  for (int i = 0; done < n; ++i) {
+    const int j = i & 1;
+    const size_t len = std::min(kPinnedBytes, n - done);
+    if (i >= 2) HIP_TRY(hipEventSynchronize(ps.ev[j]));  // buffer j's previous DMA is done
+    copy_bytes(ps.buf[j], static_cast<const char *>(src) + done, len);
+    HIP_TRY(hipMemcpyAsync(static_cast<char *>(dst) + done, ps.buf[j], len,
+                           hipMemcpyHostToDevice, st));
+    HIP_TRY(hipEventRecord(ps.ev[j], st));
+    done += len;
+  }
+  HIP_TRY(hipEventSynchronize(ps.ev[0]));
+  HIP_TRY(hipEventSynchronize(ps.ev[1]));
+  return BP_OK;
+}
+
+// The network and the raw (S, T, P) features of a host-pointer call: the
+// features' upload runs on a second host thread while the moveouts and
+// weights are compared with the kept ones (or their network is built).
+int dropin_inputs(const float *host_stp, size_t nf, const int *tau, const float *beta,
+                  size_t K, size_t S, size_t P, size_t T, hipStream_t st, DevBuf &f_stp,
+                  DropinUse &use) {
   BP_TRY(f_stp.alloc(nf * sizeof(float)));
+  int up_rc = BP_OK, dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  std::string up_err;  // the upload thread's message, reported by this thread
+  std::thread up([&] {
+    try {
+      up_rc = hipSetDevice(dev) == hipSuccess  // a new thread starts on device 0
+                  ? upload_host(f_stp.p, host_stp, nf * sizeof(float), st)
+                  : fail(BP_EHIP, "hipSetDevice(%d) failed", dev);
+    } catch (...) {
+      up_rc = fail(BP_EHIP, "feature upload failed");
+    }
+    if (up_rc != BP_OK) up_err = g_err;
+  });
+  int rc = BP_OK;
+  try {
+    rc = dropin_network(tau, beta, K, S, P, T, st, use);
+  } catch (...) {
+    up.join();
+    throw;
+  }
+  up.join();
+  BP_TRY(rc);
+  if (up_rc != BP_OK) return fail(up_rc, "%s", up_err.c_str());
+  return BP_OK;
+}
+
+// Raw (S, T, P) features on the device -> planar rows padded for the
+// network (no LDS tile takes the guarded edge path).
+int stage_dropin_features(const bp_network *net, const DevBuf &f_stp, size_t S, size_t T,
+                          size_t P, DevBuf &f_pad, bp_feature_view &v, hipStream_t st) {
+  const size_t ld = (size_t)net->pad_lo + T + (size_t)net->pad_hi;
   BP_TRY(f_pad.alloc(S * P * ld * sizeof(float)));
-  HIP_TRY(hipMemcpyAsync(f_stp.p, host_stp, nf * sizeof(float), hipMemcpyHostToDevice, st));
   hipLaunchKernelGGL(bp::interleaved_to_planar_kernel,
                      dim3((unsigned)((ld + 255) / 256), (unsigned)S), dim3(256), 0, st,
                      f_stp.as<float>(), (int)S, (int)T, (int)P, (long long)ld,
@@ -2210,20 +2294,18 @@ int beamform_max(float *waveform_features, int *time_delays, float *weights_sour
   Stream st;
   BP_TRY(st.create());
   DropinUse use;
-  BP_TRY(dropin_network(time_delays, weights_sources, n_sources, n_stations, n_phases, n_samples,
-                        st.s, use));
-  bp_network *net = use.net.get();
   const size_t T = n_samples, nf = n_stations * n_samples * n_phases;
   bp_feature_view v{};
   DevBuf f_stp, f_pad, keys, bmax, barg;
-  BP_TRY(stage_dropin_features(net, waveform_features, n_stations, T, n_phases, f_stp, f_pad,
-                               v, st.s));
+  BP_TRY(dropin_inputs(waveform_features, nf, time_delays, weights_sources, n_sources,
+                       n_stations, n_phases, n_samples, st.s, f_stp, use));
+  bp_network *net = use.net.get();
+  BP_TRY(stage_dropin_features(net, f_stp, n_stations, T, n_phases, f_pad, v, st.s));
   BP_TRY(keys.alloc(T * sizeof(unsigned long long)));
   BP_TRY(bmax.alloc(T * sizeof(float)));
   BP_TRY(barg.alloc(T * sizeof(int)));
   // the caller's beam_max is kept where no source is valid (beamform.c:332-333)
   HIP_TRY(hipMemcpyAsync(bmax.p, beam_max, T * sizeof(float), hipMemcpyHostToDevice, st.s));
-  (void)nf;
   BP_TRY(network_max(net, v, out_of_bounds, bmax.as<float>(), barg.as<int>(),
                      keys.as<unsigned long long>(), st.s));
   BP_TRY(dropin_finish(time_delays, weights_sources, n_sources, n_stations, n_phases, st.s, use));
@@ -2254,13 +2336,13 @@ int beamform(float *waveform_features, int *time_delays, float *weights_sources,
   Stream st;
   BP_TRY(st.create());
   DropinUse use;
-  BP_TRY(dropin_network(time_delays, weights_sources, n_sources, n_stations, n_phases, n_samples,
-                        st.s, use));
-  bp_network *net = use.net.get();
   bp_feature_view v{};
   DevBuf f_stp, f_pad, dbeam;
-  BP_TRY(stage_dropin_features(net, waveform_features, n_stations, T, n_phases, f_stp, f_pad, v,
-                               st.s));
+  BP_TRY(dropin_inputs(waveform_features, n_stations * T * n_phases, time_delays,
+                       weights_sources, n_sources, n_stations, n_phases, n_samples, st.s, f_stp,
+                       use));
+  bp_network *net = use.net.get();
+  BP_TRY(stage_dropin_features(net, f_stp, n_stations, T, n_phases, f_pad, v, st.s));
   BP_TRY(dbeam.alloc(K * T * sizeof(float)));
   // invalid strict cells read back as 0, as the wrapper's zero-filled buffer
   const long long n = (long long)(K * T);
