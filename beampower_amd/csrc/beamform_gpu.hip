@@ -793,6 +793,118 @@ int lds_bytes_of(int fmt, int P, bool none, int Lw) {
   return none ? lds_bytes_of<BP_BF16, true>(P, Lw) : lds_bytes_of<BP_BF16, false>(P, Lw);
 }
 
+// Runs f(begin, end) over [0, n) bytes split across up to 16 host threads
+// (pieces of at least `piece` bytes; small inputs stay on the calling
+// thread).
+template <class F>
+void parallel_bytes(size_t n, F f, size_t piece = (size_t)8 << 20) {
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const size_t nt = std::min<size_t>({(size_t)16, (size_t)hw, (n + piece - 1) / piece});
+  if (nt <= 1) {
+    f((size_t)0, n);
+    return;
+  }
+  const size_t step = (n + nt - 1) / nt;
+  std::vector<std::thread> th;
+  th.reserve(nt - 1);
+  for (size_t i = 1; i < nt; ++i) {
+    const size_t b = std::min(n, i * step), e = std::min(n, b + step);
+    th.emplace_back([=] { f(b, e); });
+  }
+  f((size_t)0, std::min(n, step));
+  for (auto &x : th) x.join();
+}
+
+bool same_bytes(const void *a, const void *b, size_t n) {
+  std::atomic<bool> same{true};
+  parallel_bytes(n, [&](size_t lo, size_t hi) {
+    if (std::memcmp(static_cast<const char *>(a) + lo, static_cast<const char *>(b) + lo,
+                    hi - lo) != 0)
+      same = false;
+  });
+  return same;
+}
+
+void copy_bytes(void *dst, const void *src, size_t n, size_t piece = (size_t)8 << 20) {
+  parallel_bytes(
+      n,
+      [&](size_t lo, size_t hi) {
+        std::memcpy(static_cast<char *>(dst) + lo, static_cast<const char *>(src) + lo,
+                    hi - lo);
+      },
+      piece);
+}
+
+// Uploads from host memory the caller owns (pageable: the runtime's bounce
+// buffers reach about 20 GB/s).  Here the data is copied by several host
+// threads into one of two pinned 16 MiB buffers while the other one's DMA
+// runs (about 50 GB/s over the link).  Two such pairs are allocated once per
+// process, so two uploads can run at once (a call's features on one host
+// thread, its moveouts and weights on another); an upload that finds both
+// in use, or that is smaller than two buffers, copies from pageable memory.
+// upload_host returns when its last DMA has completed (the buffers are then
+// free for the next upload).
+struct PinnedStage {
+  std::mutex mu;
+  bool tried = false;  // one attempt to allocate (a failed one is not repeated)
+  int device = -1;
+  void *buf[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+};
+constexpr size_t kPinnedBytes = (size_t)16 << 20;
+constexpr size_t kPinnedPiece = (size_t)2 << 20;  // per host thread of a chunk's copy
+
+constexpr int kPinnedStages = 2;
+
+PinnedStage *pinned_stages() {
+  static PinnedStage *p = new PinnedStage[kPinnedStages];  // never freed: no device call at exit
+  return p;
+}
+
+int upload_host(void *dst, const void *src, size_t n, hipStream_t st) {
+  if (n < 2 * kPinnedBytes) {
+    HIP_TRY(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, st));
+    return BP_OK;
+  }
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  PinnedStage *stages = pinned_stages();
+  std::unique_lock<std::mutex> lock;
+  int k = 0;
+  for (; k < kPinnedStages; ++k) {
+    lock = std::unique_lock<std::mutex>(stages[k].mu, std::try_to_lock);
+    if (lock.owns_lock()) break;
+  }
+  PinnedStage &ps = stages[k < kPinnedStages ? k : 0];
+  if (lock.owns_lock() && !ps.tried) {
+    ps.tried = true;
+    if (hipHostMalloc(&ps.buf[0], kPinnedBytes, hipHostMallocDefault) == hipSuccess &&
+        hipHostMalloc(&ps.buf[1], kPinnedBytes, hipHostMallocDefault) == hipSuccess &&
+        hipEventCreateWithFlags(&ps.ev[0], hipEventDisableTiming) == hipSuccess &&
+        hipEventCreateWithFlags(&ps.ev[1], hipEventDisableTiming) == hipSuccess) {
+      ps.device = dev;
+    }
+  }
+  if (!lock.owns_lock() || ps.device != dev) {
+    HIP_TRY(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, st));
+    return BP_OK;
+  }
+  size_t done = 0;
+  for (int i = 0; done < n; ++i) {
+    const int j = i & 1;
+    const size_t len = std::min(kPinnedBytes, n - done);
+    if (i >= 2) HIP_TRY(hipEventSynchronize(ps.ev[j]));  // buffer j's previous DMA is done
+    copy_bytes(ps.buf[j], static_cast<const char *>(src) + done, len, kPinnedPiece);
+    HIP_TRY(hipMemcpyAsync(static_cast<char *>(dst) + done, ps.buf[j], len,
+                           hipMemcpyHostToDevice, st));
+    HIP_TRY(hipEventRecord(ps.ev[j], st));
+    done += len;
+  }
+  HIP_TRY(hipEventSynchronize(ps.ev[0]));
+  HIP_TRY(hipEventSynchronize(ps.ev[1]));
+  return BP_OK;
+}
+
 // n_samples: the samples of the one call a kOrderAutoWide plan is made for.
 int network_create(const int *tau, const float *beta, size_t K, size_t S, size_t P,
                    int order, hipStream_t st, std::unique_ptr<bp_network> &out,
@@ -809,8 +921,8 @@ int network_create(const int *tau, const float *beta, size_t K, size_t S, size_t
   auto net = make();
   BP_TRY(net->tau.alloc(ksp * sizeof(int)));
   BP_TRY(net->beta.alloc(ks * sizeof(float)));
-  HIP_TRY(hipMemcpyAsync(net->tau.p, tau, ksp * sizeof(int), hipMemcpyHostToDevice, st));
-  HIP_TRY(hipMemcpyAsync(net->beta.p, beta, ks * sizeof(float), hipMemcpyHostToDevice, st));
+  BP_TRY(upload_host(net->tau.p, tau, ksp * sizeof(int), st));
+  BP_TRY(upload_host(net->beta.p, beta, ks * sizeof(float), st));
   BP_TRY(build_plan(net.get(), st));
 
   bool try_order = order == BP_ORDER_SIMILARITY || (order == BP_ORDER_AUTO && K > kTileSources);
@@ -880,44 +992,6 @@ void degenerate_max(size_t T, size_t K, int oob, float *bmax, int *barg) {
 // results do not depend on the plan's source order.  One network is kept
 // (the last one built); bp_dropin_cache_clear() releases it, and
 // BP_DROPIN_CACHE=0 in the environment turns the reuse off.
-
-// Runs f(begin, end) over [0, n) bytes split across up to 16 host threads
-// (pieces of at least 8 MiB; small inputs stay on the calling thread).
-template <class F>
-void parallel_bytes(size_t n, F f) {
-  const size_t piece = (size_t)8 << 20;
-  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-  const size_t nt = std::min<size_t>({(size_t)16, (size_t)hw, (n + piece - 1) / piece});
-  if (nt <= 1) {
-    f((size_t)0, n);
-    return;
-  }
-  const size_t step = (n + nt - 1) / nt;
-  std::vector<std::thread> th;
-  th.reserve(nt - 1);
-  for (size_t i = 1; i < nt; ++i) {
-    const size_t b = std::min(n, i * step), e = std::min(n, b + step);
-    th.emplace_back([=] { f(b, e); });
-  }
-  f((size_t)0, std::min(n, step));
-  for (auto &x : th) x.join();
-}
-
-bool same_bytes(const void *a, const void *b, size_t n) {
-  std::atomic<bool> same{true};
-  parallel_bytes(n, [&](size_t lo, size_t hi) {
-    if (std::memcmp(static_cast<const char *>(a) + lo, static_cast<const char *>(b) + lo,
-                    hi - lo) != 0)
-      same = false;
-  });
-  return same;
-}
-
-void copy_bytes(void *dst, const void *src, size_t n) {
-  parallel_bytes(n, [&](size_t lo, size_t hi) {
-    std::memcpy(static_cast<char *>(dst) + lo, static_cast<const char *>(src) + lo, hi - lo);
-  });
-}
 
 struct DropinCache {
   std::mutex mu;
@@ -1017,61 +1091,6 @@ int dropin_finish(const int *tau, const float *beta, size_t K, size_t S, size_t 
       return BP_OK;
     }
   }
-  return BP_OK;
-}
-
-// Upload of a host-pointer call's features.  Pageable memory goes through
-// the runtime's bounce buffers at about 20 GB/s; here it is copied by
-// several host threads into one of two pinned 8 MiB buffers while the
-// other one's DMA runs (about 50 GB/s over the link).  The pinned pair is
-// allocated once per process; a call that finds it in use by another
-// thread, or on another device, uploads from pageable memory.  The call
-// returns when its last DMA has completed (the buffers are then free).
-struct PinnedStage {
-  std::mutex mu;
-  bool tried = false;  // one attempt to allocate (a failed one is not repeated)
-  int device = -1;
-  void *buf[2] = {nullptr, nullptr};
-  hipEvent_t ev[2] = {nullptr, nullptr};
-};
-constexpr size_t kPinnedBytes = (size_t)8 << 20;
-
-PinnedStage &pinned_stage() {
-  static PinnedStage *p = new PinnedStage;  // never freed (see dropin_cache)
-  return *p;
-}
-
-int upload_host(void *dst, const void *src, size_t n, hipStream_t st) {
-  PinnedStage &ps = pinned_stage();
-  std::unique_lock<std::mutex> lock(ps.mu, std::try_to_lock);
-  int dev = 0;
-  HIP_TRY(hipGetDevice(&dev));
-  if (lock.owns_lock() && !ps.tried) {
-    ps.tried = true;
-    if (hipHostMalloc(&ps.buf[0], kPinnedBytes, hipHostMallocDefault) == hipSuccess &&
-        hipHostMalloc(&ps.buf[1], kPinnedBytes, hipHostMallocDefault) == hipSuccess &&
-        hipEventCreateWithFlags(&ps.ev[0], hipEventDisableTiming) == hipSuccess &&
-        hipEventCreateWithFlags(&ps.ev[1], hipEventDisableTiming) == hipSuccess) {
-      ps.device = dev;
-    }
-  }
-  if (!lock.owns_lock() || ps.device != dev || n < 2 * kPinnedBytes) {
-    HIP_TRY(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, st));
-    return BP_OK;
-  }
-  size_t done = 0;
-  for (int i = 0; done < n; ++i) {
-    const int j = i & 1;
-    const size_t len = std::min(kPinnedBytes, n - done);
-    if (i >= 2) HIP_TRY(hipEventSynchronize(ps.ev[j]));  // buffer j's previous DMA is done
-    copy_bytes(ps.buf[j], static_cast<const char *>(src) + done, len);
-    HIP_TRY(hipMemcpyAsync(static_cast<char *>(dst) + done, ps.buf[j], len,
-                           hipMemcpyHostToDevice, st));
-    HIP_TRY(hipEventRecord(ps.ev[j], st));
-    done += len;
-  }
-  HIP_TRY(hipEventSynchronize(ps.ev[0]));
-  HIP_TRY(hipEventSynchronize(ps.ev[1]));
   return BP_OK;
 }
 
