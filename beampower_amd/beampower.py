@@ -15,6 +15,12 @@ behaviour -- with device="gpu" served by the gfx950 library
 * prestack_traces(device="gpu") returns the prestacked traces (the
   reference returns zeros there: its only prestack call is under
   `if device == "cpu"`, beampower.py:271-282);
+* mode="direct" on the GPU makes one library call, bp_beamform_max_fused
+  or bp_beamform_fused, instead of prestack_waveform_features followed by
+  beamform_max / beamform: the features go to the device unstacked and are
+  prestacked there, so the prestacked traces never cross PCIe (same bits);
+* inputs that already have the library's dtype and C order are passed
+  without the copies of `.flatten().astype(...)` (same values);
 * a failing library call raises RuntimeError with the library's message
   instead of terminating the process (the reference's CUDA path calls
   exit(), beamform.cu:17-26,259-266).
@@ -28,6 +34,22 @@ from .core import check, load_library
 
 _P_F = ct.POINTER(ct.c_float)
 _P_I = ct.POINTER(ct.c_int)
+
+
+def _flat(x, dtype):
+    """x in C order as a 1-D array of dtype: the values of
+    x.flatten().astype(dtype), copied only where x is not already so."""
+    return np.ascontiguousarray(x, dtype=dtype).reshape(-1)
+
+
+def _fusable(waveform_features, time_delays, weights_phases, weights_sources):
+    """Shapes of the fused GPU call: (S, C, T) features, (K, S, P) delays,
+    (S, C, P) phase weights and K * S source weights."""
+    S, C, _ = waveform_features.shape
+    K, S_d, P = time_delays.shape
+    return (S_d == S and np.ndim(weights_phases) == 3
+            and tuple(np.shape(weights_phases)) == (S, C, P)
+            and np.size(weights_sources) == K * S)
 
 
 def beamform(
@@ -93,12 +115,33 @@ def beamform(
         print(f"Mode should either be 'direct' or 'differential', not {mode}.")
         return 1
 
+    if (on_gpu and mode in ["normal", "direct"]
+            and _fusable(waveform_features, time_delays, weights_phases, weights_sources)):
+        wf = _flat(waveform_features, np.float32)
+        wp = _flat(weights_phases, np.float32)
+        td = _flat(time_delays, np.int32)
+        ws = _flat(weights_sources, np.float32)
+        args = (wf.ctypes.data_as(_P_F), wp.ctypes.data_as(_P_F), td.ctypes.data_as(_P_I),
+                ws.ctypes.data_as(_P_F), n_samples, n_sources, n_stations,
+                waveform_features.shape[1], n_phases, oob)
+        if reduce in ["none", "None", None]:
+            beam = np.zeros(n_sources * n_samples, dtype=np.float32)
+            check(lib.bp_beamform_fused(*args, beam.ctypes.data_as(_P_F)),
+                  "bp_beamform_fused", lib)
+            return beam.reshape(n_sources, n_samples)
+        beam_max = np.zeros(n_samples, dtype=np.float32)
+        beam_argmax = np.zeros(n_samples, dtype=np.int32)
+        check(lib.bp_beamform_max_fused(*args, beam_max.ctypes.data_as(_P_F),
+                                        beam_argmax.ctypes.data_as(_P_I)),
+              "bp_beamform_max_fused", lib)
+        return beam_max, beam_argmax
+
     if mode in ["normal", "direct", "differential"]:
         F = prestack_traces(waveform_features, weights_phases, num_threads=num_threads,
                             device=device)
-        F = F.flatten().astype(np.float32)
-        time_delays = time_delays.flatten().astype(np.int32)
-        weights_sources = weights_sources.flatten().astype(np.float32)
+        F = _flat(F, np.float32)
+        time_delays = _flat(time_delays, np.int32)
+        weights_sources = _flat(weights_sources, np.float32)
         args = (F.ctypes.data_as(_P_F), time_delays.ctypes.data_as(_P_I),
                 weights_sources.ctypes.data_as(_P_F), n_samples, n_sources, n_stations,
                 n_phases)
@@ -156,8 +199,8 @@ def prestack_traces(waveform_features, weights_phases, num_threads=None, device=
     n_stations, n_channels, n_samples = waveform_features.shape
     _, _, n_phases = weights_phases.shape
     out = np.zeros(n_stations * n_samples * n_phases, dtype=np.float32)
-    wf = waveform_features.flatten().astype(np.float32)
-    wp = weights_phases.flatten().astype(np.float32)
+    wf = _flat(waveform_features, np.float32)
+    wp = _flat(weights_phases, np.float32)
     status = lib.prestack_waveform_features(
         wf.ctypes.data_as(_P_F), wp.ctypes.data_as(_P_F), n_samples, n_stations, n_channels,
         n_phases, int(num_threads), out.ctypes.data_as(_P_F))

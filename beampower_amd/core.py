@@ -51,7 +51,8 @@ LIBRARIES = {
 # Symbols the gpu library declares in include/beamform_gpu.h.
 GPU_SYMBOLS = (
     "beamform", "beamform_max", "prestack_waveform_features", "beamform_differential",
-    "bp_dropin_cache_clear", "bp_last_error", "bp_version",
+    "bp_dropin_cache_clear", "bp_beamform_max_fused", "bp_beamform_fused",
+    "bp_last_error", "bp_version",
     "bp_network_create", "bp_network_destroy", "bp_network_info", "bp_network_classes",
     "bp_network_create_ordered", "bp_network_source_order", "bp_kernel_shape",
     "bp_network_beamform_max", "bp_network_beamform",
@@ -134,6 +135,10 @@ def _declare_gpu_extension(lib):
     lib.bp_network_create.argtypes = [vp, vp, _SZ, _SZ, _SZ, ct.POINTER(vp)]
     lib.bp_network_destroy.argtypes = [vp]
     lib.bp_dropin_cache_clear.argtypes = []
+    lib.bp_beamform_max_fused.argtypes = [_P_F, _P_F, _P_I, _P_F, _SZ, _SZ, _SZ, _SZ, _SZ,
+                                          ct.c_int, _P_F, _P_I]
+    lib.bp_beamform_fused.argtypes = [_P_F, _P_F, _P_I, _P_F, _SZ, _SZ, _SZ, _SZ, _SZ,
+                                      ct.c_int, _P_F]
     lib.bp_network_create_ordered.argtypes = [vp, vp, _SZ, _SZ, _SZ, ct.c_int,
                                               ct.POINTER(vp)]
     lib.bp_network_source_order.argtypes = [vp, _P_I, vp]

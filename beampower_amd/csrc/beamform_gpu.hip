@@ -1101,6 +1101,7 @@ int dropin_inputs(const float *host_stp, size_t nf, const int *tau, const float 
                   size_t K, size_t S, size_t P, size_t T, hipStream_t st, DevBuf &f_stp,
                   DropinUse &use) {
   BP_TRY(f_stp.alloc(nf * sizeof(float)));
+  if (nf == 0) return dropin_network(tau, beta, K, S, P, T, st, use);
   int up_rc = BP_OK, dev = 0;
   HIP_TRY(hipGetDevice(&dev));
   std::string up_err;  // the upload thread's message, reported by this thread
@@ -1144,6 +1145,144 @@ int stage_dropin_features(const bp_network *net, const DevBuf &f_stp, size_t S, 
   v.n_samples = T;
   v.row_stride = ld;
   v.pad_before = (size_t)net->pad_lo;
+  return BP_OK;
+}
+
+// Features of a host-pointer call: prestacked (S, T, P), as beamform_max
+// and beamform take them, or raw (S, C, T) with their (S, C, P) phase
+// weights, as the fused entry points take them (raw; with C = 0 neither
+// array is read).
+struct HostFeatures {
+  const float *data = nullptr;
+  const float *alpha = nullptr;
+  size_t C = 0;
+  bool raw = false;
+};
+
+// Network and padded FP32 rows of a host-pointer call.  Raw features are
+// prestacked on the device straight into the padded rows
+// (beamform.c:108-152, the FMA chain of prestack_kernel).
+int dropin_stage(const HostFeatures &hf, const int *tau, const float *beta, size_t K,
+                 size_t S, size_t P, size_t T, hipStream_t st, DropinUse &use, DevBuf &f_in,
+                 DevBuf &d_alpha, DevBuf &f_pad, bp_feature_view &v) {
+  if (!hf.raw) {
+    BP_TRY(dropin_inputs(hf.data, S * T * P, tau, beta, K, S, P, T, st, f_in, use));
+    return stage_dropin_features(use.net.get(), f_in, S, T, P, f_pad, v, st);
+  }
+  const size_t C = hf.C;
+  BP_TRY(d_alpha.alloc(S * C * P * sizeof(float)));
+  if (C > 0)
+    HIP_TRY(hipMemcpyAsync(d_alpha.p, hf.alpha, S * C * P * sizeof(float),
+                           hipMemcpyHostToDevice, st));
+  BP_TRY(dropin_inputs(hf.data, S * C * T, tau, beta, K, S, P, T, st, f_in, use));
+  const bp_network *net = use.net.get();
+  const size_t ld = (size_t)net->pad_lo + T + (size_t)net->pad_hi;
+  BP_TRY(f_pad.alloc(S * P * ld * sizeof(float)));
+  hipLaunchKernelGGL(bp::prestack_rows_kernel, dim3((unsigned)((ld + 255) / 256), (unsigned)S),
+                     dim3(256), 0, st, f_in.as<float>(), d_alpha.as<float>(), (int)S, (int)C,
+                     (int)T, (int)P, (long long)ld, (long long)net->pad_lo, f_pad.as<float>());
+  HIP_TRY(hipGetLastError());
+  v = bp_feature_view{};
+  v.data = f_pad.p;
+  v.fmt = BP_F32;
+  v.n_samples = T;
+  v.row_stride = ld;
+  v.pad_before = (size_t)net->pad_lo;
+  return BP_OK;
+}
+
+// beamform_max (beamform.c:230-336) of a host-pointer call.
+int dropin_max(const HostFeatures &hf, const int *time_delays, const float *weights_sources,
+               size_t n_samples, size_t n_sources, size_t n_stations, size_t n_phases,
+               int out_of_bounds, float *beam_max, int *beam_argmax) {
+  if (out_of_bounds != 0 && out_of_bounds != 1)
+    return fail(BP_EINVAL, "out_of_bounds must be 0 (strict) or 1 (flexible), not %d",
+                out_of_bounds);
+  BP_TRY(check_shapes(n_samples, n_sources, n_stations, n_phases));
+  if (n_samples == 0) return BP_OK;
+  if (!beam_max || !beam_argmax) return fail(BP_EINVAL, "NULL output");
+  if (n_sources == 0 || n_stations == 0 || n_phases == 0) {
+    degenerate_max(n_samples, n_sources, out_of_bounds, beam_max, beam_argmax);
+    return BP_OK;
+  }
+  if ((!hf.data && !(hf.raw && hf.C == 0)) || (hf.raw && hf.C > 0 && !hf.alpha) ||
+      !time_delays || !weights_sources)
+    return fail(BP_EINVAL, "NULL input");
+  BP_TRY(check_device());
+  Stream st;
+  BP_TRY(st.create());
+  DropinUse use;
+  const size_t T = n_samples;
+  bp_feature_view v{};
+  DevBuf f_in, d_alpha, f_pad, keys, bmax, barg;
+  BP_TRY(dropin_stage(hf, time_delays, weights_sources, n_sources, n_stations, n_phases, T,
+                      st.s, use, f_in, d_alpha, f_pad, v));
+  bp_network *net = use.net.get();
+  BP_TRY(keys.alloc(T * sizeof(unsigned long long)));
+  BP_TRY(bmax.alloc(T * sizeof(float)));
+  BP_TRY(barg.alloc(T * sizeof(int)));
+  // the caller's beam_max is kept where no source is valid (beamform.c:332-333)
+  HIP_TRY(hipMemcpyAsync(bmax.p, beam_max, T * sizeof(float), hipMemcpyHostToDevice, st.s));
+  BP_TRY(network_max(net, v, out_of_bounds, bmax.as<float>(), barg.as<int>(),
+                     keys.as<unsigned long long>(), st.s));
+  BP_TRY(dropin_finish(time_delays, weights_sources, n_sources, n_stations, n_phases, st.s, use));
+  HIP_TRY(hipMemcpyAsync(beam_max, bmax.p, T * sizeof(float), hipMemcpyDeviceToHost, st.s));
+  HIP_TRY(hipMemcpyAsync(beam_argmax, barg.p, T * sizeof(int), hipMemcpyDeviceToHost, st.s));
+  HIP_TRY(hipStreamSynchronize(st.s));
+  return BP_OK;
+}
+
+// beamform (reduce='none', beamform.c:154-228) of a host-pointer call.
+int dropin_none(const HostFeatures &hf, const int *time_delays, const float *weights_sources,
+                size_t n_samples, size_t n_sources, size_t n_stations, size_t n_phases,
+                int out_of_bounds, float *beam) {
+  if (out_of_bounds != 0 && out_of_bounds != 1)
+    return fail(BP_EINVAL, "out_of_bounds must be 0 (strict) or 1 (flexible), not %d",
+                out_of_bounds);
+  BP_TRY(check_shapes(n_samples, n_sources, n_stations, n_phases));
+  if (n_samples == 0 || n_sources == 0) return BP_OK;
+  if (!beam) return fail(BP_EINVAL, "NULL output");
+  const size_t T = n_samples, K = n_sources;
+  if (n_stations == 0 || n_phases == 0) {
+    // no terms: strict leaves every cell invalid (0), flexible gives 0 beams
+    std::memset(beam, 0, K * T * sizeof(float));
+    return BP_OK;
+  }
+  if ((!hf.data && !(hf.raw && hf.C == 0)) || (hf.raw && hf.C > 0 && !hf.alpha) ||
+      !time_delays || !weights_sources)
+    return fail(BP_EINVAL, "NULL input");
+  BP_TRY(check_device());
+  Stream st;
+  BP_TRY(st.create());
+  DropinUse use;
+  bp_feature_view v{};
+  DevBuf f_in, d_alpha, f_pad, dbeam;
+  BP_TRY(dropin_stage(hf, time_delays, weights_sources, n_sources, n_stations, n_phases, T,
+                      st.s, use, f_in, d_alpha, f_pad, v));
+  bp_network *net = use.net.get();
+  BP_TRY(dbeam.alloc(K * T * sizeof(float)));
+  // invalid strict cells read back as 0, as the wrapper's zero-filled buffer
+  const long long n = (long long)(K * T);
+  const unsigned fill_blocks = (unsigned)std::min<long long>(65536, (n + 255) / 256);
+  hipLaunchKernelGGL(bp::fill_f32_kernel, dim3(fill_blocks), dim3(256), 0, st.s,
+                     dbeam.as<float>(), n, 0.0f);
+  HIP_TRY(hipGetLastError());
+  std::shared_ptr<Schedule> sc;
+  BP_TRY(ensure_schedule(net, BP_F32, nullptr, st.s, sc));
+  bp::StackArgs a = make_args_view(net, v, out_of_bounds, *sc);
+  a.beam = dbeam.as<float>();
+  BP_TRY((launch_stack<BP_F32, true>(net, a, 0, (int)K, st.s)));
+  BP_TRY(dropin_finish(time_delays, weights_sources, n_sources, n_stations, n_phases, st.s, use));
+  HIP_TRY(hipMemcpyAsync(beam, dbeam.p, K * T * sizeof(float), hipMemcpyDeviceToHost, st.s));
+  HIP_TRY(hipStreamSynchronize(st.s));
+  return BP_OK;
+}
+
+// Channels of a fused call: C fits the prestack kernel and S * C * T
+// elements are addressable.
+int check_channels(size_t T, size_t S, size_t C) {
+  if (!size_ok(C) || (S > 0 && C > 0 && T > 0 && (S * C * T) / (S * C) != T))
+    return fail(BP_EINVAL, "n_stations * n_channels * n_samples overflows (C=%zu)", C);
   return BP_OK;
 }
 
@@ -2297,87 +2436,48 @@ int bp_prestack_planar(const float *u_sct, const float *weights_phases, size_t n
 int beamform_max(float *waveform_features, int *time_delays, float *weights_sources,
                  size_t n_samples, size_t n_sources, size_t n_stations, size_t n_phases,
                  int out_of_bounds, float *beam_max, int *beam_argmax) try {
-  if (out_of_bounds != 0 && out_of_bounds != 1)
-    return fail(BP_EINVAL, "out_of_bounds must be 0 (strict) or 1 (flexible), not %d",
-                out_of_bounds);
-  BP_TRY(check_shapes(n_samples, n_sources, n_stations, n_phases));
-  if (n_samples == 0) return BP_OK;
-  if (!beam_max || !beam_argmax) return fail(BP_EINVAL, "NULL output");
-  if (n_sources == 0 || n_stations == 0 || n_phases == 0) {
-    degenerate_max(n_samples, n_sources, out_of_bounds, beam_max, beam_argmax);
-    return BP_OK;
-  }
-  if (!waveform_features || !time_delays || !weights_sources)
-    return fail(BP_EINVAL, "NULL input");
-  BP_TRY(check_device());
-  Stream st;
-  BP_TRY(st.create());
-  DropinUse use;
-  const size_t T = n_samples, nf = n_stations * n_samples * n_phases;
-  bp_feature_view v{};
-  DevBuf f_stp, f_pad, keys, bmax, barg;
-  BP_TRY(dropin_inputs(waveform_features, nf, time_delays, weights_sources, n_sources,
-                       n_stations, n_phases, n_samples, st.s, f_stp, use));
-  bp_network *net = use.net.get();
-  BP_TRY(stage_dropin_features(net, f_stp, n_stations, T, n_phases, f_pad, v, st.s));
-  BP_TRY(keys.alloc(T * sizeof(unsigned long long)));
-  BP_TRY(bmax.alloc(T * sizeof(float)));
-  BP_TRY(barg.alloc(T * sizeof(int)));
-  // the caller's beam_max is kept where no source is valid (beamform.c:332-333)
-  HIP_TRY(hipMemcpyAsync(bmax.p, beam_max, T * sizeof(float), hipMemcpyHostToDevice, st.s));
-  BP_TRY(network_max(net, v, out_of_bounds, bmax.as<float>(), barg.as<int>(),
-                     keys.as<unsigned long long>(), st.s));
-  BP_TRY(dropin_finish(time_delays, weights_sources, n_sources, n_stations, n_phases, st.s, use));
-  HIP_TRY(hipMemcpyAsync(beam_max, bmax.p, T * sizeof(float), hipMemcpyDeviceToHost, st.s));
-  HIP_TRY(hipMemcpyAsync(beam_argmax, barg.p, T * sizeof(int), hipMemcpyDeviceToHost, st.s));
-  HIP_TRY(hipStreamSynchronize(st.s));
-  return BP_OK;
+  HostFeatures hf;
+  hf.data = waveform_features;
+  return dropin_max(hf, time_delays, weights_sources, n_samples, n_sources, n_stations,
+                    n_phases, out_of_bounds, beam_max, beam_argmax);
 } BP_CATCH
 
 int beamform(float *waveform_features, int *time_delays, float *weights_sources,
              size_t n_samples, size_t n_sources, size_t n_stations, size_t n_phases,
              int out_of_bounds, float *beam) try {
-  if (out_of_bounds != 0 && out_of_bounds != 1)
-    return fail(BP_EINVAL, "out_of_bounds must be 0 (strict) or 1 (flexible), not %d",
-                out_of_bounds);
-  BP_TRY(check_shapes(n_samples, n_sources, n_stations, n_phases));
-  if (n_samples == 0 || n_sources == 0) return BP_OK;
-  if (!beam) return fail(BP_EINVAL, "NULL output");
-  const size_t T = n_samples, K = n_sources;
-  if (n_stations == 0 || n_phases == 0) {
-    // no terms: strict leaves every cell invalid (0), flexible gives 0 beams
-    std::memset(beam, 0, K * T * sizeof(float));
-    return BP_OK;
-  }
-  if (!waveform_features || !time_delays || !weights_sources)
-    return fail(BP_EINVAL, "NULL input");
-  BP_TRY(check_device());
-  Stream st;
-  BP_TRY(st.create());
-  DropinUse use;
-  bp_feature_view v{};
-  DevBuf f_stp, f_pad, dbeam;
-  BP_TRY(dropin_inputs(waveform_features, n_stations * T * n_phases, time_delays,
-                       weights_sources, n_sources, n_stations, n_phases, n_samples, st.s, f_stp,
-                       use));
-  bp_network *net = use.net.get();
-  BP_TRY(stage_dropin_features(net, f_stp, n_stations, T, n_phases, f_pad, v, st.s));
-  BP_TRY(dbeam.alloc(K * T * sizeof(float)));
-  // invalid strict cells read back as 0, as the wrapper's zero-filled buffer
-  const long long n = (long long)(K * T);
-  const unsigned fill_blocks = (unsigned)std::min<long long>(65536, (n + 255) / 256);
-  hipLaunchKernelGGL(bp::fill_f32_kernel, dim3(fill_blocks), dim3(256), 0, st.s,
-                     dbeam.as<float>(), n, 0.0f);
-  HIP_TRY(hipGetLastError());
-  std::shared_ptr<Schedule> sc;
-  BP_TRY(ensure_schedule(net, BP_F32, nullptr, st.s, sc));
-  bp::StackArgs a = make_args_view(net, v, out_of_bounds, *sc);
-  a.beam = dbeam.as<float>();
-  BP_TRY((launch_stack<BP_F32, true>(net, a, 0, (int)K, st.s)));
-  BP_TRY(dropin_finish(time_delays, weights_sources, n_sources, n_stations, n_phases, st.s, use));
-  HIP_TRY(hipMemcpyAsync(beam, dbeam.p, K * T * sizeof(float), hipMemcpyDeviceToHost, st.s));
-  HIP_TRY(hipStreamSynchronize(st.s));
-  return BP_OK;
+  HostFeatures hf;
+  hf.data = waveform_features;
+  return dropin_none(hf, time_delays, weights_sources, n_samples, n_sources, n_stations,
+                     n_phases, out_of_bounds, beam);
+} BP_CATCH
+
+int bp_beamform_max_fused(const float *waveform_features, const float *weights_phases,
+                          const int *time_delays, const float *weights_sources,
+                          size_t n_samples, size_t n_sources, size_t n_stations,
+                          size_t n_channels, size_t n_phases, int out_of_bounds,
+                          float *beam_max, int *beam_argmax) try {
+  BP_TRY(check_channels(n_samples, n_stations, n_channels));
+  HostFeatures hf;
+  hf.data = waveform_features;
+  hf.alpha = weights_phases;
+  hf.C = n_channels;
+  hf.raw = true;
+  return dropin_max(hf, time_delays, weights_sources, n_samples, n_sources, n_stations,
+                    n_phases, out_of_bounds, beam_max, beam_argmax);
+} BP_CATCH
+
+int bp_beamform_fused(const float *waveform_features, const float *weights_phases,
+                      const int *time_delays, const float *weights_sources, size_t n_samples,
+                      size_t n_sources, size_t n_stations, size_t n_channels,
+                      size_t n_phases, int out_of_bounds, float *beam) try {
+  BP_TRY(check_channels(n_samples, n_stations, n_channels));
+  HostFeatures hf;
+  hf.data = waveform_features;
+  hf.alpha = weights_phases;
+  hf.C = n_channels;
+  hf.raw = true;
+  return dropin_none(hf, time_delays, weights_sources, n_samples, n_sources, n_stations,
+                     n_phases, out_of_bounds, beam);
 } BP_CATCH
 
 int prestack_waveform_features(float *waveform_features, float *weights_phases,

@@ -485,6 +485,27 @@ __global__ void prestack_kernel(const float *__restrict__ u,
   }
 }
 
+// The same prestack written straight into planar rows of stride ld that
+// hold sample t at t + pad, with zeros in the padding (the rows a network's
+// LDS tiles stage without edge guards).
+__global__ void prestack_rows_kernel(const float *__restrict__ u,
+                                     const float *__restrict__ alpha, int S, int C, int T,
+                                     int P, long long ld, long long pad,
+                                     float *__restrict__ out) {
+  const int s = blockIdx.y;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ld) return;
+  const long long t = i - pad;
+  const bool in_range = t >= 0 && t < T;
+  for (int p = 0; p < P; ++p) {
+    float acc = 0.0f;
+    if (in_range)
+      for (int c = 0; c < C; ++c)
+        acc = fmaf(alpha[((size_t)s * C + c) * P + p], u[((size_t)s * C + c) * T + t], acc);
+    out[((size_t)s * P + p) * ld + i] = acc;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Reduction helpers.
 // ---------------------------------------------------------------------------

@@ -86,6 +86,28 @@ int beamform_differential(float *waveform_features, int *time_delays, float *wei
                           size_t n_samples, size_t n_sources, size_t n_stations,
                           size_t n_phases, int num_threads, float *beam);
 
+/* The reference wrapper's two library calls for device="gpu" in one:
+ * prestack_waveform_features (beamform_cpu.h:4, beamform.c:108-152) followed
+ * by beamform_max (beamform_gpu.h:2) or beamform (beamform_gpu.h:1), as
+ * beampower.py:103-105,156-190 makes them.  The (n_stations, n_channels,
+ * n_samples) features and the (n_stations, n_channels, n_phases) phase
+ * weights go to the device as they are and are prestacked there, straight
+ * into the rows the stack kernels read, so the prestacked array never
+ * crosses PCIe.  Results are bit-identical to the two calls; the other
+ * arguments, the outputs and the reuse of the network for equal
+ * time_delays and weights_sources are those of beamform_max / beamform.
+ * Host pointers, synchronous.  Extension: beampower_amd.beamform uses them
+ * for device="gpu". */
+int bp_beamform_max_fused(const float *waveform_features, const float *weights_phases,
+                          const int *time_delays, const float *weights_sources,
+                          size_t n_samples, size_t n_sources, size_t n_stations,
+                          size_t n_channels, size_t n_phases, int out_of_bounds,
+                          float *beam_max, int *beam_argmax);
+int bp_beamform_fused(const float *waveform_features, const float *weights_phases,
+                      const int *time_delays, const float *weights_sources, size_t n_samples,
+                      size_t n_sources, size_t n_stations, size_t n_channels,
+                      size_t n_phases, int out_of_bounds, float *beam);
+
 /* ---- extension API (no counterpart in the reference) -------------------- */
 
 /* Thread-local message for the last non-zero status of this thread. */

@@ -154,6 +154,62 @@ def test_degenerate_shapes_follow_reference_semantics():
         assert np.all(m == expect) and np.all(a == 0)
 
 
+def test_fused_entry_points_check_arguments_without_a_device():
+    """bp_beamform_max_fused / bp_beamform_fused refuse what beamform_max /
+    beamform refuse, before touching a device, and keep the degenerate
+    shapes' semantics."""
+    lib = bp.load_library("gpu")
+    P_F, P_I = ct.POINTER(ct.c_float), ct.POINTER(ct.c_int)
+    u = np.zeros(12, np.float32)
+    alpha = np.ones(6, np.float32)
+    tau = np.zeros(4, np.int32)
+    beta = np.ones(2, np.float32)
+    m = np.full(2, 7.0, np.float32)
+    a = np.full(2, 9, np.int32)
+    ins = (u.ctypes.data_as(P_F), alpha.ctypes.data_as(P_F), tau.ctypes.data_as(P_I),
+           beta.ctypes.data_as(P_F))
+    rc = lib.bp_beamform_max_fused(*ins, 2, 2, 1, 3, 2, 5, m.ctypes.data_as(P_F),
+                                   a.ctypes.data_as(P_I))
+    assert rc == -22 and "out_of_bounds" in core.last_error(lib)
+    assert lib.bp_beamform_fused(*ins, 2, 2, 1, 3, 2, -1, m.ctypes.data_as(P_F)) == -22
+    assert lib.bp_beamform_max_fused(None, None, None, None, 1 << 40, 2, 1, 3, 2, 0,
+                                     None, None) == -22
+    assert "too large" in core.last_error(lib)
+    assert lib.bp_beamform_max_fused(None, None, None, None, 5, 2, 1, 1 << 40, 2, 0,
+                                     None, None) == -22
+    assert "n_channels" in core.last_error(lib)
+    # channels present but no phase weights
+    rc = lib.bp_beamform_max_fused(ins[0], None, ins[2], ins[3], 2, 2, 1, 3, 2, 0,
+                                   m.ctypes.data_as(P_F), a.ctypes.data_as(P_I))
+    assert rc == -22 and "NULL input" in core.last_error(lib)
+    for K, S, oob, expect in ((0, 3, 0, 7.0), (4, 0, 1, 0.0)):
+        m = np.full(5, 7.0, np.float32)
+        a = np.full(5, 9, np.int32)
+        assert lib.bp_beamform_max_fused(None, None, None, None, 5, K, S, 3, 2, oob,
+                                         m.ctypes.data_as(P_F), a.ctypes.data_as(P_I)) == 0
+        assert np.all(m == expect) and np.all(a == 0)
+
+
+def test_wrapper_passes_library_typed_inputs_without_copies():
+    from beampower_amd import beampower as bpw
+    x = np.arange(24, dtype=np.float32).reshape(2, 3, 4)
+    y = bpw._flat(x, np.float32)
+    assert np.shares_memory(x, y) and y.shape == (24,)
+    for z, dt in ((np.asfortranarray(x), np.float32), (x.astype(np.float64), np.float32),
+                  (x[:, :, ::2], np.float32), (x * 1.7, np.int32)):
+        got = bpw._flat(z, dt)
+        want = z.flatten().astype(dt)
+        assert got.dtype == want.dtype and np.array_equal(got, want)
+    u = np.zeros((5, 3, 100), np.float32)
+    tau = np.zeros((7, 5, 2), np.int32)
+    assert bpw._fusable(u, tau, np.zeros((5, 3, 2)), np.ones((7, 5)))
+    assert not bpw._fusable(u, tau, np.zeros((5, 3, 1)), np.ones((7, 5)))
+    assert not bpw._fusable(u, tau, np.zeros((5, 2, 2)), np.ones((7, 5)))
+    assert not bpw._fusable(u, tau, np.zeros((5, 3, 2)), np.ones((7, 4)))
+    assert not bpw._fusable(u, np.zeros((7, 4, 2), np.int32), np.zeros((5, 3, 2)),
+                            np.ones((7, 4)))
+
+
 def test_sec_to_samp_follows_tutorial():
     # notebooks/3_beampower.ipynb:380-392: sign(t) * int(|t*sr| + 0.2)
     t = np.array([0.0, 0.03, 0.039, -0.039, 1.0, 2.99, -2.99, 0.0319])
