@@ -1330,10 +1330,17 @@ int init_batch_attrs() {
 
 // Windows of one LDS class run by one fused launch: descriptors (host copy
 // and device), their work items numbered consecutively, the XCD ranges.
+// Default grid of a batch: one persistent launch per window class, three
+// work-groups per CU (the occupancy of the stack kernel's one-per-item
+// grid) over a work queue.  On the 256-window batched workload it equals
+// the windows launched one by one; on a serving-like mix it is 3 % faster.
+constexpr int kBatchBlocksPerCU = 3;
+
 struct FusedSet {
   std::vector<bp::BatchWindow> wins;
   DevBuf dwins;
   DevBuf dsplit;            // 9 item indices: XCD x walks [split[x], split[x+1])
+  DevBuf dqueue;            // 8 work-queue counters of a persistent launch
   long long split_h[9] = {};  // host copy (the source of an asynchronous upload)
   long long n_items = 0;
   long long max_range = 0;  // largest XCD range (items)
@@ -1347,7 +1354,7 @@ struct FusedSet {
 
 struct bp_batch {
   int fmt = BP_F32, P = 0, n = 0, max_T = 0;
-  bp_batch_config cfg{kTimeGroup, 2, -1, 0.0f};
+  bp_batch_config cfg{kTimeGroup, 2, kBatchBlocksPerCU, 0.0f};
   // per LDS class: every window fused (blocks_per_cu >= 0), or the windows
   // that fill the GPU alone in launches of their own and the others fused
   // (blocks_per_cu < 0)
@@ -1454,6 +1461,7 @@ int finish_fused(bp_batch *b, FusedSet &f) {
   }
   for (int x = 0; x < 8; ++x) f.max_range = std::max(f.max_range, split[x + 1] - split[x]);
   BP_TRY(b->table(f.dsplit, sizeof(f.split_h)));
+  BP_TRY(b->table(f.dqueue, 8 * sizeof(unsigned long long)));
   return b->put(f.dsplit, f.split_h, sizeof(f.split_h));
 }
 
@@ -1472,9 +1480,8 @@ int launch_window_alone(const bp::StackArgs &a, int nb, hipStream_t st) {
 }
 
 // A fused launch over the windows of `f`.  blocks_per_cu > 0: a persistent
-// grid of that many work-groups per CU; otherwise one work-group per item
-// (8 x the largest XCD range; the hardware then refills CUs as work-groups
-// finish, which keeps co-resident work-groups out of phase).
+// grid of that many work-groups per CU over a work queue; otherwise one
+// work-group per item (8 x the largest XCD range).
 template <int FMT, int P>
 int launch_fused(const FusedSet &f, int nb, int blocks_per_cu, hipStream_t st) {
   if (f.n_items == 0) return BP_OK;
@@ -1487,16 +1494,24 @@ int launch_fused(const FusedSet &f, int nb, int blocks_per_cu, hipStream_t st) {
     const int per_cu = std::max(1, std::min(blocks_per_cu, kLdsPerCU / lds));
     want = std::min((long long)kNumCU * per_cu, (f.n_items + 7) / 8 * 8);
   }
-  // the kernel walks its XCD's items in a loop: any grid covers them
+  // the kernel walks its XCD's items in a loop: any grid covers them.  A
+  // persistent grid takes its items from a work queue, cleared here by a
+  // kernel (no memset node when the batch is captured in a graph).
   const unsigned grid = (unsigned)std::max<long long>(8, std::min(want, kMaxLaunchBlocks));
+  unsigned long long *queue = nullptr;
+  if (blocks_per_cu > 0) {
+    queue = f.dqueue.as<unsigned long long>();
+    hipLaunchKernelGGL(bp::fill_u64_kernel, dim3(1), dim3(64), 0, st, queue, 8LL, 0ull);
+    HIP_TRY(hipGetLastError());
+  }
   if (nb == 3)
     hipLaunchKernelGGL((bp::stack_batch_kernel<FMT, P, kKW, kW, 3>), dim3(grid), dim3(64 * kW),
                        lds, st, f.dwins.as<bp::BatchWindow>(), (int)f.wins.size(),
-                       f.dsplit.as<long long>());
+                       f.dsplit.as<long long>(), queue);
   else
     hipLaunchKernelGGL((bp::stack_batch_kernel<FMT, P, kKW, kW, 2>), dim3(grid), dim3(64 * kW),
                        lds, st, f.dwins.as<bp::BatchWindow>(), (int)f.wins.size(),
-                       f.dsplit.as<long long>());
+                       f.dsplit.as<long long>(), queue);
   HIP_TRY(hipGetLastError());
   return BP_OK;
 }
@@ -2617,6 +2632,7 @@ int batch_create(int n_windows, bp_network *const *networks, const bp_feature_vi
   BP_TRY(init_batch_attrs());
   BP_TRY(init_kernel_attrs());
   bp_batch_config cfg = b->cfg;
+  if (pl) cfg.blocks_per_cu = pl->blocks_per_cu;
   {
     std::lock_guard<std::mutex> lock(g_tune_mu);
     auto it = g_tune_cache.find(tune_key(b.get()));
@@ -2634,8 +2650,9 @@ size_t bp_batch_table_bytes(int n_windows) {
   const size_t n = (size_t)std::max(0, n_windows);
   auto up = [](size_t v) { return (v + 255) / 256 * 256; };
   // four fused sets (two classes, all windows and the small ones): window
-  // descriptors and XCD split each; the keys table
-  return 4 * (up(n * sizeof(bp::BatchWindow)) + up(9 * sizeof(long long))) +
+  // descriptors, XCD split and work-queue counters each; the keys table
+  return 4 * (up(n * sizeof(bp::BatchWindow)) + up(9 * sizeof(long long)) +
+              up(8 * sizeof(unsigned long long))) +
          up(n * sizeof(bp::BatchKeys)) + 256;
 }
 

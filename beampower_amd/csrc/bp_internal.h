@@ -11,11 +11,14 @@
 // that the caller owns and keeps until the batch is destroyed, uploaded
 // asynchronously on `stream` (launch the batch on the same stream).  Such a
 // batch allocates no device memory and synchronises nothing, so creating
-// and destroying it cannot stall work on other streams.
+// and destroying it cannot stall work on other streams.  blocks_per_cu is
+// the batch's grid policy (bp_batch_config), unless a tuned configuration
+// for its shape exists.
 struct bp_batch_placement {
   void *ws;
   size_t ws_bytes;
   hipStream_t stream;
+  int blocks_per_cu;
 };
 
 // Device bytes the tables of a batch of n_windows windows need at most.

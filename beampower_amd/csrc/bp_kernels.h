@@ -1315,16 +1315,23 @@ void stack_lds_kernel(StackArgs a) {
 // Batched form: one launch over the LDS-class tiles of many windows (each
 // its own network, length and feature buffer), reduce='max'.  Window w
 // contributes n_time_tiles(w) * n_list(w) work items, numbered from
-// item_begin(w).  The grid is a multiple of 8 work-groups, either persistent
-// (a few per CU, each looping over items) or one per item (8 x the largest
-// XCD range; surplus work-groups exit at once).  Work-group b runs on XCD
-// b % 8, and XCD x walks the contiguous item range
-// [split[x], split[x+1]), so that within an XCD consecutive items
-// are consecutive time tiles of one window (the same L2 locality as one
-// launch per window).  Items cost different amounts (S varies per window),
-// so the host places the XCD boundaries `split` at equal shares of the total
-// cost (items x stations) rather than of the item count; within an XCD the
-// round-robin over its work-groups mixes the windows of its range.
+// item_begin(w).  The grid is a multiple of 8 work-groups.  Work-group b
+// runs on XCD b % 8, and XCD x walks the contiguous item range
+// [split[x], split[x+1]), so that within an XCD consecutive items are
+// consecutive time tiles of one window (the same L2 locality as one launch
+// per window).  Items cost different amounts (S varies per window), so the
+// host places the XCD boundaries `split` at equal shares of the total cost
+// (items x stations) rather than of the item count.
+//   queue == NULL: one work-group per item (8 x the largest XCD range;
+//     surplus work-groups exit at once), or a grid that strides over its
+//     XCD's range;
+//   queue != NULL: a persistent grid over a work queue.  queue[x] counts
+//     the items of XCD x's range handed out so far (zero at launch).  A
+//     work-group takes the next item of its own XCD's range with an atomic
+//     add until the range is exhausted, then takes items of the other
+//     XCDs' ranges the same way, so no XCD idles while another still has
+//     work.  Counters only grow, so every work-group ends after at most
+//     eight failed takes.
 // ---------------------------------------------------------------------------
 struct BatchWindow {
   StackArgs a;           // the window's arguments: its tiles of one LDS class
@@ -1334,32 +1341,59 @@ struct BatchWindow {
 typedef const __attribute__((address_space(4))) BatchWindow const_batch_window;
 
 template <int FMT, int P, int KW, int W, int NB>
+__device__ __forceinline__ void batch_item(const_batch_window *cw, int n_wins, long long L,
+                                           unsigned char *smem) {
+  // the window holding item L: the last with item_begin <= L
+  int lo = 0, hi = n_wins - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (cw[mid].item_begin <= L)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  // the window's arguments stay in constant memory: fields are scalar
+  // loads where used, like a kernel argument's
+  typedef const __attribute__((address_space(4))) StackArgs const_args;
+  const_args &a = cw[lo].a;
+  int tt, li;
+  block_tile((int)(L - cw[lo].item_begin), a.n_time_tiles, a.n_list, a.time_group, tt, li);
+  StackLds<FMT, P, KW, W, NB, false>::tile_job(a, tt, as_const(a.tiles)[li], smem);
+  __syncthreads();  // the LDS is reused by the next item
+}
+
+template <int FMT, int P, int KW, int W, int NB>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(BP_WAVES_PER_EU)))
-void stack_batch_kernel(const BatchWindow *wins, int n_wins, const long long *split) {
+void stack_batch_kernel(const BatchWindow *wins, int n_wins, const long long *split,
+                        unsigned long long *queue) {
   extern __shared__ __align__(16) unsigned char smem[];
   const_batch_window *cw = (const_batch_window *)wins;
   typedef const __attribute__((address_space(4))) long long const_ll;
   const int x = blockIdx.x & 7;
-  const long long end = ((const_ll *)split)[x + 1];
-  const int step = gridDim.x >> 3;
-  for (long long L = ((const_ll *)split)[x] + (blockIdx.x >> 3); L < end; L += step) {
-    // the window holding item L: the last with item_begin <= L
-    int lo = 0, hi = n_wins - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (cw[mid].item_begin <= L)
-        lo = mid;
-      else
-        hi = mid - 1;
+  if (!queue) {
+    const long long end = ((const_ll *)split)[x + 1];
+    const int step = gridDim.x >> 3;
+    for (long long L = ((const_ll *)split)[x] + (blockIdx.x >> 3); L < end; L += step)
+      batch_item<FMT, P, KW, W, NB>(cw, n_wins, L, smem);
+    return;
+  }
+  // The taken item is broadcast through the first word of the LDS, which
+  // tile_job clears before it uses it (the reduction keys).
+  unsigned long long *slot = reinterpret_cast<unsigned long long *>(smem);
+  for (int r = 0; r < 8; ++r) {
+    const int q = (x + r) & 7;
+    const long long b0 = ((const_ll *)split)[q];
+    const long long n = ((const_ll *)split)[q + 1] - b0;
+    for (;;) {
+      if (threadIdx.x == 0) *slot = atomicAdd(queue + q, 1ull);
+      __syncthreads();
+      const unsigned long long g = *slot;
+      __syncthreads();  // every thread has read the slot before it is reused
+      const long long i = (long long)(((unsigned long long)(unsigned)uniform((int)(g >> 32)) << 32) |
+                                      (unsigned)uniform((int)g));
+      if (i >= n) break;
+      batch_item<FMT, P, KW, W, NB>(cw, n_wins, b0 + i, smem);
     }
-    // the window's arguments stay in constant memory: fields are scalar
-    // loads where used, like a kernel argument's
-    typedef const __attribute__((address_space(4))) StackArgs const_args;
-    const_args &a = cw[lo].a;
-    int tt, li;
-    block_tile((int)(L - cw[lo].item_begin), a.n_time_tiles, a.n_list, a.time_group, tt, li);
-    StackLds<FMT, P, KW, W, NB, false>::tile_job(a, tt, as_const(a.tiles)[li], smem);
-    __syncthreads();  // the LDS is reused by the next item
   }
 }
 

@@ -397,8 +397,13 @@ void bp_server::launch(std::vector<Request *> batch, int slot) {
     }
     f.reqs = batch;
     hipStream_t st = compute[slot];
+    // Grid: windows that fill the GPU alone in launches of their own, the
+    // others one work-group per item.  A persistent grid holds every CU
+    // slot until its queue is empty, so a batch launched on the other slot
+    // would wait for all of it; one work-group per item lets the hardware
+    // interleave the two (p50 at 50 req/s: 6 against 12 ms persistent).
     const bp_batch_placement pl{static_cast<char *>(tables) + (size_t)slot * table_bytes,
-                                table_bytes, st};
+                                table_bytes, st, -1};
     rc = bp_batch_create_placed(n, nv.data(), views.data(), oob.data(), bmax.data(),
                                 barg.data(), keys.data(), pl, &f.batch);
     if (rc != BP_OK) fail(rc, bp_last_error());

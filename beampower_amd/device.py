@@ -412,12 +412,13 @@ class Batch:
         return dict(time_group=c.time_group, stage_buffers=c.stage_buffers,
                     blocks_per_cu=c.blocks_per_cu, ms=c.ms)
 
-    def set_config(self, time_group=8, stage_buffers=2, blocks_per_cu=-1):
+    def set_config(self, time_group=8, stage_buffers=2, blocks_per_cu=3):
         """Launch configuration (bp_batch_config; the defaults are the
         library's): time tiles per block-order group, staging buffers, and
-        -1 = windows that fill the GPU alone in launches of their own, the
-        others fused; 0 = all fused, one work-group per item; 1-3 = all
-        fused, a persistent grid of that many work-groups per CU."""
+        1-4 = all windows in one persistent launch of that many work-groups
+        per CU taking work items from a queue; 0 = all fused, one work-group
+        per item; -1 = windows that fill the GPU alone in launches of their
+        own, the others fused."""
         c = BatchConfig(time_group, stage_buffers, blocks_per_cu, 0.0)
         check(self.lib.bp_batch_set_config(self._h, ct.byref(c)), "bp_batch_set_config",
               self.lib)

@@ -212,6 +212,21 @@ def bench_batched(args, log):
                   "windows_one_by_one_ms": t_alone * 1e3,
                   "identical_to_windows_alone": bool(same)}
     log(f"batched f32: {t_graph * 1e3:.1f} ms (graph), {t_alone * 1e3:.1f} ms one by one")
+    # every grid policy on the same batch (one timed replay each)
+    if not args.tune:
+        cfg = b32.config()
+        out["grids"] = []
+        for bpc in (-1, 0, 2, 3):
+            b32.set_config(cfg["time_group"], cfg["stage_buffers"], bpc)
+            t = timed(b32.graph_launch, 1)
+            ok = all(np.array_equal(b32.beam_max[i].cpu().numpy().view(np.uint32),
+                                    ref[i][0].view(np.uint32))
+                     and np.array_equal(b32.beam_argmax[i].cpu().numpy(), ref[i][1])
+                     for i in range(n))
+            out["grids"].append({"blocks_per_cu": bpc, "graph_ms": t * 1e3,
+                                 "identical_to_windows_alone": bool(ok)})
+            log(f"batched f32, blocks_per_cu {bpc}: {t * 1e3:.1f} ms")
+        b32.set_config(cfg["time_group"], cfg["stage_buffers"], cfg["blocks_per_cu"])
     del b32
     q = [(net, bpd.pack(F, bpd.BP_BF16, *net.padding()), oob) for net, F, oob in planar]
     b16 = bpd.Batch(q)
@@ -231,10 +246,10 @@ def bench_batched_mix(args, log):
     """The batched form on a serving-like mix: requests of the four serving
     networks (20/50/80/120 stations, 10k-300k sources, 30 s - 10 min
     windows, networks shared by several windows), FP32 features, in one
-    batch under each grid policy -- large windows in launches of their own
-    (the default), everything fused with one work-group per work item, and
-    one persistent launch of 1, 2 or 3 work-groups per CU -- against the
-    windows launched one by one; all bit-identical."""
+    batch under each grid policy -- large windows in launches of their own,
+    everything fused with one work-group per work item, and one persistent
+    launch of 1, 2 or 3 (the default) work-groups per CU over a work queue --
+    against the windows launched one by one; all bit-identical."""
     import numpy as np
     import torch
     from beampower_amd import device as bpd

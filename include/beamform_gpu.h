@@ -351,11 +351,11 @@ int bp_prestack_planar(const float *u_sct, const float *weights_phases,
  * window per call): n_windows windows, each with its own network, feature
  * view, bounds mode and DEVICE outputs (beam_max, beam_argmax and keys
  * scratch of n_samples elements, as for bp_network_beamform_max), all in one
- * feature format and with one number of phases.  By default a window with
- * at least 4096 work items (it fills the GPU alone) runs its LDS-class
- * tiles in a launch of its own and the other windows share one fused launch
- * per window class (see bp_batch_config); results are bit-identical to
- * bp_network_beamform_max_view on each window alone.
+ * feature format and with one number of phases.  By default the LDS-class
+ * tiles of all windows run in one persistent launch per window class, three
+ * work-groups per CU taking work items from a queue (see bp_batch_config);
+ * results are bit-identical to bp_network_beamform_max_view on each window
+ * alone.
  * Windows of one network may have different station scales.  Creation
  * builds missing schedules synchronously, and the batch holds the schedules
  * of its windows: later calls or batches of the same networks with other
@@ -372,11 +372,12 @@ typedef struct bp_batch bp_batch;
 typedef struct {
   int time_group;     /* time tiles per block-order group (default 8; 1 = time-major) */
   int stage_buffers;  /* 2 or 3 (3 only where two work-groups still fit a CU) */
-  int blocks_per_cu;  /* -1 (default): windows of >= 4096 work items in launches of their
-                         own, the others fused with one work-group per work item; 0: all
-                         fused, one work-group per work item; 1-4: all fused, one persistent
-                         launch of that many work-groups per CU (fewer where their LDS
-                         does not fit) */
+  int blocks_per_cu;  /* 1-4 (default 3): all windows in one persistent launch of that many
+                         work-groups per CU (fewer where their LDS does not fit), which
+                         take work items from a queue, their XCD's first; 0: all fused,
+                         one work-group per work item; -1: windows of >= 4096 work items
+                         in launches of their own, the others fused with one work-group
+                         per work item */
   float ms;           /* measured time of the whole batch (tuning output) */
 } bp_batch_config;
 int bp_batch_create(int n_windows, bp_network *const *networks,
