@@ -407,6 +407,22 @@ def bench_none_subset(args, log, w, F_plain):
     log(f"reduce='none' subset: {ms:.2f} ms per step")
     del net, V, beam
     torch.cuda.empty_cache()
+    # the Python call with host arrays: uploads, prestack, kernels, and the
+    # (K, T) beams copied back into a numpy array
+    import beampower_amd as bp
+    call_ms = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        b = bp.beamform(w.waveform_features, tau, w.weights_phases, beta, device="gpu",
+                        reduce="none")
+        call_ms.append((time.perf_counter() - t0) * 1e3)
+        del b
+    out["python_call"] = {"ms": min(call_ms), "calls_ms": call_ms,
+                          "note": "beampower_amd.beamform(..., reduce='none') with numpy "
+                                  "inputs; the 1.47 GB of beams come back through pinned "
+                                  "staging buffers"}
+    log(f"reduce='none' subset, python call: {min(call_ms):.1f} ms")
+    bp.load_library("gpu").bp_dropin_cache_clear()
     if not args.no_cpu_baseline:
         from oracle import oracle
         kind = "reference" if oracle.ref_available("cmake") else "port"

@@ -116,3 +116,13 @@ def test_c2_none_subset_matches_oracle(gpu_lib, c2, c2_max):
     t_hit = np.flatnonzero(hit)
     assert t_hit.size > 0
     assert bits_equal(b[pos[t_hit], t_hit], m[t_hit])
+
+
+def test_c2_prestack_on_gpu(gpu_lib, c2):
+    """The CPU library's prestack signature on the GPU at this shape: the
+    (50, 3, 90000) features go up and the (50, 90000, 2) traces come back
+    through pinned staging buffers; bit-identical to the oracle's chain."""
+    import beampower_amd as bp
+    w, F = c2
+    Fg = bp.prestack_traces(w.waveform_features, w.weights_phases, device="gpu")
+    assert bits_equal(Fg, F)
