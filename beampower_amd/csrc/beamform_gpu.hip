@@ -2332,12 +2332,37 @@ int beamform_differential(float *waveform_features, int *time_delays, float *wei
   BP_TRY(f_stp.alloc(S * T * P * sizeof(float)));
   BP_TRY(f_pl.alloc(S * T * P * sizeof(float)));
   BP_TRY(dbeam.alloc(K * sizeof(float)));
-  HIP_TRY(hipMemcpyAsync(tau.p, time_delays, K * S * P * sizeof(int), hipMemcpyHostToDevice,
-                         st.s));
-  HIP_TRY(hipMemcpyAsync(beta.p, weights, K * S * sizeof(float), hipMemcpyHostToDevice, st.s));
+  // the delay and weight tables (2.8 GB at 1,225 station pairs x 192,000
+  // sources) go up through the two pinned stages at once, the weights on a
+  // second host thread
   HIP_TRY(hipMemcpyAsync(f_stp.p, waveform_features, S * T * P * sizeof(float),
                          hipMemcpyHostToDevice, st.s));
   HIP_TRY(hipMemcpyAsync(dbeam.p, beam, K * sizeof(float), hipMemcpyHostToDevice, st.s));
+  {
+    int dev = 0, up_rc = BP_OK;
+    HIP_TRY(hipGetDevice(&dev));
+    std::string up_err;
+    std::thread up([&] {
+      try {
+        up_rc = hipSetDevice(dev) == hipSuccess  // a new thread starts on device 0
+                    ? upload_host(beta.p, weights, K * S * sizeof(float), st.s)
+                    : fail(BP_EHIP, "hipSetDevice(%d) failed", dev);
+      } catch (...) {
+        up_rc = fail(BP_EHIP, "weight upload failed");
+      }
+      if (up_rc != BP_OK) up_err = g_err;
+    });
+    int rc = BP_OK;
+    try {
+      rc = upload_host(tau.p, time_delays, K * S * P * sizeof(int), st.s);
+    } catch (...) {
+      up.join();
+      throw;
+    }
+    up.join();
+    BP_TRY(rc);
+    if (up_rc != BP_OK) return fail(up_rc, "%s", up_err.c_str());
+  }
   hipLaunchKernelGGL(bp::interleaved_to_planar_kernel,
                      dim3((unsigned)((T + 255) / 256), (unsigned)S), dim3(256), 0, st.s,
                      f_stp.as<float>(), (int)S, (int)T, (int)P, (long long)T, 0ll,
