@@ -146,6 +146,13 @@ def poisson_run(server, requests, rate, seed=0):
     fmas = sum(np.prod(np.array(server.networks[t.network])[[0, 2, 3]], dtype=np.float64)
                * t.features.shape[2] for t in tickets)
     batch = np.array([t.info["batch_windows"] for t in tickets])
+    # where a request's time goes (medians, ms): waiting for HBM budget,
+    # upload and prestack, waiting for a launch, on the GPU, results copied
+    stages = (("admit", "submit_s", "admit_s"), ("stage", "admit_s", "ready_s"),
+              ("queue", "ready_s", "launch_s"), ("gpu", "launch_s", "gpu_done_s"),
+              ("download", "gpu_done_s", "done_s"))
+    parts = {name: float(np.median([t.info[b] - t.info[a] for t in tickets]) * 1e3)
+             for name, a, b in stages}
     summary = {
         "rate_offered": rate, "requests": len(requests),
         "p50_ms": float(np.percentile(lat, 50) * 1e3),
@@ -156,5 +163,6 @@ def poisson_run(server, requests, rate, seed=0):
         "gather_fma_per_s": fmas / span,
         "samples": samples,
         "mean_batch_windows": float(batch.mean()),
+        "median_stage_ms": parts,
     }
     return tickets, summary
