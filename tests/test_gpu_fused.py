@@ -75,6 +75,8 @@ def test_fused_golden(gpu_lib, case):
     (64, 9, 4, 3, 1500, 700, 200, 1),     # four channels, three phases
     (65, 6, 2, 5, 1200, 300, 150, 0),     # five phases: direct kernel
     (66, 10, 3, 2, 6000, 400, 4000, 0),   # spread beyond the LDS window: direct kernel
+    (69, 5, 2, 2, 7, 50, 3, 1),           # a window shorter than one time tile
+    (70, 4, 3, 2, 1, 20, 0, 0),           # one sample
 ])
 def test_fused_against_oracle(gpu_lib, seed, S, C, P, T, K, tmax, oob):
     u, alpha, tau, beta = synthetic.random_case(seed, S=S, C=C, P=P, T=T, K=K,

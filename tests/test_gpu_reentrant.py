@@ -101,3 +101,33 @@ def test_device_api_on_a_side_stream(gpu_lib):
         assert bits_equal(m.cpu().numpy(), m0.cpu().numpy())
         assert np.array_equal(a.cpu().numpy(), a0.cpu().numpy())
         assert bits_equal(b.cpu().numpy(), b0.cpu().numpy())
+
+
+def test_concurrent_large_transfers(gpu_lib):
+    """Calls whose inputs or outputs reach 32 MiB cross PCIe through the
+    two pinned staging stages; with four such calls at once two of them find
+    both stages taken and copy from pageable memory instead.  Every result
+    must equal the same call made alone."""
+    jobs = []
+    for i in range(4):
+        # reduce='none' with a 43 MB output (1,200 sources x 9,000 samples)
+        u, alpha, tau, beta = synthetic.random_case(90 + i, S=9, C=3, P=2, T=9000, K=1200,
+                                                    tau_range=(0, 300))
+        jobs.append(("none", oracle.prestack(u, alpha), tau, beta, i % 2))
+        # reduce='max' with 35 MB of features (40 stations x 110,000 samples)
+        u, alpha, tau, beta = synthetic.random_case(95 + i, S=40, C=3, P=2, T=110_000, K=300,
+                                                    tau_range=(0, 400))
+        jobs.append(("max", oracle.prestack(u, alpha), tau, beta, i % 2))
+    seq = [call(gpu_lib, *j) for j in jobs]
+    with ThreadPoolExecutor(max_workers=8) as ex:
+        par = list(ex.map(lambda j: call(gpu_lib, *j), jobs))
+    for i, (x, y) in enumerate(zip(seq, par)):
+        for u, v in zip(x, y):
+            assert (bits_equal(u, v) if u.dtype == np.float32 else np.array_equal(u, v)), \
+                (i, jobs[i][0])
+    kind, F, tau, beta, oob = jobs[0]
+    assert bits_equal(seq[0][0], oracle.beamform(F, tau, beta, oob))
+    kind, F, tau, beta, oob = jobs[1]
+    mo, ao = oracle.beamform_max(F, tau, beta, oob)
+    assert bits_equal(seq[1][0], mo) and np.array_equal(seq[1][1], ao)
+    gpu_lib.bp_dropin_cache_clear()
