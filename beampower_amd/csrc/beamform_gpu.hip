@@ -927,21 +927,18 @@ PinnedStage *acquire_pinned(std::unique_lock<std::mutex> &lock) {
   return &ps;
 }
 
-int upload_host(void *dst, const void *src, size_t n, hipStream_t st) {
-  std::unique_lock<std::mutex> lock;
-  PinnedStage *ps = n < 2 * kPinnedBytes ? nullptr : acquire_pinned(lock);
-  if (!ps) {
-    HIP_TRY(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, st));
-    return BP_OK;
-  }
-  // chunks of 2, 2, 4, 8 and then 16 MiB: the first DMA starts after 2 MiB
-  // are staged instead of 16
+// n bytes to dst through the stage ps: fill(pinned, offset, len) writes
+// bytes [offset, offset + len) of the data into a pinned buffer.  Chunks
+// of 2, 2, 4, 8 and then 16 MiB (the first DMA starts after 2 MiB are
+// staged instead of 16); every chunk but the last is a multiple of 2 MiB.
+template <class Fill>
+int upload_pinned(void *dst, size_t n, hipStream_t st, PinnedStage *ps, Fill fill) {
   size_t done = 0;
   for (int i = 0; done < n; ++i) {
     const int j = i & 1;
     const size_t len = std::min({kPinnedBytes, n - done, std::max(kPinnedPiece, done)});
     if (i >= 2) HIP_TRY(hipEventSynchronize(ps->ev[j]));  // buffer j's previous DMA is done
-    copy_bytes(ps->buf[j], static_cast<const char *>(src) + done, len, kPinnedPiece);
+    fill(ps->buf[j], done, len);
     HIP_TRY(hipMemcpyAsync(static_cast<char *>(dst) + done, ps->buf[j], len,
                            hipMemcpyHostToDevice, st));
     HIP_TRY(hipEventRecord(ps->ev[j], st));
@@ -949,6 +946,50 @@ int upload_host(void *dst, const void *src, size_t n, hipStream_t st) {
   }
   HIP_TRY(hipEventSynchronize(ps->ev[0]));
   HIP_TRY(hipEventSynchronize(ps->ev[1]));
+  return BP_OK;
+}
+
+int upload_host(void *dst, const void *src, size_t n, hipStream_t st) {
+  std::unique_lock<std::mutex> lock;
+  PinnedStage *ps = n < 2 * kPinnedBytes ? nullptr : acquire_pinned(lock);
+  if (!ps) {
+    HIP_TRY(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, st));
+    return BP_OK;
+  }
+  return upload_pinned(dst, n, st, ps, [&](void *pin, size_t off, size_t len) {
+    copy_bytes(pin, static_cast<const char *>(src) + off, len, kPinnedPiece);
+  });
+}
+
+// The two phases' delays of each (source, station pair) as one word of two
+// int16 (phase 0 in the low half), each saturated to the int16 range: the
+// words diff_pack_pairs_kernel makes on the device, made here by the host
+// threads that stage the upload, so half the delay bytes cross PCIe.
+// (K, S, 2) int32 -> (K, S) words on the device.  Returns false, having
+// done nothing, when the delays are under 32 MiB or no pinned stage is
+// free (the caller then uploads them as they are).
+inline unsigned sat16_host(int v) {
+  return (unsigned)(v < -32768 ? -32768 : v > 32767 ? 32767 : v) & 0xffffu;
+}
+int upload_packed_pairs(unsigned *dst, const int *tau, size_t n_words, hipStream_t st,
+                        bool &packed) {
+  packed = false;
+  std::unique_lock<std::mutex> lock;
+  PinnedStage *ps = 2 * n_words * sizeof(int) < 2 * kPinnedBytes ? nullptr : acquire_pinned(lock);
+  if (!ps) return BP_OK;
+  BP_TRY(upload_pinned(dst, n_words * sizeof(unsigned), st, ps,
+                       [&](void *pin, size_t off, size_t len) {
+                         unsigned *out = static_cast<unsigned *>(pin);
+                         const int *in = tau + 2 * (off / sizeof(unsigned));
+                         parallel_bytes(
+                             len / sizeof(unsigned),
+                             [&](size_t lo, size_t hi) {
+                               for (size_t i = lo; i < hi; ++i)
+                                 out[i] = sat16_host(in[2 * i]) | (sat16_host(in[2 * i + 1]) << 16);
+                             },
+                             kPinnedPiece / sizeof(unsigned));
+                       }));
+  packed = true;
   return BP_OK;
 }
 
@@ -2327,14 +2368,17 @@ int beamform_differential(float *waveform_features, int *time_delays, float *wei
   const size_t T = n_samples, K = n_sources, S = n_stations, P = n_phases;
   BP_TRY(check_differential(BP_F32, (long long)S, (long long)P, (long long)T));
   DevBuf tau, beta, tauT, betaT, f_stp, f_pl, dbeam;
-  BP_TRY(tau.alloc(K * S * P * sizeof(int)));
+  const bool pairs = diff_pairs_ok((long long)P, (long long)T);
+  BP_TRY(tau.alloc((pairs ? K * S : K * S * P) * sizeof(int)));
   BP_TRY(beta.alloc(K * S * sizeof(float)));
   BP_TRY(f_stp.alloc(S * T * P * sizeof(float)));
   BP_TRY(f_pl.alloc(S * T * P * sizeof(float)));
   BP_TRY(dbeam.alloc(K * sizeof(float)));
   // the delay and weight tables (2.8 GB at 1,225 station pairs x 192,000
   // sources) go up through the two pinned stages at once, the weights on a
-  // second host thread
+  // second host thread; two phases' delays go up as packed int16 pairs
+  // where the kernel reads them so (windows up to 32,768 samples)
+  bool packed = false;
   HIP_TRY(hipMemcpyAsync(f_stp.p, waveform_features, S * T * P * sizeof(float),
                          hipMemcpyHostToDevice, st.s));
   HIP_TRY(hipMemcpyAsync(dbeam.p, beam, K * sizeof(float), hipMemcpyHostToDevice, st.s));
@@ -2354,7 +2398,11 @@ int beamform_differential(float *waveform_features, int *time_delays, float *wei
     });
     int rc = BP_OK;
     try {
-      rc = upload_host(tau.p, time_delays, K * S * P * sizeof(int), st.s);
+      if (pairs) rc = upload_packed_pairs(tau.as<unsigned>(), time_delays, K * S, st.s, packed);
+      if (rc == BP_OK && !packed) {
+        if (pairs) rc = tau.alloc(K * S * P * sizeof(int));
+        if (rc == BP_OK) rc = upload_host(tau.p, time_delays, K * S * P * sizeof(int), st.s);
+      }
     } catch (...) {
       up.join();
       throw;
@@ -2371,18 +2419,21 @@ int beamform_differential(float *waveform_features, int *time_delays, float *wei
   bp::StackArgs a{};
   a.F = f_pl.p;
   a.ldf = (long long)T;
-  a.tau = tau.as<int>();
+  a.tau = packed ? nullptr : tau.as<int>();  // the kernel reads only the tables
   a.beta = beta.as<float>();
   a.T = (int)T;
   a.K = (int)K;
   a.S = (int)S;
   a.P = (int)P;
-  const bool pairs = diff_pairs_ok((long long)P, (long long)T);
   int wform = 0;
   float cw = 0.0f;
   BP_TRY(build_diff_weights(beta.as<float>(), (long long)K, (int)S, (int)P, betaT, wform, cw,
                             st.s));
-  BP_TRY(build_diff_tau(tau.as<int>(), (long long)K, (int)S, (int)P, pairs, tauT, st.s));
+  if (packed)  // the pairs, transposed as build_diff_tau transposes and packs the int32 delays
+    BP_TRY(transpose_diff_table(tau.p, (long long)K, (int)S,
+                                bp::diff_padded_stations((int)S, (int)P), tauT, st.s));
+  else
+    BP_TRY(build_diff_tau(tau.as<int>(), (long long)K, (int)S, (int)P, pairs, tauT, st.s));
   BP_TRY(launch_differential(BP_F32, a, tauT.as<int>(), betaT.as<float>(), wform, cw,
                              dbeam.as<float>(), pairs, st.s));
   HIP_TRY(hipMemcpyAsync(beam, dbeam.p, K * sizeof(float), hipMemcpyDeviceToHost, st.s));

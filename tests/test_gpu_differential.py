@@ -85,6 +85,32 @@ def test_random_shapes_vs_oracle(gpu_lib, seed, S, P, T, K):
     assert bits_equal(got, oracle.beamform_differential(F, tau, beta, init=init))
 
 
+@pytest.mark.parametrize("P,T", [(2, 1001), (2, 32768), (3, 1001)])
+def test_host_call_with_large_tables(gpu_lib, P, T):
+    """Delay tables of 32 MiB or more through the host-pointer call: with
+    two phases and windows up to 32,768 samples the host threads that stage
+    the upload pack the delays into saturated int16 pairs; otherwise the
+    int32 delays go up through the pinned stage.  Delays at and beyond the
+    int16 range (which saturate, and then still point outside the window)
+    are planted.  Bit-identical to the oracle."""
+    K, S = 30_000, 150
+    rng = np.random.default_rng(40 + P + T)
+    F = rng.standard_normal((S, T, P)).astype(np.float32)
+    tau = rng.integers(-(T // 2), T // 2 + 1, size=(K, S, P)).astype(np.int32)
+    plant = rng.random(tau.shape) < 0.01
+    tau[plant] = rng.choice(np.array([-40_000, -32_769, -32_768, -32_767, 32_767, 32_768,
+                                      40_000, 2**31 - 1, -2**31], np.int64),
+                            size=int(plant.sum())).astype(np.int32)
+    beta = rng.uniform(-1.0, 2.0, (K, S)).astype(np.float32)
+    beta[rng.random((K, S)) < 0.1] = 0.0
+    assert tau.nbytes >= 32 << 20
+    init = np.full(K, -3.0, np.float32)
+    got = abi_differential(gpu_lib, F, tau, beta, init)
+    ref = oracle.beamform_differential(F, tau, beta, init=init, num_threads=8)
+    assert bits_equal(got, ref)
+    assert np.sum(got != init) > 100  # sources that stay in range are written
+
+
 def test_c5_device_api_vs_oracle_and_event(gpu_lib):
     """20 stations (190 pairs), 4,000 sources: the device-resident network,
     float32 planar features, bit-exact; the injected source is the argmax."""
