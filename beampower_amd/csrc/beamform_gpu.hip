@@ -993,6 +993,70 @@ int upload_packed_pairs(unsigned *dst, const int *tau, size_t n_words, hipStream
   return BP_OK;
 }
 
+// The used-bit weight table of the differential kernel, built by host
+// threads from (K, S) weights when every nonzero weight has one bit
+// pattern (a weight is nonzero when it compares unequal to 0.0f: -0.0 is
+// zero, NaN is not): the words diff_weight_bits_kernel and diff_mask_kernel
+// make on the device, in their layout -- word w of source k holds bit s % 32
+// for station s = 32 w + (s % 32), at ((k / 64) * W + w) * 64 + k % 64, zero
+// for padded sources and stations.  cw: the weight (1.0 when none is
+// nonzero).  Returns false as soon as two nonzero patterns differ.
+bool host_used_bits(const float *beta, size_t K, size_t S, int W, std::vector<unsigned> &words,
+                    float &cw) {
+  const size_t nblk = (K + 63) / 64;
+  words.resize(nblk * (size_t)W * 64);
+  std::atomic<bool> differ{false};
+  std::mutex mu;
+  bool any = false;
+  unsigned lo = 0u, hi = 0u;
+  parallel_bytes(
+      nblk,
+      [&](size_t b0, size_t b1) {
+        bool tany = false;
+        unsigned tlo = 0u, thi = 0u;
+        for (size_t b = b0; b < b1 && !differ.load(std::memory_order_relaxed); ++b) {
+          for (size_t r = 0; r < 64; ++r) {
+            const size_t k = b * 64 + r;
+            for (int w = 0; w < W; ++w) {
+              unsigned m = 0u;
+              const size_t s0 = (size_t)w * 32;
+              if (k < K && s0 < S) {
+                const float *row = beta + k * S;
+                const size_t s1 = std::min(S, s0 + 32);
+                for (size_t s = s0; s < s1; ++s) {
+                  const float v = row[s];
+                  if (v != 0.0f) {
+                    unsigned u;
+                    std::memcpy(&u, &v, 4);
+                    tlo = tany ? std::min(tlo, u) : u;
+                    thi = tany ? std::max(thi, u) : u;
+                    tany = true;
+                    m |= 1u << (s - s0);
+                  }
+                }
+              }
+              words[(b * (size_t)W + (size_t)w) * 64 + r] = m;
+            }
+          }
+          if (tany && tlo != thi) differ = true;
+        }
+        std::lock_guard<std::mutex> lock(mu);
+        if (tany) {
+          lo = any ? std::min(lo, tlo) : tlo;
+          hi = any ? std::max(hi, thi) : thi;
+          any = true;
+        }
+      },
+      256);
+  if (differ || lo != hi) return false;
+  if (!any) {
+    cw = 1.0f;  // no nonzero weight: the bits are all clear, cw is not applied
+  } else {
+    std::memcpy(&cw, &lo, 4);
+  }
+  return true;
+}
+
 // Device -> host memory the caller owns, after the work queued on `st`:
 // 16 MiB chunks go by DMA into one pinned buffer while host threads copy
 // the previous chunk out of the other (the runtime's pageable path, one
@@ -2370,14 +2434,17 @@ int beamform_differential(float *waveform_features, int *time_delays, float *wei
   DevBuf tau, beta, tauT, betaT, f_stp, f_pl, dbeam;
   const bool pairs = diff_pairs_ok((long long)P, (long long)T);
   BP_TRY(tau.alloc((pairs ? K * S : K * S * P) * sizeof(int)));
-  BP_TRY(beta.alloc(K * S * sizeof(float)));
+  int wform = 0;  // 2: used-bit weights built on the host (betaT holds them)
+  float cw = 0.0f;
+  std::vector<unsigned> bits_h;
   BP_TRY(f_stp.alloc(S * T * P * sizeof(float)));
   BP_TRY(f_pl.alloc(S * T * P * sizeof(float)));
   BP_TRY(dbeam.alloc(K * sizeof(float)));
   // the delay and weight tables (2.8 GB at 1,225 station pairs x 192,000
   // sources) go up through the two pinned stages at once, the weights on a
   // second host thread; two phases' delays go up as packed int16 pairs
-  // where the kernel reads them so (windows up to 32,768 samples)
+  // where the kernel reads them so (windows up to 32,768 samples), and
+  // weights with one nonzero value as their used bits
   bool packed = false;
   HIP_TRY(hipMemcpyAsync(f_stp.p, waveform_features, S * T * P * sizeof(float),
                          hipMemcpyHostToDevice, st.s));
@@ -2388,9 +2455,20 @@ int beamform_differential(float *waveform_features, int *time_delays, float *wei
     std::string up_err;
     std::thread up([&] {
       try {
-        up_rc = hipSetDevice(dev) == hipSuccess  // a new thread starts on device 0
-                    ? upload_host(beta.p, weights, K * S * sizeof(float), st.s)
-                    : fail(BP_EHIP, "hipSetDevice(%d) failed", dev);
+        if (hipSetDevice(dev) != hipSuccess) {  // a new thread starts on device 0
+          up_rc = fail(BP_EHIP, "hipSetDevice(%d) failed", dev);
+        } else if (P >= 1 && P <= 4 && K * S * sizeof(float) >= 2 * kPinnedBytes &&
+                   host_used_bits(weights, K, S, bp::diff_mask_words((int)S, (int)P), bits_h,
+                                  cw)) {
+          // one nonzero weight value: its used bits instead of the weights
+          up_rc = betaT.alloc(bits_h.size() * sizeof(unsigned));
+          if (up_rc == BP_OK)
+            up_rc = upload_host(betaT.p, bits_h.data(), bits_h.size() * sizeof(unsigned), st.s);
+          if (up_rc == BP_OK) wform = 2;
+        } else {
+          up_rc = beta.alloc(K * S * sizeof(float));
+          if (up_rc == BP_OK) up_rc = upload_host(beta.p, weights, K * S * sizeof(float), st.s);
+        }
       } catch (...) {
         up_rc = fail(BP_EHIP, "weight upload failed");
       }
@@ -2425,10 +2503,9 @@ int beamform_differential(float *waveform_features, int *time_delays, float *wei
   a.K = (int)K;
   a.S = (int)S;
   a.P = (int)P;
-  int wform = 0;
-  float cw = 0.0f;
-  BP_TRY(build_diff_weights(beta.as<float>(), (long long)K, (int)S, (int)P, betaT, wform, cw,
-                            st.s));
+  if (wform != 2)
+    BP_TRY(build_diff_weights(beta.as<float>(), (long long)K, (int)S, (int)P, betaT, wform, cw,
+                              st.s));
   if (packed)  // the pairs, transposed as build_diff_tau transposes and packs the int32 delays
     BP_TRY(transpose_diff_table(tau.p, (long long)K, (int)S,
                                 bp::diff_padded_stations((int)S, (int)P), tauT, st.s));

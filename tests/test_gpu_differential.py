@@ -111,6 +111,38 @@ def test_host_call_with_large_tables(gpu_lib, P, T):
     assert np.sum(got != init) > 100  # sources that stay in range are written
 
 
+@pytest.mark.parametrize("kind", ["ones", "mask01", "const", "zeros", "nan", "late_second_value"])
+def test_host_call_with_uniform_weights(gpu_lib, kind):
+    """Weights of 32 MiB or more with one nonzero value go up as the
+    used-bit table that the host threads build (the device's
+    diff_weight_bits_kernel / diff_mask_kernel words); any second value,
+    even in the last row, sends the float32 weights instead.  -0.0 counts
+    as zero, NaN as a value.  Bit-identical to the oracle."""
+    K, S, P, T = 60_000, 150, 2, 1001
+    rng = np.random.default_rng(77)
+    F = rng.standard_normal((S, T, P)).astype(np.float32)
+    tau = rng.integers(-(T // 2), T // 2 + 1, size=(K, S, P)).astype(np.int32)
+    beta = np.ones((K, S), np.float32)
+    if kind == "mask01":
+        beta[rng.random((K, S)) < 0.3] = 0.0
+        beta[rng.random((K, S)) < 0.05] = -0.0
+    elif kind == "const":
+        beta[:] = 0.37
+        beta[rng.random((K, S)) < 0.2] = -0.0
+    elif kind == "zeros":
+        beta[:] = 0.0
+    elif kind == "nan":
+        beta[:] = np.nan
+        beta[rng.random((K, S)) < 0.5] = 0.0
+    elif kind == "late_second_value":
+        beta[-1, -1] = 2.0
+    assert beta.nbytes >= 32 << 20
+    init = np.full(K, -3.0, np.float32)
+    got = abi_differential(gpu_lib, F, tau, beta, init)
+    ref = oracle.beamform_differential(F, tau, beta, init=init, num_threads=8)
+    assert bits_equal(got, ref)
+
+
 def test_c5_device_api_vs_oracle_and_event(gpu_lib):
     """20 stations (190 pairs), 4,000 sources: the device-resident network,
     float32 planar features, bit-exact; the injected source is the argmax."""
