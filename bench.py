@@ -536,6 +536,19 @@ def bench_differential(args, log):
     del net2, F, beam
     torch.cuda.synchronize()
     torch.cuda.empty_cache()
+    # the Python call with host arrays: every table crosses PCIe on each call
+    import beampower_amd as bp
+    call_ms = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        bp.beamform(w.waveform_features, w.time_delays, w.weights_phases, w.weights_sources,
+                    device="gpu", mode="differential", reduce="none")
+        call_ms.append((time.perf_counter() - t0) * 1e3)
+    out["python_call"] = {"ms": min(call_ms), "calls_ms": call_ms,
+                          "note": "beampower_amd.beamform(..., mode='differential') with numpy "
+                                  "inputs: delays packed into int16 pairs and uniform weights "
+                                  "into used bits by the staging host threads, 0.97 GB over PCIe"}
+    log(f"differential python call: {min(call_ms):.1f} ms")
     if not args.no_cpu_baseline:
         # the reference engine single-threaded (its per-source bounds are
         # shared between OpenMP threads), whole calls repeated for about
