@@ -166,49 +166,12 @@ struct DevBuf {
   }
 };
 
-// Idle streams of finished calls, kept for later calls on the same device:
-// creating a stream took 0.1-1.6 ms per call of the host-pointer entry
-// points.  Never destroyed (no device call at exit).
-struct StreamPool {
-  std::mutex mu;
-  std::vector<std::pair<int, hipStream_t>> idle;  // (device, stream)
-};
-StreamPool &stream_pool() {
-  static StreamPool *p = new StreamPool;
-  return *p;
-}
-constexpr size_t kMaxIdleStreams = 16;
-
-// A call's own non-blocking stream (default priority), taken from the pool
-// or created; returned to the pool when the call is done with it and it is
-// idle, else destroyed (an error path may leave work queued).
 struct Stream {
   hipStream_t s = nullptr;
-  int dev = -1;
   ~Stream() {
-    if (!s) return;
-    if (hipStreamQuery(s) == hipSuccess) {
-      StreamPool &pool = stream_pool();
-      std::lock_guard<std::mutex> lock(pool.mu);
-      if (pool.idle.size() < kMaxIdleStreams) {
-        pool.idle.emplace_back(dev, s);
-        return;
-      }
-    }
-    (void)hipStreamDestroy(s);
+    if (s) (void)hipStreamDestroy(s);
   }
   int create() {
-    HIP_TRY(hipGetDevice(&dev));
-    {
-      StreamPool &pool = stream_pool();
-      std::lock_guard<std::mutex> lock(pool.mu);
-      for (size_t i = pool.idle.size(); i-- > 0;) {
-        if (pool.idle[i].first != dev) continue;
-        s = pool.idle[i].second;
-        pool.idle.erase(pool.idle.begin() + (std::ptrdiff_t)i);
-        return BP_OK;
-      }
-    }
     HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     return BP_OK;
   }
