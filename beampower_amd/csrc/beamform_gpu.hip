@@ -894,6 +894,18 @@ PinnedStage *acquire_pinned(std::unique_lock<std::mutex> &lock) {
 // bytes [offset, offset + len) of the data into a pinned buffer.  Chunks
 // of 2, 2, 4, 8 and then 16 MiB (the first DMA starts after 2 MiB are
 // staged instead of 16); every chunk but the last is a multiple of 2 MiB.
+// Declared after the stage's lock, so that it runs first: on an early
+// return (an error, an exception of a host thread) DMAs of the stage's
+// buffers may still be queued on the stream, and they must end before the
+// stage goes to its next user.
+struct DrainOnExit {
+  hipStream_t st;
+  bool armed = true;
+  ~DrainOnExit() {
+    if (armed) (void)hipStreamSynchronize(st);
+  }
+};
+
 template <class Fill>
 int upload_pinned(void *dst, size_t n, hipStream_t st, PinnedStage *ps, Fill fill) {
   size_t done = 0;
@@ -919,9 +931,12 @@ int upload_host(void *dst, const void *src, size_t n, hipStream_t st) {
     HIP_TRY(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, st));
     return BP_OK;
   }
-  return upload_pinned(dst, n, st, ps, [&](void *pin, size_t off, size_t len) {
+  DrainOnExit drain{st};
+  const int rc = upload_pinned(dst, n, st, ps, [&](void *pin, size_t off, size_t len) {
     copy_bytes(pin, static_cast<const char *>(src) + off, len, kPinnedPiece);
   });
+  drain.armed = rc != BP_OK;
+  return rc;
 }
 
 // The two phases' delays of each (source, station pair) as one word of two
@@ -940,6 +955,7 @@ int upload_packed_pairs(unsigned *dst, const int *tau, size_t n_words, hipStream
   std::unique_lock<std::mutex> lock;
   PinnedStage *ps = 2 * n_words * sizeof(int) < 2 * kPinnedBytes ? nullptr : acquire_pinned(lock);
   if (!ps) return BP_OK;
+  DrainOnExit drain{st};
   BP_TRY(upload_pinned(dst, n_words * sizeof(unsigned), st, ps,
                        [&](void *pin, size_t off, size_t len) {
                          unsigned *out = static_cast<unsigned *>(pin);
@@ -952,6 +968,7 @@ int upload_packed_pairs(unsigned *dst, const int *tau, size_t n_words, hipStream
                              },
                              kPinnedPiece / sizeof(unsigned));
                        }));
+  drain.armed = false;
   packed = true;
   return BP_OK;
 }
@@ -1033,6 +1050,7 @@ int download_host(void *dst, const void *src, size_t n, hipStream_t st) {
     HIP_TRY(hipStreamSynchronize(st));
     return BP_OK;
   }
+  DrainOnExit drain{st};
   size_t prev_off = 0, prev_len = 0;
   for (size_t done = 0, i = 0; done < n || prev_len; ++i) {
     const int j = (int)(i & 1);
@@ -1052,6 +1070,7 @@ int download_host(void *dst, const void *src, size_t n, hipStream_t st) {
     prev_len = len;
     done += len;
   }
+  drain.armed = false;
   return BP_OK;
 }
 
