@@ -981,10 +981,11 @@ int upload_packed_pairs(unsigned *dst, const int *tau, size_t n_words, hipStream
 // for station s = 32 w + (s % 32), at ((k / 64) * W + w) * 64 + k % 64, zero
 // for padded sources and stations.  cw: the weight (1.0 when none is
 // nonzero).  Returns false as soon as two nonzero patterns differ.
-bool host_used_bits(const float *beta, size_t K, size_t S, int W, std::vector<unsigned> &words,
-                    float &cw) {
+bool host_used_bits(const float *beta, size_t K, size_t S, int W,
+                    std::unique_ptr<unsigned[]> &words, size_t &n_words, float &cw) {
   const size_t nblk = (K + 63) / 64;
-  words.resize(nblk * (size_t)W * 64);
+  n_words = nblk * (size_t)W * 64;
+  words.reset(new unsigned[n_words]);  // every word is written below (not zeroed first)
   std::atomic<bool> differ{false};
   std::mutex mu;
   bool any = false;
@@ -2418,7 +2419,8 @@ int beamform_differential(float *waveform_features, int *time_delays, float *wei
   BP_TRY(tau.alloc((pairs ? K * S : K * S * P) * sizeof(int)));
   int wform = 0;  // 2: used-bit weights built on the host (betaT holds them)
   float cw = 0.0f;
-  std::vector<unsigned> bits_h;
+  std::unique_ptr<unsigned[]> bits_h;
+  size_t n_bits = 0;
   BP_TRY(f_stp.alloc(S * T * P * sizeof(float)));
   BP_TRY(f_pl.alloc(S * T * P * sizeof(float)));
   BP_TRY(dbeam.alloc(K * sizeof(float)));
@@ -2441,11 +2443,11 @@ int beamform_differential(float *waveform_features, int *time_delays, float *wei
           up_rc = fail(BP_EHIP, "hipSetDevice(%d) failed", dev);
         } else if (P >= 1 && P <= 4 && K * S * sizeof(float) >= 2 * kPinnedBytes &&
                    host_used_bits(weights, K, S, bp::diff_mask_words((int)S, (int)P), bits_h,
-                                  cw)) {
+                                  n_bits, cw)) {
           // one nonzero weight value: its used bits instead of the weights
-          up_rc = betaT.alloc(bits_h.size() * sizeof(unsigned));
+          up_rc = betaT.alloc(n_bits * sizeof(unsigned));
           if (up_rc == BP_OK)
-            up_rc = upload_host(betaT.p, bits_h.data(), bits_h.size() * sizeof(unsigned), st.s);
+            up_rc = upload_host(betaT.p, bits_h.get(), n_bits * sizeof(unsigned), st.s);
           if (up_rc == BP_OK) wform = 2;
         } else {
           up_rc = beta.alloc(K * S * sizeof(float));
