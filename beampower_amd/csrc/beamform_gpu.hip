@@ -177,14 +177,21 @@ struct Stream {
   }
 };
 
+// The current device is a gfx950.  The answer is kept per device: reading
+// the properties cost about a millisecond per host-pointer call.
 int check_device() {
+  constexpr int kMaxDevices = 64;
+  static std::atomic<int> known[kMaxDevices];  // 0 unknown, 1 gfx950
   int dev = 0;
   HIP_TRY(hipGetDevice(&dev));
+  if (dev >= 0 && dev < kMaxDevices && known[dev].load(std::memory_order_acquire) == 1)
+    return BP_OK;
   hipDeviceProp_t prop;
   HIP_TRY(hipGetDeviceProperties(&prop, dev));
   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
     return fail(BP_ENODEV, "device %d is %s; this library is built for gfx950", dev,
                 prop.gcnArchName);
+  if (dev >= 0 && dev < kMaxDevices) known[dev].store(1, std::memory_order_release);
   return BP_OK;
 }
 
