@@ -597,6 +597,23 @@ __device__ __forceinline__ void block_tile(int L, int n_time_tiles, int n_list, 
 #ifndef BP_PIPE_DEPTH
 #define BP_PIPE_DEPTH 1
 #endif
+// Where in a step's products the next step's LDS reads are issued, in
+// 16-byte words of the step (FP32: 8 words, read 0 and read 1; 16-bit: 4
+// words, one read): a wave then alternates products and reads instead of
+// issuing its reads in one burst before its products.
+// Measured on the drop-in workload (HIP events, builds interleaved in one
+// process): FP32 reads before words 3 and 5 of 8 run 1-2 % faster than both
+// reads before the products (0, 0); 16-bit samples are fastest with the
+// read first.
+#ifndef BP_GATHER_AT_F32_0
+#define BP_GATHER_AT_F32_0 3
+#endif
+#ifndef BP_GATHER_AT_F32_1
+#define BP_GATHER_AT_F32_1 5
+#endif
+#ifndef BP_GATHER_AT_16
+#define BP_GATHER_AT_16 0
+#endif
 
 // Work-group barrier without the memory-model fence of __syncthreads(),
 // which waits for every outstanding global->LDS copy of the wave (the
@@ -675,12 +692,15 @@ struct Fmt;
 template <>
 struct Fmt<BP_F32> {
   static constexpr int ES = 4, SPA = 4;
+  // word q (0..3) of a 16-byte LDS word: sample q
+  __device__ __forceinline__ static void mac_word(float *acc, unsigned w, int q, int bw) {
+    acc[q] = fmaf(__int_as_float(bw), __uint_as_float(w), acc[q]);
+  }
   __device__ __forceinline__ static void mac(float *acc, const uint4 &v, int bw) {
-    const float b = __int_as_float(bw);
-    acc[0] = fmaf(b, __uint_as_float(v.x), acc[0]);
-    acc[1] = fmaf(b, __uint_as_float(v.y), acc[1]);
-    acc[2] = fmaf(b, __uint_as_float(v.z), acc[2]);
-    acc[3] = fmaf(b, __uint_as_float(v.w), acc[3]);
+    mac_word(acc, v.x, 0, bw);
+    mac_word(acc, v.y, 1, bw);
+    mac_word(acc, v.z, 2, bw);
+    mac_word(acc, v.w, 3, bw);
   }
 };
 
@@ -693,30 +713,36 @@ struct Fmt<BP_F16> {
   __device__ __forceinline__ static float hi(unsigned w) {
     return (float)__builtin_bit_cast(_Float16, (unsigned short)(w >> 16));
   }
-  __device__ __forceinline__ static void mac(float *acc, const uint4 &v, int bw) {
+  // word q (0..3) of a 16-byte LDS word: samples 2q and 2q + 1
+  __device__ __forceinline__ static void mac_word(float *acc, unsigned w, int q, int bw) {
     const float b = __int_as_float(bw);
-    const unsigned w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      acc[2 * i] = fmaf(lo(w[i]), b, acc[2 * i]);
-      acc[2 * i + 1] = fmaf(hi(w[i]), b, acc[2 * i + 1]);
-    }
+    acc[2 * q] = fmaf(lo(w), b, acc[2 * q]);
+    acc[2 * q + 1] = fmaf(hi(w), b, acc[2 * q + 1]);
+  }
+  __device__ __forceinline__ static void mac(float *acc, const uint4 &v, int bw) {
+    mac_word(acc, v.x, 0, bw);
+    mac_word(acc, v.y, 1, bw);
+    mac_word(acc, v.z, 2, bw);
+    mac_word(acc, v.w, 3, bw);
   }
 };
 
 template <>
 struct Fmt<BP_BF16> {
   static constexpr int ES = 2, SPA = 8;
-  __device__ __forceinline__ static void mac(float *acc, const uint4 &v, int bw) {
+  // word q (0..3) of a 16-byte LDS word: samples 2q and 2q + 1
+  __device__ __forceinline__ static void mac_word(float *acc, unsigned w, int q, int bw) {
     const bf16x2_t whi = __builtin_bit_cast(bf16x2_t, (unsigned)bw & 0xffff0000u);  // (0, w)
     const bf16x2_t wlo = __builtin_bit_cast(bf16x2_t, (unsigned)bw >> 16);          // (w, 0)
-    const unsigned w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const bf16x2_t x = __builtin_bit_cast(bf16x2_t, w[i]);
-      acc[2 * i] = __builtin_amdgcn_fdot2_f32_bf16(x, wlo, acc[2 * i], false);
-      acc[2 * i + 1] = __builtin_amdgcn_fdot2_f32_bf16(x, whi, acc[2 * i + 1], false);
-    }
+    const bf16x2_t x = __builtin_bit_cast(bf16x2_t, w);
+    acc[2 * q] = __builtin_amdgcn_fdot2_f32_bf16(x, wlo, acc[2 * q], false);
+    acc[2 * q + 1] = __builtin_amdgcn_fdot2_f32_bf16(x, whi, acc[2 * q + 1], false);
+  }
+  __device__ __forceinline__ static void mac(float *acc, const uint4 &v, int bw) {
+    mac_word(acc, v.x, 0, bw);
+    mac_word(acc, v.y, 1, bw);
+    mac_word(acc, v.z, 2, bw);
+    mac_word(acc, v.w, 3, bw);
   }
 };
 
@@ -929,12 +955,13 @@ struct StackLds {
   // address `addr` (an absolute LDS address, so that no base is added).
   typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
   typedef const __attribute__((address_space(3))) u32x4 lds_u32x4;
+  __device__ __forceinline__ static void gather_one(unsigned addr, int r, uint4 (&v)[R]) {
+    const u32x4 w = *(lds_u32x4 *)(size_t)(addr + 1024u * r);
+    v[r] = make_uint4(w.x, w.y, w.z, w.w);
+  }
   __device__ __forceinline__ static void gather(unsigned addr, uint4 (&v)[R]) {
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const u32x4 w = *(lds_u32x4 *)(size_t)(addr + 1024u * r);
-      v[r] = make_uint4(w.x, w.y, w.z, w.w);
-    }
+    for (int r = 0; r < R; ++r) gather_one(addr, r, v);
   }
   __device__ __forceinline__ static unsigned lds_address(const unsigned char *p) {
     return (unsigned)(size_t)(const __attribute__((address_space(3))) unsigned char *)p;
@@ -963,6 +990,11 @@ struct StackLds {
   // boff: byte offset of the station's staging buffer in the staging area.
   static constexpr int STEPS = KW * P;
   static constexpr int DEPTH = BP_PIPE_DEPTH < STEPS ? BP_PIPE_DEPTH : STEPS - 1;
+  // where in a step's 4R words the next step's reads are issued: 32-bit
+  // samples read 1 KiB twice per step, 16-bit samples once
+  __host__ __device__ static constexpr int gather_at(int r) {
+    return ES == 4 ? (r == 0 ? BP_GATHER_AT_F32_0 : BP_GATHER_AT_F32_1) : BP_GATHER_AT_16;
+  }
 
   __device__ __forceinline__ static void consume_station(const int (&sv)[KW * (P + 1)],
                                                          const unsigned char *stage,
@@ -991,10 +1023,26 @@ struct StackLds {
     for (int i = 0; i < STEPS; ++i) {
       const int kk = i / P;
       const int n = i + DEPTH;
-      if (n < STEPS) gather(step_addr(n), v[n % (DEPTH + 1)]);
       const int bw = sv[STEPS + kk];
+      // the step's 4R words in order (each word is one or two products),
+      // with read r of step n's gather issued before word gather_at(r)
+      const uint4(&x)[R] = v[i % (DEPTH + 1)];
+      unsigned an = 0;
 #pragma unroll
-      for (int r = 0; r < R; ++r) F::mac(&acc[kk][SPA * r], v[i % (DEPTH + 1)][r], bw);
+      for (int u = 0; u < 4 * R; ++u) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          if (u == gather_at(r) && n < STEPS) {
+            __builtin_amdgcn_sched_barrier(0);
+            if (r == 0) an = step_addr(n);
+            gather_one(an, r, v[n % (DEPTH + 1)]);
+            __builtin_amdgcn_sched_barrier(0);
+          }
+        }
+        const uint4 &w = x[u / 4];
+        const unsigned word = u % 4 == 0 ? w.x : u % 4 == 1 ? w.y : u % 4 == 2 ? w.z : w.w;
+        F::mac_word(&acc[kk][SPA * (u / 4)], word, u % 4, bw);
+      }
 #pragma unroll
       for (int j = 0; j < NACC; ++j) asm volatile("" : "+v"(acc[kk][j]));
       asm volatile("" : "+v"(lb2));
