@@ -36,7 +36,8 @@ FLAGS_PATH = os.path.join(LIB_DIR, "build_flags.json")
 # macros of the kernel shape the tuner may set (bp_kernels.h, beamform_gpu.hip)
 SHAPE_MACROS = ("BP_KW", "BP_W", "BP_WAVES_PER_EU", "BP_PIPE_DEPTH", "BP_STAGE_BUFFERS",
                 "BP_TIME_GROUP", "BP_NACC", "BP_SMALL_WINDOW", "BP_SMALL_SHARE_PCT",
-                "BP_GATHER_AT_F32_0", "BP_GATHER_AT_F32_1", "BP_GATHER_AT_16")
+                "BP_GATHER_AT_F32_0", "BP_GATHER_AT_F32_1", "BP_GATHER_AT_16",
+                "BP_PIPE_DEPTH_16")
 
 
 def shape_flags(macros):

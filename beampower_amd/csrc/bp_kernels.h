@@ -591,11 +591,17 @@ __device__ __forceinline__ void block_tile(int L, int n_time_tiles, int n_list, 
   tt = g * TG + (rem - li * gsize);
 }
 
-// (source, phase) gather steps in flight.  One keeps the registers within
-// the 6-waves-per-SIMD budget below (two spill there, three ran 1.5 % slower
-// at 4 waves per SIMD).
+// (source, phase) gather steps in flight for FP32 samples.  One keeps the
+// registers within the 6-waves-per-SIMD budget below (two spill there,
+// three ran 1.5 % slower at 4 waves per SIMD).
 #ifndef BP_PIPE_DEPTH
 #define BP_PIPE_DEPTH 1
+#endif
+// 16-bit samples: one 16-byte read per step, so two steps ahead fit the
+// registers (1.3 % faster than one step ahead on the drop-in workload in
+// BF16; two steps ahead made FP32 4 % slower).
+#ifndef BP_PIPE_DEPTH_16
+#define BP_PIPE_DEPTH_16 2
 #endif
 // Where in a step's products the next step's LDS reads are issued, in
 // 16-byte words of the step (FP32: 8 words, read 0 and read 1; 16-bit: 4
@@ -989,7 +995,8 @@ struct StackLds {
   // the lane's LDS offset (later gathers depend on it, so they stay behind).
   // boff: byte offset of the station's staging buffer in the staging area.
   static constexpr int STEPS = KW * P;
-  static constexpr int DEPTH = BP_PIPE_DEPTH < STEPS ? BP_PIPE_DEPTH : STEPS - 1;
+  static constexpr int DEPTH_FMT = ES == 4 ? BP_PIPE_DEPTH : BP_PIPE_DEPTH_16;
+  static constexpr int DEPTH = DEPTH_FMT < STEPS ? DEPTH_FMT : STEPS - 1;
   // where in a step's 4R words the next step's reads are issued: 32-bit
   // samples read 1 KiB twice per step, 16-bit samples once
   __host__ __device__ static constexpr int gather_at(int r) {
