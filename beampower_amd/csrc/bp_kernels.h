@@ -603,6 +603,16 @@ __device__ __forceinline__ void block_tile(int L, int n_time_tiles, int n_list, 
 #ifndef BP_PIPE_DEPTH_16
 #define BP_PIPE_DEPTH_16 2
 #endif
+// Steps (of a station's KW*P gather steps) at which the copies of a later
+// station's phase p are issued: BP_DMA_AT + p * BP_DMA_GAP.  All phases at
+// step 4 measured 1.2 % faster than at step 0 (FP32, drop-in workload);
+// spreading the phases over the station was slower.
+#ifndef BP_DMA_AT
+#define BP_DMA_AT 4
+#endif
+#ifndef BP_DMA_GAP
+#define BP_DMA_GAP 0
+#endif
 // Where in a step's products the next step's LDS reads are issued, in
 // 16-byte words of the step (FP32: 8 words, read 0 and read 1; 16-bit: 4
 // words, one read): a wave then alternates products and reads instead of
@@ -892,19 +902,28 @@ struct StackLds {
   // global load to LDS, the compiler then keeps its waits for the gathers
   // exact (lgkmcnt(n)) while copies are in flight instead of draining all
   // outstanding LDS reads at every wait.
+  __device__ __forceinline__ static void dma_phase(const StationSrc &src, const RowPlan &rp,
+                                                   int SB, int wave, int p,
+                                                   unsigned char *__restrict__ buf) {
+#pragma unroll
+    for (int i = 0; i < NRMAX; ++i) {
+      if (i >= rp.n) break;
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(
+          src.rs, (lds_void *)(buf + p * SB + 1024 * (wave + i * W)), 16, rp.voff[i],
+          src.soff[p], 0, 0);
+    }
+  }
   __device__ __forceinline__ static void dma_station(const StationSrc &src, const RowPlan &rp,
                                                      int SB, int wave,
                                                      unsigned char *__restrict__ buf) {
 #pragma unroll
-    for (int p = 0; p < P; ++p) {
-#pragma unroll
-      for (int i = 0; i < NRMAX; ++i) {
-        if (i >= rp.n) break;
-        __builtin_amdgcn_raw_ptr_buffer_load_lds(
-            src.rs, (lds_void *)(buf + p * SB + 1024 * (wave + i * W)), 16, rp.voff[i],
-            src.soff[p], 0, 0);
-      }
-    }
+    for (int p = 0; p < P; ++p) dma_phase(src, rp, SB, wave, p, buf);
+  }
+  // Step of a station's gathers at which the copies of phase p of a later
+  // station are issued: spread through the station rather than in one burst
+  // before it.
+  __host__ __device__ static constexpr int dma_step(int p) {
+    return BP_DMA_AT + p * BP_DMA_GAP < STEPS ? BP_DMA_AT + p * BP_DMA_GAP : STEPS - 1;
   }
 
   // Edge tiles (a window crosses [0, T)): element-wise guarded copies
@@ -1003,10 +1022,12 @@ struct StackLds {
     return ES == 4 ? (r == 0 ? BP_GATHER_AT_F32_0 : BP_GATHER_AT_F32_1) : BP_GATHER_AT_16;
   }
 
+  // hook(i) runs at the start of step i (the copies of a later station).
+  template <class Hook>
   __device__ __forceinline__ static void consume_station(const int (&sv)[KW * (P + 1)],
                                                          const unsigned char *stage,
                                                          unsigned boff, int lane,
-                                                         float (&acc)[KW][NACC]) {
+                                                         float (&acc)[KW][NACC], Hook hook) {
     const unsigned lb = 16u * (unsigned)lane;
     unsigned long long lb2 = ((unsigned long long)lb << 32) | lb;
     const unsigned base = lds_address(stage) + boff;
@@ -1028,6 +1049,7 @@ struct StackLds {
     for (int i = 0; i < DEPTH; ++i) gather(step_addr(i), v[i % (DEPTH + 1)]);
 #pragma unroll
     for (int i = 0; i < STEPS; ++i) {
+      hook(i);
       const int kk = i / P;
       const int n = i + DEPTH;
       const int bw = sv[STEPS + kk];
@@ -1069,8 +1091,19 @@ struct StackLds {
                                                       unsigned boff,
                                                       unsigned char *__restrict__ nxt,
                                                       int lane, float (&acc)[KW][NACC]) {
-    if (dma) dma_station(src, rp, SB, wave, nxt);
-    consume_station(sv, stage, boff, lane, acc);
+    consume_station(sv, stage, boff, lane, acc, [&](int i) {
+      bool here = false;  // i is a constant of the unrolled steps
+#pragma unroll
+      for (int p = 0; p < P; ++p) here = here || i == dma_step(p);
+      if (!here) return;
+      __builtin_amdgcn_sched_barrier(0);
+      if (dma) {
+#pragma unroll
+        for (int p = 0; p < P; ++p)
+          if (i == dma_step(p)) dma_phase(src, rp, SB, wave, p, nxt);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    });
   }
 
   // Sample of accumulator j of a lane: run j / SPA, position j % SPA.
@@ -1125,7 +1158,7 @@ struct StackLds {
         load_sched(sched_w + (size_t)s * KT * (P + 1), sv);
         stage_station_guarded(a, tlo, s, t0, tid, stage);
         __syncthreads();
-        consume_station(sv, stage, 0u, lane, acc);
+        consume_station(sv, stage, 0u, lane, acc, [](int) {});
         __syncthreads();
       }
     } else {
