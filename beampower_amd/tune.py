@@ -2,7 +2,7 @@
 
     python -m beampower_amd.tune [--stations 50] [--grid 80 80 30] [--box-km 100]
                                  [--samples 90000] [--fmt f32|bf16|f16] [--reps 6] [--apply]
-                                 [--set shapes|tiles] [--build-dir DIR]
+                                 [--set shapes|tiles|schedule] [--build-dir DIR]
                                  [--build-only | --prebuilt]
 
 The shift-and-stack kernel keeps a tile's accumulators in registers, so its
@@ -76,7 +76,16 @@ TILE_CANDIDATES = (
     {"BP_KW": 3, "BP_W": 16, "BP_NACC": 16, "BP_WAVES_PER_EU": 4, "BP_PIPE_DEPTH": 1},  # 48 x 1024
     {"BP_KW": 4, "BP_W": 16, "BP_NACC": 16, "BP_WAVES_PER_EU": 4, "BP_PIPE_DEPTH": 1},  # 64 x 1024
 )
-CANDIDATE_SETS = {"shapes": CANDIDATES, "tiles": TILE_CANDIDATES}
+# Where each wave issues its LDS reads and its global->LDS copies among its
+# products (the order of one instruction stream, results unchanged): the
+# FP32 reads of the next step before product words a and b of a step's 8,
+# and a later station's copies at step c of the station's gather steps.
+SCHEDULE_CANDIDATES = tuple(
+    {"BP_GATHER_AT_F32_0": a, "BP_GATHER_AT_F32_1": b, "BP_DMA_AT": c}
+    for a, b, c in ((3, 5, 4), (0, 0, 0), (4, 4, 4), (3, 5, 0), (3, 5, 6), (2, 6, 4), (4, 6, 4),
+                    (3, 4, 4)))
+CANDIDATE_SETS = {"shapes": CANDIDATES, "tiles": TILE_CANDIDATES,
+                  "schedule": SCHEDULE_CANDIDATES}
 FORMATS = {"f32": 0, "bf16": 1, "f16": 2}
 
 
@@ -192,7 +201,8 @@ def main(argv=None):
     ap.add_argument("--reps", type=int, default=6)
     ap.add_argument("--jobs", type=int, default=4, help="parallel builds")
     ap.add_argument("--set", choices=sorted(CANDIDATE_SETS), default="shapes",
-                    help="candidate set: kernel shapes around the default, or source/time tiles")
+                    help="candidate set: kernel shapes around the default, source/time tiles, "
+                         "or where the reads and copies go among the products")
     ap.add_argument("--build-dir", default=None,
                     help="keep the candidate builds in this directory")
     ap.add_argument("--build-only", action="store_true",

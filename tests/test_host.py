@@ -259,7 +259,8 @@ def test_tuner_candidates_and_build_flags():
         keys = [tuple(sorted(c.items())) for c in cands]
         assert len(set(keys)) == len(keys)
         for c in cands:  # register-resident accumulators, at most 16 waves per work-group
-            kw, nacc, w = c.get("BP_KW"), c.get("BP_NACC", defaults["BP_NACC"]), c.get("BP_W", 8)
+            kw, nacc = c.get("BP_KW", defaults["BP_KW"]), c.get("BP_NACC", defaults["BP_NACC"])
+            w = c.get("BP_W", 8)
             assert kw * nacc <= 64 and w <= 16, c
     assert tune.name_of({"BP_KW": 6, "BP_W": 12}) == "kw=6,w=12"
     flags = bp_build.shape_flags({"BP_KW": 5, "BP_PIPE_DEPTH": 2})
