@@ -251,7 +251,7 @@ def test_tuner_candidates_and_build_flags():
             src += fh.read()
     for m in bp_build.SHAPE_MACROS:
         assert f"#ifndef {m}\n#define {m} " in src, m  # every tunable macro has a default
-    defaults = {m: int(v) for m, v in re.findall(r"#define (BP_[A-Z_]+) (\d+)\b", src)
+    defaults = {m: int(v) for m, v in re.findall(r"#define (BP_[A-Z0-9_]+) (\d+)\b", src)
                 if m in bp_build.SHAPE_MACROS}
     for cands in tune.CANDIDATE_SETS.values():
         first = cands[0]
