@@ -21,7 +21,7 @@ LIB_DIR = os.path.join(PKG_DIR, "lib")
 LIB_PATH = os.path.join(LIB_DIR, "beamform_gpu.so")
 SOURCES = [os.path.join(CSRC, "beamform_gpu.hip"), os.path.join(CSRC, "bp_server.hip")]
 HEADERS = [os.path.join(CSRC, f) for f in ("bp_device.h", "bp_internal.h", "bp_kernels.h",
-                                            "bp_order.h")] + [
+                                            "bp_order.h", "bp_host_pool.h")] + [
     os.path.join(REPO_DIR, "include", "beamform_gpu.h")]
 
 

@@ -30,6 +30,7 @@
 #include "beamform_gpu.h"
 #include "bp_internal.h"
 #include "bp_kernels.h"
+#include "bp_host_pool.h"
 #include "bp_order.h"
 
 #define BP_VERSION "0.1.0"
@@ -801,25 +802,20 @@ int lds_bytes_of(int fmt, int P, bool none, int Lw) {
 }
 
 // Runs f(begin, end) over [0, n) bytes split across up to 16 host threads
-// (pieces of at least `piece` bytes; small inputs stay on the calling
-// thread).
+// of the library's worker pool (pieces of at least `piece` bytes; small
+// inputs stay on the calling thread).
 template <class F>
 void parallel_bytes(size_t n, F f, size_t piece = (size_t)8 << 20) {
-  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-  const size_t nt = std::min<size_t>({(size_t)16, (size_t)hw, (n + piece - 1) / piece});
+  const size_t nt = std::min<size_t>(16, (n + piece - 1) / piece);
   if (nt <= 1) {
     f((size_t)0, n);
     return;
   }
   const size_t step = (n + nt - 1) / nt;
-  std::vector<std::thread> th;
-  th.reserve(nt - 1);
-  for (size_t i = 1; i < nt; ++i) {
+  bp::HostPool::shared().run(nt, [&](size_t i) {
     const size_t b = std::min(n, i * step), e = std::min(n, b + step);
-    th.emplace_back([=] { f(b, e); });
-  }
-  f((size_t)0, std::min(n, step));
-  for (auto &x : th) x.join();
+    if (b < e) f(b, e);
+  });
 }
 
 bool same_bytes(const void *a, const void *b, size_t n) {

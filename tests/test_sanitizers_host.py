@@ -5,6 +5,11 @@
   -fsanitize=address,undefined orders shuffled and degenerate delay tables
   on 1 to 16 threads and checks that every thread count gives the
   one-thread permutation.
+* beampower_amd/csrc/bp_host_pool.h (the worker threads behind the
+  library's host copies and comparisons): a driver built with each of the
+  two sanitizer sets runs jobs from eight caller threads at once, nested
+  jobs, jobs whose tasks throw, and pools of no and one worker, and checks
+  that every task ran exactly once.
 * oracle/beamform_oracle.c (OpenMP): a driver built with
   -fsanitize=address,undefined runs every oracle entry point (prestack,
   beams, maxima over the whole window and a sample range, differential mode,
@@ -66,6 +71,61 @@ int main() {
   const std::vector<int> none = bp::similarity_order(tau.data(), 100, 0, 48, 4);
   if (small.size() != 1 || none.size() != 100) return 5;
   std::puts("order ok");
+  return 0;
+}
+'''
+
+POOL_DRIVER = r'''
+#include "bp_host_pool.h"
+#include <cstdio>
+#include <stdexcept>
+#include <thread>
+#include <vector>
+
+static int check(bp::HostPool &pool, size_t n, int rounds) {
+  for (int r = 0; r < rounds; ++r) {
+    std::vector<std::atomic<int>> hits(n);
+    for (auto &h : hits) h.store(0);
+    pool.run(n, [&](size_t i) { hits[i].fetch_add(1); });
+    for (size_t i = 0; i < n; ++i)
+      if (hits[i].load() != 1) return 1;
+  }
+  return 0;
+}
+
+int main() {
+  bp::HostPool &pool = bp::HostPool::shared();
+  int bad = check(pool, 0, 2) + check(pool, 1, 3) + check(pool, 37, 20);
+  // eight callers at once, each with its own jobs
+  std::vector<std::thread> callers;
+  std::atomic<int> caller_bad{0};
+  for (int c = 0; c < 8; ++c)
+    callers.emplace_back([&, c] { caller_bad += check(pool, 5 + 3 * c, 30); });
+  for (auto &t : callers) t.join();
+  bad += caller_bad.load();
+  // nested: tasks that run jobs of their own on the same pool
+  std::atomic<long> total{0};
+  pool.run(6, [&](size_t i) {
+    pool.run(10, [&](size_t j) { total += (long)(i * 10 + j); });
+  });
+  if (total.load() != 1770) bad += 1;
+  // a throwing task: the others still run, the first exception reaches the caller
+  std::atomic<int> ran{0};
+  bool caught = false;
+  try {
+    pool.run(12, [&](size_t i) {
+      ran += 1;
+      if (i % 5 == 3) throw std::runtime_error("task failed");
+    });
+  } catch (const std::runtime_error &) {
+    caught = true;
+  }
+  if (!caught || ran.load() != 12) bad += 1;
+  // pools of no and one worker
+  bp::HostPool none(0), one(1);
+  bad += check(none, 9, 3) + check(one, 9, 30);
+  if (bad) { std::printf("pool failures: %d\n", bad); return 2; }
+  std::puts("pool ok");
   return 0;
 }
 '''
@@ -177,6 +237,19 @@ def test_similarity_order_under_sanitizers(tmp_path, san):
     out = _build_and_run(tmp_path, "order_driver.cpp", ORDER_DRIVER, "g++", flags,
                          {"TSAN_OPTIONS": "halt_on_error=1", "ASAN_OPTIONS": "halt_on_error=1"})
     assert "order ok" in out
+
+
+@pytest.mark.parametrize("san", ["thread", "address,undefined"])
+def test_host_pool_under_sanitizers(tmp_path, san):
+    if shutil.which("g++") is None:
+        pytest.skip("g++ is not available")
+    flags = ["-O1", "-g", "-std=c++17", f"-fsanitize={san}", "-fno-omit-frame-pointer",
+             "-pthread", "-I", CSRC]
+    if "undefined" in san:
+        flags.append("-fno-sanitize-recover=undefined")
+    out = _build_and_run(tmp_path, "pool_driver.cpp", POOL_DRIVER, "g++", flags,
+                         {"TSAN_OPTIONS": "halt_on_error=1", "ASAN_OPTIONS": "halt_on_error=1"})
+    assert "pool ok" in out
 
 
 def test_oracle_under_address_and_undefined_sanitizers(tmp_path):
