@@ -1023,13 +1023,16 @@ struct StackLds {
   }
 
   // hook(i) runs at the start of step i (the copies of a later station).
+  // lb2: the lane's byte offset (16 * lane) in both halves; the caller
+  // keeps it in one register pair for the whole tile (the fences below
+  // pass it through, so no station copies it).
   template <class Hook>
   __device__ __forceinline__ static void consume_station(const int (&sv)[KW * (P + 1)],
                                                          const unsigned char *stage,
-                                                         unsigned boff, int lane,
+                                                         unsigned boff,
+                                                         unsigned long long &lb2,
                                                          float (&acc)[KW][NACC], Hook hook) {
-    const unsigned lb = 16u * (unsigned)lane;
-    unsigned long long lb2 = ((unsigned long long)lb << 32) | lb;
+    const unsigned lb = (unsigned)lb2;
     const unsigned base = lds_address(stage) + boff;
     uint4 v[DEPTH + 1][R];
     unsigned addr[STEPS];
@@ -1090,8 +1093,9 @@ struct StackLds {
                                                       const unsigned char *__restrict__ stage,
                                                       unsigned boff,
                                                       unsigned char *__restrict__ nxt,
-                                                      int lane, float (&acc)[KW][NACC]) {
-    consume_station(sv, stage, boff, lane, acc, [&](int i) {
+                                                      unsigned long long &lb2,
+                                                      float (&acc)[KW][NACC]) {
+    consume_station(sv, stage, boff, lb2, acc, [&](int i) {
       bool here = false;  // i is a constant of the unrolled steps
 #pragma unroll
       for (int p = 0; p < P; ++p) here = here || i == dma_step(p);
@@ -1152,13 +1156,14 @@ struct StackLds {
     // schedule rows of this wave: station s at sched_w + s * KT * (P + 1)
     const_int *sched_w = as_const(a.sched) + ((size_t)tile * S * KT + wave * KW) * (P + 1);
     int sv[KW * (P + 1)];
+    unsigned long long lb2 = ((unsigned long long)(16u * lane) << 32) | (16u * lane);
     if (edge) {
       // synchronous: stage, barrier, consume, barrier
       for (int s = 0; s < S; ++s) {
         load_sched(sched_w + (size_t)s * KT * (P + 1), sv);
         stage_station_guarded(a, tlo, s, t0, tid, stage);
         __syncthreads();
-        consume_station(sv, stage, 0u, lane, acc, [](int) {});
+        consume_station(sv, stage, 0u, lb2, acc, [](int) {});
         __syncthreads();
       }
     } else {
@@ -1182,12 +1187,13 @@ struct StackLds {
       barrier_no_fence();
       load_sched(sched_w, sv);
       if (NB - 1 < S) station_src(a, tlo, tall0, NB - 1, t0, src);
-      int ci = 0;  // ring index of station s
+      int ci = 0;       // ring index of station s
+      int di = NB - 1;  // ring index of station s + NB - 1 (kept as its own scalar: derived
+                        // from ci, the select went through the VALU)
       for (int s = 0; s < S; ++s) {
         const int sd = s + NB - 1;
-        const int di = ci == 0 ? NB - 1 : ci - 1;  // ring index of station sd
         station_step(sd < S, src, rp, SB, wave, sv, stage, (unsigned)(ci * bbytes),
-                     stage + di * bbytes, lane, acc);
+                     stage + di * bbytes, lb2, acc);
         if (s + 1 < S) {
           // next station's operands and staging sources: the scalar loads
           // overlap the barrier
@@ -1201,6 +1207,7 @@ struct StackLds {
         else
           wait_vmem(rows * max(0, min(sd, S - 1) - (s + 1)));
         barrier_no_fence();
+        di = ci;
         ci = ci + 1 == NB ? 0 : ci + 1;
       }
     }
