@@ -5,8 +5,8 @@ windows of 140,000-150,001 samples, 33-40 MB of features, whose uploads
 go through the pinned staging buffers and the host worker pool, compared
 bit for bit with the oracle over the whole window (beamform.c:230-336):
 strict and flexible bounds, negative delays, moveouts up to 6,000 samples,
-a source tile that only the direct kernel takes, a short window just above
-16,384 samples, and the kept network (built, re-planned, reused).
+a source tile that only the direct kernel takes, 600 feature rows of a
+16,500-sample window, and the kept network (built, re-planned, reused).
 
 Bar: bit-identical maxima and argmax.
 """
