@@ -37,7 +37,7 @@ FLAGS_PATH = os.path.join(LIB_DIR, "build_flags.json")
 SHAPE_MACROS = ("BP_KW", "BP_W", "BP_WAVES_PER_EU", "BP_PIPE_DEPTH", "BP_STAGE_BUFFERS",
                 "BP_TIME_GROUP", "BP_NACC", "BP_SMALL_WINDOW", "BP_SMALL_SHARE_PCT",
                 "BP_GATHER_AT_F32_0", "BP_GATHER_AT_F32_1", "BP_GATHER_AT_16",
-                "BP_PIPE_DEPTH_16", "BP_DMA_AT", "BP_DMA_GAP")
+                "BP_PIPE_DEPTH_16", "BP_DMA_AT", "BP_DMA_GAP", "BP_PRIO", "BP_PRIO_PCT")
 
 
 def shape_flags(macros):
